@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""bench.py - the product hot path on one MI355X.
+
+Default workload: one 2^27 x 2^27-limb product (base-10^18 uint64 limbs drawn by splitmix64 from seeds 0x5EED0003 /
+0x5EED0004, top limb nonzero), transform length 2^28, three primes, operands resident in HBM.  A step is one full
+product c = a b through the C ABI (load, 9 transforms with the fused pointwise product, CRT with the device carry scan).
+Also measured in the same run: the batched binary-splitting leaves (4096 products, log-uniform lengths in
+[64, 65536] limbs) in one batched call against one call per product, and, on the host, GMP mpz_mul on a bounded sample
+of the same operands as the CPU baseline.
+
+Prints one JSON line (rank 0).  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B = 10 ** 18
+METRIC = "2^27-limb product secs + % HBM roofline; batched vs per-launch leaf mults/s"
+HBM_PEAK_GBS = 8000.0
+MI300A_REF_S = 0.1263          # the reference's single-device 2^27 x 2^27 product (results/D13b_strategy_e0.txt:26)
+G1, G2, G3 = np.uint64(0x9E3779B97F4A7C15), np.uint64(0xBF58476D1CE4E5B9), np.uint64(0x94D049BB133111EB)
+
+
+def splitmix(seed, count, offset=0):
+    """values offset+1 .. offset+count of the splitmix64 stream seeded with `seed` (vectorized)."""
+    i = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * G1
+        z = (z ^ (z >> np.uint64(30))) * G2
+        z = (z ^ (z >> np.uint64(27))) * G3
+    return z ^ (z >> np.uint64(31))
+
+
+def gen_limbs(n, seed):
+    """uniform base-10^18 limbs as the reference's operand generator makes them (E/tests/harness.h:55-81)."""
+    out = np.empty(n, dtype=np.uint64)
+    step = 1 << 24
+    for o in range(0, n, step):
+        k = min(step, n - o)
+        out[o:o + k] = splitmix(seed, k, o) % np.uint64(B)
+    if out[-1] == 0:
+        out[-1] = 1
+    return out
+
+
+def leaf_workload(count=4096, lo=64, hi=65536):
+    """lengths log-uniform in [lo, hi] from a stream seeded 0x5EED0005 (na, nb drawn independently), limbs from a stream
+    seeded 0x5EED0006 (one stream cut in order: a_0, b_0, a_1, ...)"""
+    u = (splitmix(0x5EED0005, 2 * count) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    lens = np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+    lens = np.clip(lens, lo, hi)
+    total = int(lens.sum())
+    limbs = splitmix(0x5EED0006, total) % np.uint64(B)
+    ops, off = [], 0
+    for j in range(count):
+        na, nb = int(lens[2 * j]), int(lens[2 * j + 1])
+        a = limbs[off:off + na].copy(); off += na
+        b = limbs[off:off + nb].copy(); off += nb
+        if a[-1] == 0:
+            a[-1] = 1
+        if b[-1] == 0:
+            b[-1] = 1
+        ops.append((a, b))
+    return ops
+
+
+def to_dev(a):
+    import torch
+    return torch.from_numpy(a.view(np.int64)).cuda()
+
+
+def host_info():
+    model = "?"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(a, b, log_hi, reps):
+    """GMP mpz_mul (1 thread) on the first 2^(log_hi-1) and 2^log_hi limbs of the same operands; extrapolated to the
+    full operand length with the measured growth per doubling."""
+    import oracle
+    if not oracle.have_gmp():
+        return None
+    n_full = len(a)
+    t = {}
+    for lg in (log_hi - 1, log_hi):
+        k = 1 << lg
+        t[lg] = oracle.gmp_time_mul(a[:k], b[:k], reps)
+    growth = t[log_hi] / t[log_hi - 1]
+    doublings = np.log2(n_full) - log_hi
+    est = t[log_hi] * growth ** doublings
+    model, ncpu = host_info()
+    return {"value": round(est, 4), "unit": "s per 2^%d x 2^%d-limb product" % (int(np.log2(n_full)), int(np.log2(n_full))),
+            "cores": 1, "kind": "port",
+            "sample": ("GMP mpz_mul (oracle GMP bridge), 1 thread, on the first 2^%d and 2^%d limbs of the same seeded "
+                       "operands, median of %d each, conversions off the clock: %.3f s and %.3f s; value extrapolated "
+                       "by the measured growth per doubling (x%.3f) over %.0f doublings; host %s, %s cpus"
+                       % (log_hi - 1, log_hi, reps, t[log_hi - 1], t[log_hi], growth, doublings, model, ncpu)),
+            "sample_s": {"2^%d" % k: round(v, 4) for k, v in t.items()}}
+
+
+def dominant(ks):
+    name = max(ks, key=lambda k: ks[k][1])
+    launches, ms, byts = ks[name]
+    return name, launches, ms, byts
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        return json.load(open(path)).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def bench_leaves(nttm, reps):
+    import torch
+    ops = leaf_workload()
+    items = []
+    for a, b in ops:
+        c = torch.empty(len(a) + len(b), dtype=torch.int64, device="cuda")
+        items.append((to_dev(a), to_dev(b), None, c))
+    prods = nttm.make_prods(items)
+    nttm.mul_batch(prods)                                    # warm (workspace, code)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nttm.mul_batch(prods)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    ref = [it[3].clone() for it in items[:64]]
+    t0 = time.perf_counter()
+    nttm.mul_each(prods)
+    torch.cuda.synchronize()
+    t_each = time.perf_counter() - t0
+    same = all(torch.equal(r, it[3]) for r, it in zip(ref, items[:64]))
+    return {"products": len(ops), "batched_products_per_s": round(len(ops) / t_batch, 1),
+            "per_launch_products_per_s": round(len(ops) / t_each, 1), "speedup": round(t_each / t_batch, 2),
+            "batched_ms": round(1e3 * t_batch, 3), "per_launch_ms": round(1e3 * t_each, 3),
+            "batch_equals_each_first64": bool(same)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-limbs", type=int, default=27, help="operand length 2^k limbs (default 27)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log", type=int, default=22)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-leaves", action="store_true")
+    ap.add_argument("--leaf-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the product's multi-GPU forms are not built (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ntt_amd as nttm
+
+    n = 1 << args.log_limbs
+    print("[bench] generating 2 x 2^%d limbs" % args.log_limbs, file=sys.stderr, flush=True)
+    a = gen_limbs(n, 0x5EED0003)
+    b = gen_limbs(n, 0x5EED0004)
+    nttm.init(0)
+    da, db = to_dev(a), to_dev(b)
+    dc = torch.empty(2 * n, dtype=torch.int64, device="cuda")
+    print("[bench] warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        nttm.mul(da, db, out=dc)
+    torch.cuda.synchronize()
+    first = dc.clone()
+
+    nttm.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nttm.mul(da, db, out=dc)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ks = nttm.profile_read()
+    nttm.profile(False)
+    phase = nttm.last_phase_ms()
+    deterministic = bool(torch.equal(first, dc))
+    del first
+    sec = (t1 - t0) / args.steps
+    print("[bench] %.3f ms per product" % (1e3 * sec), file=sys.stderr, flush=True)
+
+    ncoef = 2 * n - 1
+    logt = int(np.ceil(np.log2(ncoef)))
+    npts = 1 << logt
+    np_ = 3
+    b_alg = 8 * (2 * n) + 112 * np_ * npts + 8 * (2 * n)
+    name, launches, kms, kbytes = dominant(ks)
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(name), "kernel": name,
+            "launches": launches, "avg_launch_ms": round(kms / launches, 4),
+            "alg_bytes_per_launch": kbytes / launches}
+
+    leaves = None
+    if not args.no_leaves:
+        print("[bench] leaves", file=sys.stderr, flush=True)
+        del dc
+        torch.cuda.empty_cache()
+        leaves = bench_leaves(nttm, args.leaf_reps)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (GMP)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(a, b, min(args.cpu_sample_log, args.log_limbs), args.cpu_reps)
+
+    kernels = {k: {"launches": v[0], "ms": round(v[1] / args.steps, 4), "GB/s": round(v[2] / (v[1] * 1e-3) / 1e9, 1)}
+               for k, v in ks.items()}
+    line = {"metric": METRIC, "value": round(sec, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * sec, 4), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": round(sec / MI300A_REF_S, 4) if args.log_limbs == 27 else None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "2^%d x 2^%d-limb product, transform 2^%d, %d primes"
+                                   % (args.log_limbs, args.log_limbs, logt, np_),
+                       "na": n, "nb": n, "transform_points": npts, "primes": np_,
+                       "seeds": ["0x5EED0003", "0x5EED0004"], "limb_base": "10^18",
+                       "residue_arithmetic": "exact integers below 2^50 carried in FP64 (Shoup/Barrett), CRT in u64",
+                       "b_alg_bytes": b_alg, "product_hbm_GBs": round(b_alg / sec / 1e9, 1),
+                       "product_hbm_frac": round(b_alg / sec / 1e9 / HBM_PEAK_GBS, 4),
+                       "phase_ms": {"load_fwd": round(phase[0], 3), "pw_inv": round(phase[1], 3),
+                                    "crt": round(phase[2], 3)},
+                       "kernels_per_step": kernels, "deterministic": deterministic,
+                       "vs_baseline_ref": "time ratio to 0.1263 s, 1 MI300A (results/D13b_strategy_e0.txt:26)",
+                       "leaves": leaves},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    nttm.shutdown()
+
+
+if __name__ == "__main__":
+    main()

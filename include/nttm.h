@@ -1,0 +1,127 @@
+/* nttm.h - the C ABI of the MI355X multi-prime NTT product library (libnttm.so).
+ *
+ * Numbers are arrays of base-10^18 limbs (uint64_t, each < 10^18), least significant first, not necessarily
+ * normalized -- the reference's convention (E/bigint.h:12, E/rns_mul.h:48-51).  Every pointer an entry point takes
+ * for limbs, residues or products is DEVICE memory (hipMalloc) unless stated; the caller owns it.  Calls run
+ * asynchronously on the given HIP stream (a hipStream_t passed as void *, NULL = the null stream) and on the device
+ * selected by nttm_init.  One host thread per library instance; calls are not reentrant.
+ *
+ * Errors: every entry returns 0 on success or a negative NTTM_E* code, and nttm_last_error() describes the last
+ * failure.  The reference instead ends the process through ec_fatal (E/fatal.h:19-23); its limits are kept as
+ * errors here: transform length above 2^32 (E/ntt.c:517-520), a product above the prime set's term bound
+ * (E/crt.c:91-99), limbs >= 10^18 are the caller's contract (not checked on the device).
+ *
+ * Reference interfaces each entry replaces (paths relative to the reference repository, E/ = ecalc/):
+ *   nttm_init / nttm_shutdown     rns_init / rns_shutdown                         E/rns_mul.h:21,40
+ *   nttm_mul, nttm_mul_add        rns_mul_dist_db on one device = rns_local_core   E/rns_mul.h:54,76; E/rns_mul.c:1057
+ *   nttm_mul_batch                rns_mul_batch(rns_prod *, N)                     E/rns_mul.h:50-53; E/rns_mul.c:792
+ *   nttm_prod                     rns_prod                                         E/rns_mul.h:50-51
+ *   nttm_ntt_fwd / nttm_ntt_inv   ntt_fwd / ntt_inv                                E/ntt.h:45-46
+ *   nttm_pw                       ntt_pw                                           E/ntt.h:47
+ *   nttm_ntt_inv_pw               ntt_inv_pw_y (FULL / BCAST / PAIR)               E/ntt.h:52
+ *   nttm_load                     ntt_load                                         E/ntt.h:53
+ *   nttm_crt                      k_crt_batch + spill merge for one product        E/rns_int.h:50-52; E/rns_mul.c:336
+ *   nttm_max_terms                ec_np3_max_terms / ec_np_check                   E/modarith.h:40-43; E/crt.c:28-44
+ */
+#ifndef NTTM_H
+#define NTTM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NTTM_OK = 0,
+    NTTM_EINVAL = -1,      /* bad argument (length, pointer, mode) */
+    NTTM_ENOMEM = -2,      /* device allocation failed */
+    NTTM_EHIP = -3,        /* a HIP runtime call failed */
+    NTTM_EBOUND = -4,      /* the product exceeds what the prime set reconstructs, or the length exceeds 2^32 */
+    NTTM_ESTATE = -5,      /* not initialised */
+    NTTM_ECHECK = -6       /* a device-side post-condition failed (non-zero final carry, CRT look-back stall) */
+};
+
+#define NTTM_MAX_PRIMES 4
+
+/* y layouts of the fused inverse (E/ntt.h:31-35) */
+enum { NTTM_Y_FULL = 0, NTTM_Y_BCAST = 1, NTTM_Y_PAIR = 2 };
+
+typedef struct nttm_config {
+    int nprimes;                /* 0: choose per product (3 while the bound allows, else 4); 3 or 4: fixed */
+    size_t workspace_bytes;     /* 0: grow on demand; else the cap of the library's device workspace */
+} nttm_config;
+
+/* one product of a batch: c = a b (+ x).  c has na + nb limbs, or na + nb + 1 when x is given (nx <= na + nb),
+ * whose top limb is the final carry.  ncn receives the normalized length of c (the reference's k_norm). */
+typedef struct nttm_prod {
+    const uint64_t *a; size_t na;
+    const uint64_t *b; size_t nb;
+    uint64_t *c;
+    const uint64_t *x; size_t nx;
+    size_t ncn;
+} nttm_prod;
+
+int  nttm_init(int device, const nttm_config *cfg);
+void nttm_shutdown(void);
+const char *nttm_last_error(void);
+
+/* prime i of the set (0 <= i < NTTM_MAX_PRIMES), its 2-adic order, and the primitive 2^logn-th root the transforms
+ * use (g^((p-1)/2^logn) for the smallest generator g) */
+uint64_t nttm_prime(int i);
+uint64_t nttm_root(int i, int logn);
+/* largest min(na, nb) a product on np primes reconstructs exactly: floor((p_0 ... p_{np-1} - 1) / (10^18 - 1)^2) */
+uint64_t nttm_max_terms(int np);
+
+/* c[0 .. na+nb) = a b; device pointers.  na or nb == 0 gives c = 0. */
+int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream);
+/* c[0 .. na+nb+1) = a b + x, nx <= na + nb; the top limb is the final carry */
+int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                 const uint64_t *x, size_t nx, void *stream);
+/* N independent products; P is HOST memory holding device pointers.  Products are grouped by transform length and
+ * run batched; ncn is filled on return (the call synchronizes the stream). */
+int nttm_mul_batch(nttm_prod *P, size_t N, void *stream);
+/* the same N products, one product call each (the unbatched baseline of the batch tier) */
+int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
+
+/* ---- transform primitives on canonical residues in [0, p) (uint64_t), batch transforms of 2^logn contiguous ---- */
+/* natural order in -> bit-reversed out */
+int nttm_ntt_fwd(int prime, uint64_t *x, int logn, size_t batch, void *stream);
+/* bit-reversed in -> natural out, times n^-1 */
+int nttm_ntt_inv(int prime, uint64_t *x, int logn, size_t batch, void *stream);
+/* x[i] = x[i] y[i] mod p */
+int nttm_pw(int prime, uint64_t *x, const uint64_t *y, size_t count, void *stream);
+/* the pointwise product with y laid out per ymode, then the inverse: FULL y[t][k]; BCAST y[k]; PAIR y[t/2][k] */
+int nttm_ntt_inv_pw(int prime, uint64_t *x, const uint64_t *y, int ymode, int logn, size_t batch, void *stream);
+/* dst[0 .. npoints) = src limbs mod p, zero-extended (nlimbs <= npoints) */
+int nttm_load(int prime, uint64_t *dst, const uint64_t *src, size_t nlimbs, size_t npoints, void *stream);
+/* limbs of sum_k CRT(planes[0..np)[k]) 10^(18 k) + x: planes hold ncoef canonical residues each (ncoef <= nc); c gets
+ * nc limbs.  Returns NTTM_ECHECK when the value needs more than nc limbs. */
+int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, int np, size_t ncoef,
+             const uint64_t *x, size_t nx, void *stream);
+
+/* device-side time of the phases of the last nttm_mul / nttm_mul_add (ms): load+forward, pointwise+inverse, CRT */
+int nttm_last_phase_ms(double *fwd_ms, double *inv_ms, double *crt_ms);
+
+/* ---- live kernel timing (no reference counterpart; the reference times phases with events, E/rns_mul.c:548-587) ---- */
+typedef struct nttm_kstat {
+    char name[40];              /* kernel family */
+    uint64_t launches;
+    double total_ms;            /* summed device time between events recorded around each launch */
+    double alg_bytes;           /* summed algorithmic HBM bytes of those launches (the library's byte model) */
+} nttm_kstat;
+/* 1: from now on bracket every kernel launch with HIP events on its stream (clears earlier records); 0: stop */
+int nttm_profile(int enable);
+/* synchronizes the device, aggregates the records per kernel family into out[0 .. cap); returns the family count */
+int nttm_profile_read(nttm_kstat *out, int cap);
+
+/* ---- host-only introspection (no device calls; the reference has no counterpart) ---- */
+/* the pass plan of a 2^logn transform batch: 4 ints (lo, m, logw, logh) per pass into out[0 .. cap); returns passes */
+int nttm_debug_plan(int logn, uint64_t batch, int *out, int cap);
+/* the global element indices of tile `tile` of a pass of that geometry; returns the tile's element count */
+int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

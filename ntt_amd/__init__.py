@@ -1,0 +1,215 @@
+"""ntt_amd: Python binding of libnttm.so, the MI355X multi-prime NTT product of base-10^18 limb integers.
+
+The product itself is C/HIP behind the C ABI of ``include/nttm.h``; this module only loads the shared library and
+mirrors its entry points for tests and benchmarks, with device memory held in torch tensors (int64 storage of the
+uint64 limbs: every limb is below 10^18 < 2^63).  The names follow the reference's product interface
+(``rns_mul_batch`` over ``rns_prod`` records, the one-device product, the ``ntt_*`` primitives; E/rns_mul.h:50-56,
+E/ntt.h:37-66 of cecoppola/ntt).  Nothing here computes on the CPU: when the library is missing the calls raise.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libnttm.so")
+HEADER = os.path.join(ROOT, "include", "nttm.h")
+
+B = 10 ** 18
+Y_FULL, Y_BCAST, Y_PAIR = 0, 1, 2
+ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EBOUND", -5: "ESTATE", -6: "ECHECK"}
+
+
+class NttmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nttm error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+class Prod(C.Structure):
+    """nttm_prod: c = a b (+ x); ncn receives the normalized length of c."""
+    _fields_ = [("a", C.c_void_p), ("na", C.c_size_t), ("b", C.c_void_p), ("nb", C.c_size_t),
+                ("c", C.c_void_p), ("x", C.c_void_p), ("nx", C.c_size_t), ("ncn", C.c_size_t)]
+
+
+class KStat(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libnttm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libnttm.so is not built (run __graft_entry__.build() or python -m ntt_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, u64, i32 = C.c_void_p, C.c_size_t, C.c_uint64, C.c_int
+        L.nttm_init.argtypes = [i32, C.POINTER(Config)]; L.nttm_init.restype = i32
+        L.nttm_shutdown.argtypes = []; L.nttm_shutdown.restype = None
+        L.nttm_last_error.argtypes = []; L.nttm_last_error.restype = C.c_char_p
+        L.nttm_prime.argtypes = [i32]; L.nttm_prime.restype = u64
+        L.nttm_root.argtypes = [i32, i32]; L.nttm_root.restype = u64
+        L.nttm_max_terms.argtypes = [i32]; L.nttm_max_terms.restype = u64
+        L.nttm_mul.argtypes = [vp, vp, sz, vp, sz, vp]; L.nttm_mul.restype = i32
+        L.nttm_mul_add.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp]; L.nttm_mul_add.restype = i32
+        L.nttm_mul_batch.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_batch.restype = i32
+        L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
+        L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
+        L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
+        L.nttm_pw.argtypes = [i32, vp, vp, sz, vp]; L.nttm_pw.restype = i32
+        L.nttm_ntt_inv_pw.argtypes = [i32, vp, vp, i32, i32, sz, vp]; L.nttm_ntt_inv_pw.restype = i32
+        L.nttm_load.argtypes = [i32, vp, vp, sz, sz, vp]; L.nttm_load.restype = i32
+        L.nttm_crt.argtypes = [vp, sz, C.POINTER(vp), i32, sz, vp, sz, vp]; L.nttm_crt.restype = i32
+        L.nttm_last_phase_ms.argtypes = [C.POINTER(C.c_double)] * 3; L.nttm_last_phase_ms.restype = i32
+        L.nttm_profile.argtypes = [i32]; L.nttm_profile.restype = i32
+        L.nttm_profile_read.argtypes = [C.POINTER(KStat), i32]; L.nttm_profile_read.restype = i32
+        L.nttm_debug_plan.argtypes = [i32, u64, C.POINTER(C.c_int), i32]; L.nttm_debug_plan.restype = i32
+        L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
+        L.nttm_debug_tile_indices.restype = i32
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise NttmError(rc, lib().nttm_last_error().decode())
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+# ---- host-side queries (no device needed) ---------------------------------------------------------------------------
+def prime(i):
+    return int(lib().nttm_prime(i))
+
+
+def root(i, logn):
+    return int(lib().nttm_root(i, logn))
+
+
+def max_terms(np_):
+    return int(lib().nttm_max_terms(np_))
+
+
+def debug_plan(logn, batch=1):
+    out = (C.c_int * 24)()
+    n = lib().nttm_debug_plan(logn, batch, out, 24)
+    return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
+
+
+def debug_tile_indices(lo, m, logw, logh, tile):
+    buf = (C.c_uint64 * (1 << (m + logw + logh)))()
+    n = lib().nttm_debug_tile_indices(lo, m, logw, logh, tile, buf)
+    return list(buf[:n])
+
+
+# ---- device entry points ------------------------------------------------------------------------------------------------
+def init(device=0, nprimes=0, workspace_bytes=0):
+    cfg = Config(nprimes, workspace_bytes)
+    _check(lib().nttm_init(device, C.byref(cfg)))
+
+
+def shutdown():
+    lib().nttm_shutdown()
+
+
+def mul(a, b, out=None, stream=None):
+    """c = a b for int64 device tensors of limbs; returns c (na + nb limbs)."""
+    import torch
+    na, nb = a.numel(), b.numel()
+    c = out if out is not None else torch.empty(max(na + nb, 1), dtype=torch.int64, device=a.device)
+    _check(lib().nttm_mul(_ptr(c), _ptr(a), na, _ptr(b), nb, _stream(stream)))
+    return c[:na + nb]
+
+
+def mul_add(a, b, x, out=None, stream=None):
+    """c = a b + x (na + nb + 1 limbs, the top one the final carry)."""
+    import torch
+    na, nb = a.numel(), b.numel()
+    c = out if out is not None else torch.empty(na + nb + 1, dtype=torch.int64, device=a.device)
+    _check(lib().nttm_mul_add(_ptr(c), _ptr(a), na, _ptr(b), nb, _ptr(x), x.numel(), _stream(stream)))
+    return c[:na + nb + 1]
+
+
+def make_prods(items):
+    """items: list of (a, b, x_or_None, c) device tensors -> ctypes array of Prod."""
+    arr = (Prod * len(items))()
+    for i, (a, b, x, c) in enumerate(items):
+        arr[i] = Prod(a.data_ptr() if a.numel() else None, a.numel(), b.data_ptr() if b.numel() else None, b.numel(),
+                      c.data_ptr(), None if x is None else x.data_ptr(), 0 if x is None else x.numel(), 0)
+    return arr
+
+
+def mul_batch(prods, stream=None):
+    """rns_mul_batch analog over a ctypes Prod array (see make_prods); fills ncn."""
+    _check(lib().nttm_mul_batch(prods, len(prods), _stream(stream)))
+    return prods
+
+
+def mul_each(prods, stream=None):
+    _check(lib().nttm_mul_each(prods, len(prods), _stream(stream)))
+    return prods
+
+
+def ntt_fwd(prime_idx, x, logn, batch=1, stream=None):
+    _check(lib().nttm_ntt_fwd(prime_idx, _ptr(x), logn, batch, _stream(stream)))
+
+
+def ntt_inv(prime_idx, x, logn, batch=1, stream=None):
+    _check(lib().nttm_ntt_inv(prime_idx, _ptr(x), logn, batch, _stream(stream)))
+
+
+def ntt_inv_pw(prime_idx, x, y, ymode, logn, batch=1, stream=None):
+    _check(lib().nttm_ntt_inv_pw(prime_idx, _ptr(x), _ptr(y), ymode, logn, batch, _stream(stream)))
+
+
+def pw(prime_idx, x, y, stream=None):
+    _check(lib().nttm_pw(prime_idx, _ptr(x), _ptr(y), x.numel(), _stream(stream)))
+
+
+def load(prime_idx, dst, src, stream=None):
+    _check(lib().nttm_load(prime_idx, _ptr(dst), _ptr(src) if src.numel() else None, src.numel(), dst.numel(),
+                           _stream(stream)))
+
+
+def crt(planes, ncoef, nc, x=None, stream=None):
+    """limbs of sum_k CRT(planes[.][k]) B^k (+ x) for canonical residue planes; returns c (nc limbs)."""
+    import torch
+    arr = (C.c_void_p * len(planes))(*[p.data_ptr() for p in planes])
+    c = torch.empty(max(nc, 1), dtype=torch.int64, device=planes[0].device)
+    _check(lib().nttm_crt(_ptr(c), nc, arr, len(planes), ncoef, _ptr(x), 0 if x is None else x.numel(), _stream(stream)))
+    return c[:nc]
+
+
+def last_phase_ms():
+    f, i, c = C.c_double(), C.c_double(), C.c_double()
+    lib().nttm_last_phase_ms(C.byref(f), C.byref(i), C.byref(c))
+    return f.value, i.value, c.value
+
+
+def profile(enable=True):
+    """Bracket every kernel launch with HIP events (clears earlier records)."""
+    _check(lib().nttm_profile(1 if enable else 0))
+
+
+def profile_read():
+    """{kernel family: (launches, total_ms, alg_bytes)} of the launches recorded since profile(True)."""
+    arr = (KStat * 64)()
+    n = lib().nttm_profile_read(arr, 64)
+    if n < 0:
+        _check(n)
+    return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms), float(arr[i].alg_bytes)) for i in range(n)}

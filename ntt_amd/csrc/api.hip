@@ -1,0 +1,643 @@
+/* api.hip - the C ABI of libnttm.so: context and tables, the product schedule (one large product, batches of leaf
+ * products) and the transform primitives.  See include/nttm.h for the contract and the reference entry points each
+ * function replaces. */
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <stdlib.h>
+#include <vector>
+#include <algorithm>
+#include "internal.h"
+#include "plan.h"
+
+int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s);
+struct LoadArgs;
+int nttm_run_u64_to_f64(double *x, uint64_t count, hipStream_t s);
+int nttm_run_cvt_u64_f64(double *dst, const uint64_t *src, uint64_t count, hipStream_t s);
+int nttm_run_f64_to_canon(double *x, uint64_t count, const PrimeK &pk, hipStream_t s);
+int nttm_run_pw_u64(uint64_t *x, const uint64_t *y, uint64_t count, const PrimeK &pk, hipStream_t s);
+int nttm_run_load_planes(double *dst, uint64_t plane, const uint64_t *src, uint64_t nlimbs, uint64_t npoints, int np,
+                         const PrimeK *pk, const double2 *scale, int out_u64, hipStream_t s);
+int nttm_run_load_batch(double *dst, uint64_t plane, const void *desc, int which, uint64_t count, int logn, int np,
+                        const PrimeK *pk, const double2 *scale, hipStream_t s);
+int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks, int np, const CrtConst &cc,
+                          uint32_t *scratch, hipStream_t s, double alg_bytes);
+uint64_t nttm_crt_blocks(uint64_t nc);
+int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
+
+/* ---- errors ----------------------------------------------------------------------------------------------------- */
+static char g_err[512] = "";
+int nttm_fail(int code, const char *fmt, ...)
+{
+    va_list ap; va_start(ap, fmt); vsnprintf(g_err, sizeof g_err, fmt, ap); va_end(ap);
+    return code;
+}
+int nttm_hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    return nttm_fail(NTTM_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+#define HIPCK(call) do { int rc_ = nttm_hip_check((call), #call); if (rc_) return rc_; } while (0)
+#define RCK(call) do { int rc_ = (call); if (rc_) return rc_; } while (0)
+extern "C" const char *nttm_last_error(void) { return g_err; }
+
+/* ---- host modular arithmetic (exact, __int128) -------------------------------------------------------------------- */
+typedef unsigned __int128 u128;
+static uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t p) { return (uint64_t)((u128)a * b % p); }
+static uint64_t h_powmod(uint64_t a, uint64_t e, uint64_t p)
+{
+    uint64_t r = 1 % p; a %= p;
+    while (e) { if (e & 1) r = h_mulmod(r, a, p); a = h_mulmod(a, a, p); e >>= 1; }
+    return r;
+}
+static uint64_t h_inv(uint64_t a, uint64_t p) { return h_powmod(a, p - 2, p); }
+static uint64_t h_generator(uint64_t p)
+{
+    uint64_t f[64]; int nf = 0, t2 = 0; uint64_t t = p - 1;
+    while (!(t & 1)) { t >>= 1; t2 = 1; }
+    if (t2) f[nf++] = 2;
+    for (uint64_t q = 3; q * q <= t; q += 2) if (t % q == 0) { f[nf++] = q; while (t % q == 0) t /= q; }
+    if (t > 1) f[nf++] = t;
+    for (uint64_t g = 2; g < p; g++) {
+        int ok = 1;
+        for (int i = 0; i < nf && ok; i++) if (h_powmod(g, (p - 1) / f[i], p) == 1) ok = 0;
+        if (ok) return g;
+    }
+    return 0;
+}
+static double h_bal(uint64_t v, uint64_t p) { return v > p / 2 ? -(double)(p - v) : (double)v; }
+static double2 h_shoup(uint64_t v, uint64_t p) { double w = h_bal(v, p); return make_double2(w, w / (double)p); }
+
+/* ---- the context ------------------------------------------------------------------------------------------------------ */
+struct Ctx {
+    int init, dev;
+    nttm_config cfg;
+    PrimeK pk[NTTM_MAX_PRIMES];
+    uint64_t root32[NTTM_MAX_PRIMES];        /* root of order 2^32 */
+    TwTables twf[NTTM_MAX_PRIMES], twi[NTTM_MAX_PRIMES];
+    double2 *tables;
+    CrtConst cc;
+    char *ws; size_t ws_bytes;               /* device workspace */
+    uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
+    hipEvent_t ev[4];
+    double last_ms[3];
+};
+static Ctx g;
+
+static uint64_t max_terms_np(int np)
+{
+    /* floor((prod p - 1) / (B - 1)^2) by long division of the product (< 2^200) */
+    uint64_t M[4] = {1, 0, 0, 0};
+    for (int i = 0; i < np; i++) {
+        uint64_t carry = 0;
+        for (int w = 0; w < 4; w++) { u128 s = (u128)M[w] * NTTM_PRIMES[i] + carry; M[w] = (uint64_t)s; carry = (uint64_t)(s >> 64); }
+    }
+    for (int w = 0; w < 4; w++) { if (M[w]--) break; }
+    const u128 D = (u128)(NTTM_B - 1) * (NTTM_B - 1);
+    u128 rem = 0; uint64_t q[4] = {0, 0, 0, 0};
+    for (int bit = 255; bit >= 0; bit--) {
+        int top = (int)(rem >> 127);
+        rem = (rem << 1) | ((M[bit / 64] >> (bit % 64)) & 1);
+        if (top || rem >= D) { rem -= D; q[bit / 64] |= 1ULL << (bit % 64); }
+    }
+    if (q[1] || q[2] || q[3]) return UINT64_MAX;
+    return q[0];
+}
+
+extern "C" uint64_t nttm_prime(int i) { return (i >= 0 && i < NTTM_MAX_PRIMES) ? NTTM_PRIMES[i] : 0; }
+extern "C" uint64_t nttm_max_terms(int np) { return (np >= 1 && np <= NTTM_MAX_PRIMES) ? max_terms_np(np) : 0; }
+extern "C" uint64_t nttm_root(int i, int logn)
+{
+    if (i < 0 || i >= NTTM_MAX_PRIMES || logn < 0 || logn > NTTM_LOGN_MAX) return 0;
+    const uint64_t p = NTTM_PRIMES[i];
+    return h_powmod(h_generator(p), (p - 1) >> logn, p);
+}
+
+static void build_crt_const(CrtConst &cc)
+{
+    memset(&cc, 0, sizeof cc);
+    for (int i = 0; i < NTTM_MAX_PRIMES; i++) {
+        const uint64_t p = NTTM_PRIMES[i];
+        cc.pk[i].p = (double)p; cc.pk[i].pinv = 1.0 / (double)p; cc.pk[i].pu = p;
+        /* prefix product p_0 ... p_{i-1} in words, and its residue mod p_i */
+        uint64_t M[NTTM_MAX_PRIMES] = {1, 0, 0, 0};
+        for (int j = 0; j < i; j++) {
+            uint64_t carry = 0;
+            for (int w = 0; w < NTTM_MAX_PRIMES; w++) { u128 s = (u128)M[w] * NTTM_PRIMES[j] + carry; M[w] = (uint64_t)s; carry = (uint64_t)(s >> 64); }
+        }
+        for (int w = 0; w < NTTM_MAX_PRIMES; w++) cc.mprod[i][w] = M[w];
+        for (int j = 1; j <= i; j++) {
+            uint64_t m = 1;
+            for (int k = 0; k < j; k++) m = h_mulmod(m, NTTM_PRIMES[k] % p, p);
+            if (j < i) cc.mmod[i][j] = h_bal(m, p);
+            if (j == i) cc.cinv[i] = h_bal(h_inv(m, p), p);
+        }
+    }
+}
+
+static int build_tables(void)
+{
+    const size_t nst = (size_t)1 << NTTM_STAGE_LOG, nbt = (size_t)1 << NTTM_BT_LOG;
+    const size_t per_dir = nst + 2 * nbt, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
+    std::vector<double2> h(total);
+    for (int i = 0; i < NTTM_MAX_PRIMES; i++) {
+        const uint64_t p = NTTM_PRIMES[i];
+        const uint64_t gen = h_generator(p);
+        g.root32[i] = h_powmod(gen, (p - 1) >> 32, p);
+        for (int dir = 0; dir < 2; dir++) {
+            const uint64_t w32 = dir ? h_inv(g.root32[i], p) : g.root32[i];
+            double2 *st = &h[((size_t)i * 2 + dir) * per_dir], *bh = st + nst, *bl = bh + nbt;
+            st[0] = h_shoup(1, p);
+            for (int s = 0; s + 1 < NTTM_STAGE_LOG + 1 && ((size_t)1 << s) < nst; s++) {
+                const uint64_t ws = h_powmod(w32, (uint64_t)1 << (32 - (s + 1)), p);   /* order 2^(s+1) */
+                uint64_t v = 1;
+                for (size_t j = 0; j < ((size_t)1 << s); j++) { st[((size_t)1 << s) + j] = h_shoup(v, p); v = h_mulmod(v, ws, p); }
+            }
+            const uint64_t w16 = h_powmod(w32, (uint64_t)1 << NTTM_BT_LOG, p);             /* order 2^16 */
+            uint64_t vh = 1, vl = 1;
+            for (size_t j = 0; j < nbt; j++) {
+                bh[j] = h_shoup(vh, p); vh = h_mulmod(vh, w16, p);
+                bl[j] = h_shoup(vl, p); vl = h_mulmod(vl, w32, p);
+            }
+        }
+    }
+    HIPCK(hipMalloc(&g.tables, total * sizeof(double2)));
+    HIPCK(hipMemcpy(g.tables, h.data(), total * sizeof(double2), hipMemcpyHostToDevice));
+    for (int i = 0; i < NTTM_MAX_PRIMES; i++)
+        for (int dir = 0; dir < 2; dir++) {
+            const double2 *st = g.tables + ((size_t)i * 2 + dir) * per_dir;
+            TwTables t; t.stage = st; t.bh = st + nst; t.bl = st + nst + nbt;
+            if (dir) g.twi[i] = t; else g.twf[i] = t;
+        }
+    return 0;
+}
+
+static int ws_need(size_t bytes)
+{
+    if (g.cfg.workspace_bytes && bytes > g.cfg.workspace_bytes)
+        return nttm_fail(NTTM_ENOMEM, "workspace of %zu bytes exceeds the configured cap %zu", bytes, g.cfg.workspace_bytes);
+    if (bytes <= g.ws_bytes) return 0;
+    if (g.ws) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.ws)); g.ws = nullptr; g.ws_bytes = 0; }
+    if (hipMalloc(&g.ws, bytes) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "hipMalloc of %zu bytes failed", bytes); }
+    g.ws_bytes = bytes;
+    return 0;
+}
+static int hpin_need(size_t bytes)
+{
+    if (bytes <= g.hpin_bytes) return 0;
+    if (g.hpin) { HIPCK(hipDeviceSynchronize()); HIPCK(hipHostFree(g.hpin)); g.hpin = nullptr; g.hpin_bytes = 0; }
+    HIPCK(hipHostMalloc((void **)&g.hpin, bytes, hipHostMallocDefault));
+    g.hpin_bytes = bytes;
+    return 0;
+}
+
+extern "C" int nttm_init(int device, const nttm_config *cfg)
+{
+    if (g.init) return nttm_fail(NTTM_ESTATE, "nttm_init: already initialised");
+    memset(&g, 0, sizeof g);
+    if (cfg) g.cfg = *cfg;
+    if (g.cfg.nprimes && g.cfg.nprimes != 3 && g.cfg.nprimes != 4)
+        return nttm_fail(NTTM_EINVAL, "nttm_init: nprimes must be 0, 3 or 4");
+    int nd = 0;
+    HIPCK(hipGetDeviceCount(&nd));
+    if (device < 0 || device >= nd) return nttm_fail(NTTM_EINVAL, "nttm_init: device %d of %d", device, nd);
+    HIPCK(hipSetDevice(device));
+    g.dev = device;
+    for (int i = 0; i < NTTM_MAX_PRIMES; i++) { g.pk[i].p = (double)NTTM_PRIMES[i]; g.pk[i].pinv = 1.0 / (double)NTTM_PRIMES[i]; g.pk[i].pu = NTTM_PRIMES[i]; }
+    build_crt_const(g.cc);
+    RCK(build_tables());
+    for (int i = 0; i < 4; i++) HIPCK(hipEventCreate(&g.ev[i]));
+    RCK(hpin_need(1 << 16));
+    g.init = 1;
+    return 0;
+}
+
+static void prof_release(void);
+extern "C" void nttm_shutdown(void)
+{
+    if (!g.init) return;
+    (void)hipDeviceSynchronize();
+    prof_release();
+    if (g.ws) (void)hipFree(g.ws);
+    if (g.tables) (void)hipFree(g.tables);
+    if (g.hpin) (void)hipHostFree(g.hpin);
+    for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
+    memset(&g, 0, sizeof g);
+}
+
+/* ---- transforms on FP64 planes ------------------------------------------------------------------------------------------ */
+static int ceil_log2(uint64_t n) { int l = 0; while (((uint64_t)1 << l) < n) l++; return l; }
+
+static PassArgs base_args(int prime, double *x, int logn, uint64_t batch, bool inv)
+{
+    PassArgs a; memset(&a, 0, sizeof a);
+    a.x = x; a.y = nullptr; a.logn = logn; a.batch = batch; a.ymode = NTTM_Y_FULL;
+    a.pk = g.pk[prime]; a.tw = inv ? g.twi[prime] : g.twf[prime];
+    return a;
+}
+static int fwd_planes(int prime, double *x, int logn, uint64_t batch, hipStream_t s)
+{
+    Plan pl; plan_make(&pl, logn, batch, NTTM_TILE_LOG);
+    PassArgs a = base_args(prime, x, logn, batch, false);
+    for (int i = 0; i < pl.npass; i++) RCK(nttm_run_pass(false, a, pl.p[i], s));
+    return 0;
+}
+/* inverse with the pointwise product by y (ymode layout) fused into the first pass; scale by n^-1 in the last pass */
+static int inv_planes(int prime, double *x, const double *y, int ymode, int logn, uint64_t batch, int scale, hipStream_t s)
+{
+    Plan pl; plan_make(&pl, logn, batch, NTTM_TILE_LOG);
+    PassArgs a = base_args(prime, x, logn, batch, true);
+    for (int i = pl.npass - 1; i >= 0; i--) {
+        PassArgs b = a;
+        if (i == pl.npass - 1) { b.y = y; b.ymode = ymode; }
+        if (i == 0 && scale) {
+            const uint64_t p = NTTM_PRIMES[prime];
+            b.scale = h_shoup(h_inv(((uint64_t)1 << logn) % p, p), p); b.do_scale = 1;
+        }
+        RCK(nttm_run_pass(true, b, pl.p[i], s));
+    }
+    return 0;
+}
+
+/* ---- the product ------------------------------------------------------------------------------------------------------- */
+struct CrtDesc {                 /* must match crt.hip */
+    const double *pl; uint64_t plane;
+    uint64_t ncoef;
+    const uint64_t *x; uint64_t nx;
+    uint64_t *c; uint64_t nc;
+    uint64_t first_block;
+};
+struct LoadDesc {                /* must match ntt.hip */
+    const uint64_t *a; uint64_t na;
+    const uint64_t *b; uint64_t nb;
+};
+
+static int choose_np(size_t mn, int *np)
+{
+    int want = g.cfg.nprimes;
+    if (!want) want = (uint64_t)mn <= max_terms_np(3) ? 3 : 4;
+    if ((uint64_t)mn > max_terms_np(want))
+        return nttm_fail(NTTM_EBOUND, "a product of %zu terms exceeds the %d-prime bound of %llu terms", mn, want,
+                         (unsigned long long)max_terms_np(want));
+    *np = want;
+    return 0;
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+/* c = a b (+ x), device pointers; synchronous (returns after the stream has drained) */
+static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
+                    hipStream_t s)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    const size_t nc = na + nb + (x ? 1 : 0);
+    if (x && nx > na + nb) return nttm_fail(NTTM_EINVAL, "nx = %zu exceeds na + nb = %zu", nx, na + nb);
+    if (!c && nc) return nttm_fail(NTTM_EINVAL, "null result pointer");
+    if ((na && !a) || (nb && !b) || (nx && !x)) return nttm_fail(NTTM_EINVAL, "null operand pointer");
+    if (!na || !nb) {
+        if (nc) HIPCK(hipMemsetAsync(c, 0, nc * 8, s));
+        if (x && nx) HIPCK(hipMemcpyAsync(c, x, nx * 8, hipMemcpyDeviceToDevice, s));
+        HIPCK(hipStreamSynchronize(s));
+        return 0;
+    }
+    int np; RCK(choose_np(na < nb ? na : nb, &np));
+    const uint64_t ncoef = na + nb - 1;
+    int logn = ceil_log2(ncoef); if (logn < 1) logn = 1;
+    if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", logn, NTTM_LOGN_MAX);
+    const uint64_t n = (uint64_t)1 << logn;
+    const size_t planes_bytes = (size_t)2 * np * n * 8;
+    const size_t crt_words = nttm_crt_blocks(nc) + 2;
+    const size_t off_desc = align_up(planes_bytes, 256), off_crt = off_desc + 256;
+    RCK(ws_need(off_crt + crt_words * 4 + 256));
+    double *A = (double *)g.ws, *Bp = A + (size_t)np * n;
+    CrtDesc *ddesc = (CrtDesc *)(g.ws + off_desc);
+    uint32_t *scratch = (uint32_t *)(g.ws + off_crt);
+
+    double2 sc[NTTM_MAX_PRIMES];
+    for (int i = 0; i < np; i++) { const uint64_t p = NTTM_PRIMES[i]; sc[i] = h_shoup(h_inv(n % p, p), p); }
+    HIPCK(hipEventRecord(g.ev[0], s));
+    RCK(nttm_run_load_planes(A, n, a, na, n, np, g.pk, nullptr, 0, s));
+    RCK(nttm_run_load_planes(Bp, n, b, nb, n, np, g.pk, sc, 0, s));
+    for (int i = 0; i < np; i++) {
+        RCK(fwd_planes(i, A + (size_t)i * n, logn, 1, s));
+        RCK(fwd_planes(i, Bp + (size_t)i * n, logn, 1, s));
+    }
+    HIPCK(hipEventRecord(g.ev[1], s));
+    for (int i = 0; i < np; i++) RCK(inv_planes(i, A + (size_t)i * n, Bp + (size_t)i * n, NTTM_Y_FULL, logn, 1, 0, s));
+    HIPCK(hipEventRecord(g.ev[2], s));
+    CrtDesc hd; hd.pl = A; hd.plane = n; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
+    memcpy(g.hpin, &hd, sizeof hd);
+    HIPCK(hipMemcpyAsync(ddesc, g.hpin, sizeof hd, hipMemcpyHostToDevice, s));
+    RCK(nttm_crt_launch_batch(ddesc, 1, nttm_crt_blocks(nc), np, g.cc, scratch, s,
+                              8.0 * np * (double)ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? nx : 0)));
+    HIPCK(hipEventRecord(g.ev[3], s));
+    HIPCK(hipMemcpyAsync(g.hpin + 64, scratch, 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    float ms;
+    for (int k = 0; k < 3; k++) { HIPCK(hipEventElapsedTime(&ms, g.ev[k], g.ev[k + 1])); g.last_ms[k] = ms; }
+    const uint32_t flags = ((const uint32_t *)(g.hpin + 64))[1];
+    if (flags & 2) return nttm_fail(NTTM_ECHECK, "crt: carry look-back stalled");
+    if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond the top limb (wrong residues)");
+    return 0;
+}
+
+extern "C" int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream)
+{
+    return mul_core(c, a, na, b, nb, nullptr, 0, (hipStream_t)stream);
+}
+extern "C" int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
+                            size_t nx, void *stream)
+{
+    if (!x) return nttm_fail(NTTM_EINVAL, "nttm_mul_add: null x");
+    return mul_core(c, a, na, b, nb, x, nx, (hipStream_t)stream);
+}
+extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
+{
+    if (f) *f = g.last_ms[0];
+    if (i) *i = g.last_ms[1];
+    if (c) *c = g.last_ms[2];
+    return 0;
+}
+
+/* ---- batches ---------------------------------------------------------------------------------------------------------- */
+extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t i = 0; i < N; i++) {
+        RCK(mul_core(P[i].c, P[i].a, P[i].na, P[i].b, P[i].nb, P[i].x, P[i].nx, s));
+        const size_t nc = P[i].na + P[i].nb + (P[i].x ? 1 : 0);
+        P[i].ncn = nc;
+    }
+    /* normalized lengths: read back the top limbs */
+    for (size_t i = 0; i < N; i++) {
+        size_t n = P[i].ncn;
+        while (n) {
+            uint64_t v; HIPCK(hipMemcpy(&v, P[i].c + n - 1, 8, hipMemcpyDeviceToHost));
+            if (v) break;
+            n--;
+        }
+        P[i].ncn = n;
+    }
+    return 0;
+}
+
+extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    hipStream_t s = (hipStream_t)stream;
+    if (!N) return 0;
+    /* validate; zero-operand products are written directly */
+    size_t mn_max = 0;
+    std::vector<std::vector<size_t>> groups(NTTM_LOGN_MAX + 1);
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        if (q.x && q.nx > q.na + q.nb) return nttm_fail(NTTM_EINVAL, "product %zu: nx exceeds na + nb", i);
+        if (!q.c || (q.na && !q.a) || (q.nb && !q.b)) return nttm_fail(NTTM_EINVAL, "product %zu: null pointer", i);
+        if (!q.na || !q.nb) continue;
+        size_t mn = q.na < q.nb ? q.na : q.nb; if (mn > mn_max) mn_max = mn;
+        int logn = ceil_log2(q.na + q.nb - 1); if (logn < 1) logn = 1;
+        if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "product %zu: transform above 2^%d", i, NTTM_LOGN_MAX);
+        groups[logn].push_back(i);
+    }
+    int np; RCK(choose_np(mn_max ? mn_max : 1, &np));
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        if (q.na && q.nb) continue;
+        const size_t nc = q.na + q.nb + (q.x ? 1 : 0);
+        if (nc) HIPCK(hipMemsetAsync(q.c, 0, nc * 8, s));
+        if (q.x && q.nx) HIPCK(hipMemcpyAsync(q.c, q.x, q.nx * 8, hipMemcpyDeviceToDevice, s));
+    }
+    /* one group per transform length: batched load, transforms, CRT */
+    for (int logn = 1; logn <= NTTM_LOGN_MAX; logn++) {
+        const std::vector<size_t> &G = groups[logn];
+        if (G.empty()) continue;
+        const uint64_t n = (uint64_t)1 << logn, cnt = G.size();
+        uint64_t nblk = 0;
+        std::vector<CrtDesc> cd(cnt);
+        std::vector<LoadDesc> ld(cnt);
+        for (uint64_t j = 0; j < cnt; j++) {
+            const nttm_prod &q = P[G[j]];
+            ld[j].a = q.a; ld[j].na = q.na; ld[j].b = q.b; ld[j].nb = q.nb;
+            cd[j].plane = cnt * n; cd[j].ncoef = q.na + q.nb - 1; cd[j].x = q.x; cd[j].nx = q.x ? q.nx : 0;
+            cd[j].c = q.c; cd[j].nc = q.na + q.nb + (q.x ? 1 : 0); cd[j].first_block = nblk;
+            nblk += nttm_crt_blocks(cd[j].nc);
+        }
+        const size_t planes_bytes = (size_t)2 * np * cnt * n * 8;
+        const size_t off_cd = align_up(planes_bytes, 256);
+        const size_t off_ld = align_up(off_cd + cnt * sizeof(CrtDesc), 256);
+        const size_t off_ncn = align_up(off_ld + cnt * sizeof(LoadDesc), 256);
+        const size_t off_crt = align_up(off_ncn + cnt * 8, 256);
+        RCK(ws_need(off_crt + (nblk + 2) * 4 + 256));
+        RCK(hpin_need(cnt * (sizeof(CrtDesc) + sizeof(LoadDesc)) + 4096));
+        double *A = (double *)g.ws, *Bp = A + (size_t)np * cnt * n;
+        for (uint64_t j = 0; j < cnt; j++) cd[j].pl = A + j * n;
+        char *hp = (char *)g.hpin;
+        HIPCK(hipStreamSynchronize(s));                      /* the pinned staging is reused per group */
+        memcpy(hp, cd.data(), cnt * sizeof(CrtDesc));
+        memcpy(hp + cnt * sizeof(CrtDesc), ld.data(), cnt * sizeof(LoadDesc));
+        HIPCK(hipMemcpyAsync(g.ws + off_cd, hp, cnt * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + cnt * sizeof(CrtDesc), cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
+        double2 sc[NTTM_MAX_PRIMES];
+        for (int i = 0; i < np; i++) { const uint64_t p = NTTM_PRIMES[i]; sc[i] = h_shoup(h_inv(n % p, p), p); }
+        RCK(nttm_run_load_batch(A, cnt * n, g.ws + off_ld, 0, cnt, logn, np, g.pk, nullptr, s));
+        RCK(nttm_run_load_batch(Bp, cnt * n, g.ws + off_ld, 1, cnt, logn, np, g.pk, sc, s));
+        for (int i = 0; i < np; i++) {
+            RCK(fwd_planes(i, A + (size_t)i * cnt * n, logn, cnt, s));
+            RCK(fwd_planes(i, Bp + (size_t)i * cnt * n, logn, cnt, s));
+            RCK(inv_planes(i, A + (size_t)i * cnt * n, Bp + (size_t)i * cnt * n, NTTM_Y_FULL, logn, cnt, 0, s));
+        }
+        double crt_bytes = 0;
+        for (uint64_t j = 0; j < cnt; j++) crt_bytes += 8.0 * np * cd[j].ncoef + 8.0 * cd[j].nc + 8.0 * cd[j].nx;
+        RCK(nttm_crt_launch_batch(g.ws + off_cd, cnt, nblk, np, g.cc, (uint32_t *)(g.ws + off_crt), s, crt_bytes));
+        RCK(nttm_run_norm(g.ws + off_cd, cnt, (uint64_t *)(g.ws + off_ncn), s));
+        HIPCK(hipMemcpyAsync(hp, g.ws + off_ncn, cnt * 8, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(hp + cnt * 8, g.ws + off_crt, 8, hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        const uint32_t flags = ((const uint32_t *)(hp + cnt * 8))[1];
+        if (flags & 2) return nttm_fail(NTTM_ECHECK, "crt: carry look-back stalled");
+        if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
+        for (uint64_t j = 0; j < cnt; j++) P[G[j]].ncn = ((const uint64_t *)hp)[j];
+    }
+    /* zero-operand products: normalized length of x */
+    for (size_t i = 0; i < N; i++) {
+        nttm_prod &q = P[i];
+        if (q.na && q.nb) continue;
+        size_t n = q.x ? q.nx : 0;
+        HIPCK(hipStreamSynchronize(s));
+        while (n) { uint64_t v; HIPCK(hipMemcpy(&v, q.x + n - 1, 8, hipMemcpyDeviceToHost)); if (v) break; n--; }
+        q.ncn = n;
+    }
+    return 0;
+}
+
+/* ---- transform primitives on canonical uint64 residues ---------------------------------------------------------------- */
+static int prim_check(int prime, const void *x, int logn, size_t batch)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (prime < 0 || prime >= NTTM_MAX_PRIMES) return nttm_fail(NTTM_EINVAL, "prime %d", prime);
+    if (logn < 1 || logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EINVAL, "logn %d outside [1, %d]", logn, NTTM_LOGN_MAX);
+    if (!x || !batch) return nttm_fail(NTTM_EINVAL, "empty transform");
+    return 0;
+}
+extern "C" int nttm_ntt_fwd(int prime, uint64_t *x, int logn, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logn, batch));
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t cnt = (uint64_t)batch << logn;
+    RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
+    RCK(fwd_planes(prime, (double *)x, logn, batch, s));
+    return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+extern "C" int nttm_ntt_inv(int prime, uint64_t *x, int logn, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logn, batch));
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t cnt = (uint64_t)batch << logn;
+    RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
+    RCK(inv_planes(prime, (double *)x, nullptr, NTTM_Y_FULL, logn, batch, 1, s));
+    return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+extern "C" int nttm_ntt_inv_pw(int prime, uint64_t *x, const uint64_t *y, int ymode, int logn, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logn, batch));
+    if (!y) return nttm_fail(NTTM_EINVAL, "null y");
+    if (ymode < NTTM_Y_FULL || ymode > NTTM_Y_PAIR) return nttm_fail(NTTM_EINVAL, "ymode %d", ymode);
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t cnt = (uint64_t)batch << logn;
+    const uint64_t ycnt = ymode == NTTM_Y_FULL ? cnt : ymode == NTTM_Y_BCAST ? ((uint64_t)1 << logn)
+                                                     : ((uint64_t)((batch + 1) / 2) << logn);
+    RCK(ws_need(ycnt * 8));
+    RCK(nttm_run_cvt_u64_f64((double *)g.ws, y, ycnt, s));
+    RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
+    RCK(inv_planes(prime, (double *)x, (const double *)g.ws, ymode, logn, batch, 1, s));
+    return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+extern "C" int nttm_pw(int prime, uint64_t *x, const uint64_t *y, size_t count, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (prime < 0 || prime >= NTTM_MAX_PRIMES || !x || !y) return nttm_fail(NTTM_EINVAL, "nttm_pw arguments");
+    if (!count) return 0;
+    return nttm_run_pw_u64(x, y, count, g.pk[prime], (hipStream_t)stream);
+}
+extern "C" int nttm_load(int prime, uint64_t *dst, const uint64_t *src, size_t nlimbs, size_t npoints, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (prime < 0 || prime >= NTTM_MAX_PRIMES || !dst || (nlimbs && !src) || nlimbs > npoints)
+        return nttm_fail(NTTM_EINVAL, "nttm_load arguments");
+    if (!npoints) return 0;
+    return nttm_run_load_planes((double *)dst, npoints, src, nlimbs, npoints, 1, &g.pk[prime], nullptr, 1, (hipStream_t)stream);
+}
+extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, int np, size_t ncoef, const uint64_t *x,
+                        size_t nx, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (np < 3 || np > NTTM_MAX_PRIMES || !planes || (nc && !c) || (nx && !x)) return nttm_fail(NTTM_EINVAL, "nttm_crt arguments");
+    if (ncoef > nc) return nttm_fail(NTTM_EINVAL, "nttm_crt: ncoef %zu exceeds nc %zu", ncoef, nc);
+    if (!nc) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t pbytes = align_up((size_t)np * ncoef * 8, 256);
+    const size_t crt_words = nttm_crt_blocks(nc) + 2;
+    RCK(ws_need(pbytes + 256 + crt_words * 4 + 256));
+    double *pl = (double *)g.ws;
+    for (int i = 0; i < np; i++) {
+        if (!planes[i] && ncoef) return nttm_fail(NTTM_EINVAL, "null plane %d", i);
+        if (ncoef) RCK(nttm_run_cvt_u64_f64(pl + (size_t)i * ncoef, planes[i], ncoef, s));
+    }
+    CrtDesc hd; hd.pl = pl; hd.plane = ncoef; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
+    HIPCK(hipStreamSynchronize(s));
+    memcpy(g.hpin, &hd, sizeof hd);
+    CrtDesc *dd = (CrtDesc *)(g.ws + pbytes);
+    uint32_t *scratch = (uint32_t *)(g.ws + pbytes + 256);
+    HIPCK(hipMemcpyAsync(dd, g.hpin, sizeof hd, hipMemcpyHostToDevice, s));
+    RCK(nttm_crt_launch_batch(dd, 1, nttm_crt_blocks(nc), np, g.cc, scratch, s,
+                              8.0 * np * (double)ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? nx : 0)));
+    HIPCK(hipMemcpyAsync(g.hpin + 64, scratch, 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    const uint32_t flags = ((const uint32_t *)(g.hpin + 64))[1];
+    if (flags & 2) return nttm_fail(NTTM_ECHECK, "crt: carry look-back stalled");
+    if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: the value needs more than nc limbs");
+    return 0;
+}
+
+/* ---- live kernel timing ------------------------------------------------------------------------------------------------ */
+struct ProfRec { const char *name; double bytes; hipEvent_t a, b; };
+static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_evpool;
+static int g_prof_on;
+static hipEvent_t ev_get(void)
+{
+    if (!g_evpool.empty()) { hipEvent_t e = g_evpool.back(); g_evpool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    return e;
+}
+void prof_begin(const char *name, double alg_bytes, hipStream_t s)
+{
+    if (!g_prof_on) return;
+    ProfRec r; r.name = name; r.bytes = alg_bytes; r.a = ev_get(); r.b = ev_get();
+    if (!r.a || !r.b) return;
+    (void)hipEventRecord(r.a, s);
+    g_prof.push_back(r);
+}
+void prof_end(hipStream_t s)
+{
+    if (!g_prof_on || g_prof.empty()) return;
+    (void)hipEventRecord(g_prof.back().b, s);
+}
+static void prof_clear(void)
+{
+    for (auto &r : g_prof) { g_evpool.push_back(r.a); g_evpool.push_back(r.b); }
+    g_prof.clear();
+}
+static void prof_release(void)
+{
+    prof_clear();
+    for (hipEvent_t e : g_evpool) (void)hipEventDestroy(e);
+    g_evpool.clear();
+    g_prof_on = 0;
+}
+extern "C" int nttm_profile(int enable)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    HIPCK(hipDeviceSynchronize());
+    prof_clear();
+    g_prof_on = enable != 0;
+    return 0;
+}
+extern "C" int nttm_profile_read(nttm_kstat *out, int cap)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    HIPCK(hipDeviceSynchronize());
+    int n = 0;
+    for (auto &r : g_prof) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) { (void)hipGetLastError(); continue; }
+        int k = 0;
+        while (k < n && strcmp(out[k].name, r.name)) k++;
+        if (k == n) {
+            if (n >= cap) continue;
+            memset(&out[n], 0, sizeof out[n]);
+            snprintf(out[n].name, sizeof out[n].name, "%s", r.name);
+            n++;
+        }
+        out[k].launches++; out[k].total_ms += ms; out[k].alg_bytes += r.bytes;
+    }
+    return n;
+}
+
+/* ---- host-side introspection for tests: the pass plan and the tile -> element map ---------------------------------- */
+extern "C" int nttm_debug_plan(int logn, uint64_t batch, int *out, int cap)
+{
+    Plan pl; plan_make(&pl, logn, batch, NTTM_TILE_LOG);
+    int k = 0;
+    for (int i = 0; i < pl.npass && k + 4 <= cap; i++) { out[k++] = pl.p[i].lo; out[k++] = pl.p[i].m; out[k++] = pl.p[i].logw; out[k++] = pl.p[i].logh; }
+    return pl.npass;
+}
+extern "C" int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out)
+{
+    PassGeom gg; gg.lo = lo; gg.m = m; gg.logw = logw; gg.logh = logh;
+    const int T = 1 << (m + logw + logh);
+    for (int e = 0; e < T; e++) out[e] = plan_gidx(gg, tile, e);
+    return T;
+}

@@ -1,0 +1,54 @@
+/* internal.h - shared declarations of the product library's translation units (not part of the C ABI). */
+#ifndef NTTM_INTERNAL_H
+#define NTTM_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "arith.h"
+#include "../../include/nttm.h"
+
+#define NTTM_TILE_LOG 13            /* points per transform tile: 2^13 FP64 values = 64 KiB of LDS */
+#define NTTM_STAGE_LOG 14           /* stage-major twiddle table: T[2^s + j] = w_{2^(s+1)}^j for s < 14 */
+#define NTTM_BT_LOG 16              /* boundary twiddles: w_{2^32}^E = H[E >> 16] * L[E & 0xffff] */
+#define NTTM_THREADS 256
+
+struct PrimeK {                     /* per-prime constants, passed by value */
+    double p, pinv;
+    uint64_t pu;
+};
+
+struct TwTables {                   /* device pointers, one set per prime and direction */
+    const double2 *stage;           /* (w, wq) balanced, 2^NTTM_STAGE_LOG entries */
+    const double2 *bh, *bl;         /* boundary tables, 2^NTTM_BT_LOG entries each */
+};
+
+/* one pass over bits [lo, lo + m) of every transform of 2^logn points in a batch; a tile holds W low columns x 2^m
+ * rows x H high blocks (log2 of each below) */
+struct PassArgs {
+    double *x;
+    const double *y;                /* pointwise operand (inverse first pass), else null */
+    int logn, lo, m, logw, logh;
+    int ymode;
+    uint64_t batch;
+    PrimeK pk;
+    TwTables tw;
+    double2 scale;                  /* (w, wq) of a final multiply (n^-1), used when do_scale */
+    int do_scale;
+};
+
+struct Ctx;
+/* live kernel timing: when profiling is on, launches are bracketed by events (prof_begin before, prof_end after) */
+void prof_begin(const char *name, double alg_bytes, hipStream_t s);
+void prof_end(hipStream_t s);
+/* host helpers shared between files */
+int nttm_fail(int code, const char *fmt, ...);
+int nttm_hip_check(hipError_t e, const char *what);
+
+/* the CRT (crt.hip): planes hold np FP64 residue planes (values in [-p, p]); writes nc limbs of c */
+struct CrtConst {
+    PrimeK pk[NTTM_MAX_PRIMES];
+    double cinv[NTTM_MAX_PRIMES];               /* (p_0 ... p_{i-1})^-1 mod p_i, balanced */
+    double mmod[NTTM_MAX_PRIMES][NTTM_MAX_PRIMES];   /* (p_0 ... p_{j-1}) mod p_i, balanced (j < i) */
+    uint64_t mprod[NTTM_MAX_PRIMES][NTTM_MAX_PRIMES];/* p_0 ... p_{i-1} as little-endian words */
+};
+#endif

@@ -1,0 +1,107 @@
+"""The C ABI library on the CPU (no GPU calls): it loads, exports every entry point include/nttm.h declares, its host
+queries agree with the oracle and with the mathematics, and the pass planner's tiles cover every element of every
+batch exactly once inside the buffer (the kernels' own index map, evaluated on the host before any GPU run)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import ntt_amd
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "nttm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nttm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ntt_amd.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 18
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_primes():
+    L = O.lib()
+    ps = [ntt_amd.prime(i) for i in range(4)]
+    assert len(set(ps)) == 4
+    for p in ps:
+        assert L.orc_is_prime(p) and p < 2 ** 50 and (p - 1) % (1 << 32) == 0 and ((p - 1) >> 32) % 3 == 0
+    assert ntt_amd.prime(4) == 0 and ntt_amd.prime(-1) == 0
+
+
+@pytest.mark.parametrize("i", range(4))
+def test_roots_exact_order_and_chain(i):
+    L = O.lib()
+    p = ntt_amd.prime(i)
+    prev = None
+    for k in range(0, 33):
+        w = ntt_amd.root(i, k)
+        assert w == L.orc_root2k(p, k)
+        assert L.orc_powmod(w, 1 << k, p) == 1
+        if k:
+            assert L.orc_powmod(w, 1 << (k - 1), p) == p - 1
+        if prev is not None:
+            assert L.orc_mulmod(w, w, p) == prev
+        prev = w
+    assert ntt_amd.root(i, 33) == 0
+
+
+def test_max_terms():
+    ps = [ntt_amd.prime(i) for i in range(4)]
+    for np_ in (3, 4):
+        t = ntt_amd.max_terms(np_)
+        assert t == O.max_terms(ps[:np_])
+        M = 1
+        for p in ps[:np_]:
+            M *= p
+        D = (10 ** 18 - 1) ** 2
+        if (M - 1) // D >= 2 ** 64:
+            assert t == 2 ** 64 - 1                               # saturated: the bound exceeds a word
+        else:
+            assert t * D < M <= (t + 1) * D
+    assert ntt_amd.max_terms(3) >= 1 << 30                      # a 2^30-term product fits three primes
+
+
+def check_plan(logn, batch, full):
+    passes = ntt_amd.debug_plan(logn, batch)
+    assert sum(m for _, m, _, _ in passes) == logn
+    top = logn
+    total = batch << logn
+    for lo, m, logw, logh in passes:                              # contiguous bit ranges, high to low
+        assert lo + m == top and m >= 1 and logw <= lo and m + logw + logh <= 13
+        if lo > 0 and logn >= 17:
+            assert logw >= 4                                      # strided passes read >= 128-byte runs
+        top = lo
+        T = 1 << (m + logw + logh)
+        assert total % T == 0
+        tiles = total // T
+        if full:
+            seen = np.zeros(total, dtype=np.int8)
+            for t in range(tiles):
+                idx = np.array(ntt_amd.debug_tile_indices(lo, m, logw, logh, t), dtype=np.uint64)
+                assert idx.max() < total
+                seen[idx] += 1
+            assert (seen == 1).all()
+        else:
+            for t in (0, 1, tiles // 2, tiles - 1):
+                idx = ntt_amd.debug_tile_indices(lo, m, logw, logh, t)
+                assert max(idx) < total and len(set(idx)) == T
+    assert top == 0
+
+
+@pytest.mark.parametrize("logn", list(range(1, 19)))
+@pytest.mark.parametrize("batch", [1, 2, 3, 8])
+def test_plan_covers_every_element_once(logn, batch):
+    check_plan(logn, batch, full=(batch << logn) <= (1 << 16))
+
+
+@pytest.mark.parametrize("logn", [20, 21, 24, 27, 28, 30, 32])
+def test_plan_large_bounds(logn):
+    check_plan(logn, 1, full=False)

@@ -1,0 +1,197 @@
+"""GPU parity of the product (nttm_mul, nttm_mul_add, nttm_mul_batch, nttm_mul_each) against the oracle: the schoolbook
+and GMP exactly, closed forms, edge sizes and operand kinds, batch == one-by-one, and the two headline shapes (2^20 and
+2^27 limbs per operand).  Mirrors the reference's t_mul sections 1, 2, 4 and 4b (E/tests/t_mul.c:116-328)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import dev, host, say
+
+pytestmark = pytest.mark.gpu
+B = O.B
+
+
+def to_int(limbs):
+    v = 0
+    for x in reversed([int(t) for t in limbs]):
+        v = v * B + x
+    return v
+
+
+def operands(na, nb, seed, kind=O.GEN_UNIFORM):
+    a, s = O.gen_limbs(na, seed, kind)
+    b, _ = O.gen_limbs(nb, s, kind)
+    return a, b
+
+
+EDGE = [(1, 1), (1, 2), (2, 1), (3, 5), (17, 1), (1, 17), (64, 64), (511, 512), (512, 511), (384 * 5, 5), (1000, 999),
+        (2048, 2048), (4097, 4095), (8192, 1)]
+
+
+@pytest.mark.parametrize("na,nb", EDGE)
+def test_mul_edges_vs_school(nttm, na, nb):
+    a, b = operands(na, nb, 1000 * na + nb)
+    say("mul", na, nb)
+    c = host(nttm.mul(dev(a), dev(b)))
+    assert np.array_equal(c, O.mul_school(a, b))
+
+
+@pytest.mark.parametrize("kind", range(5))
+def test_mul_kinds(nttm, kind):
+    a, b = operands(3000, 2999, 77 + kind, kind)
+    say("mul kind", kind)
+    c = host(nttm.mul(dev(a), dev(b)))
+    assert np.array_equal(c, O.mul_school(a, b))
+
+
+@pytest.mark.skipif(not O.have_gmp(), reason="GMP absent")
+@pytest.mark.parametrize("na,nb", [(1 << 14, 1 << 14), ((1 << 16) + 1, (1 << 16) - 1), (1 << 18, 3), (123457, 65537),
+                                   ((1 << 19) - 7, 1 << 19)])
+def test_mul_vs_gmp(nttm, na, nb):
+    a, b = operands(na, nb, na ^ nb)
+    say("mul gmp", na, nb)
+    c = host(nttm.mul(dev(a), dev(b)))
+    assert len(c) == na + nb and (c < B).all()
+    assert O.gmp_check(a, b, c)
+
+
+def test_mul_closed_forms(nttm):
+    n = 1 << 17
+    ones = np.full(n, B - 1, dtype=np.uint64)
+    say("all ones", n)
+    c = host(nttm.mul(dev(ones), dev(ones)))                     # (B^n - 1)^2 = B^2n - 2 B^n + 1
+    assert int(c[0]) == 1 and not c[1:n].any() and int(c[n]) == B - 2 and (c[n + 1:] == B - 1).all()
+    a, _ = O.gen_limbs(5000, 3)
+    say("times one / zero-padded powers")
+    c = host(nttm.mul(dev(a), dev(np.array([1], dtype=np.uint64))))
+    assert np.array_equal(c[:5000], a) and int(c[5000]) == 0
+    pa = np.zeros(7, dtype=np.uint64); pa[6] = 1
+    pb = np.zeros(4, dtype=np.uint64); pb[3] = 1
+    c = host(nttm.mul(dev(pa), dev(pb)))                         # B^6 B^3 = B^9
+    assert int(c[9]) == 1 and c.sum() == 1
+
+
+def test_mul_add(nttm):
+    for na, nb, nx in ((1, 1, 1), (100, 37, 137), (2049, 2048, 10), (5000, 4000, 9000)):
+        a, b = operands(na, nb, na + 3 * nb)
+        x, _ = O.gen_limbs(nx, nx)
+        say("mul_add", na, nb, nx)
+        c = host(nttm.mul_add(dev(a), dev(b), dev(x)))
+        assert len(c) == na + nb + 1
+        assert np.array_equal(c, O.mul_school(a, b, x))
+    ones = np.full(3000, B - 1, dtype=np.uint64)                 # the final carry limb: (B^n - 1)^2 + (B^2n - 1)
+    x = np.full(6000, B - 1, dtype=np.uint64)
+    c = host(nttm.mul_add(dev(ones), dev(ones), dev(x)))
+    assert to_int(c) == (B ** 3000 - 1) ** 2 + B ** 6000 - 1 and int(c[6000]) == 1
+
+
+def test_zero_operands(nttm):
+    import torch
+    a, _ = O.gen_limbs(10, 1)
+    empty = torch.empty(0, dtype=torch.int64, device="cuda")
+    say("zero operands")
+    c = host(nttm.mul(dev(a), empty))
+    assert len(c) == 10 and not c.any()
+    x, _ = O.gen_limbs(5, 2)
+    c = host(nttm.mul_add(empty, dev(a), dev(x)))
+    assert np.array_equal(c[:5], x) and not c[5:].any()
+
+
+def make_batch(nprod, seed, max_len=5000, with_x=True, shared_b=True):
+    rng = np.random.default_rng(seed)
+    items, host_ops = [], []
+    s = seed
+    for i in range(nprod):
+        na = int(np.exp(rng.uniform(0, np.log(max_len))))
+        nb = int(np.exp(rng.uniform(0, np.log(max_len))))
+        a, s = O.gen_limbs(na, s)
+        b, s = O.gen_limbs(nb, s)
+        if shared_b and i % 2 == 1:                           # the tree's P1 Q2 + P2 and Q1 Q2 share Q2
+            b = host_ops[-1][1]
+        x = None
+        if with_x and i % 3 == 0:
+            x, s = O.gen_limbs(int(rng.integers(1, na + nb + 1)), s)
+        host_ops.append((a, b, x))
+    return host_ops
+
+
+def run_batch(nttm, host_ops, each=False):
+    import torch
+    items = []
+    for a, b, x in host_ops:
+        nc = len(a) + len(b) + (1 if x is not None else 0)
+        c = torch.full((max(nc, 1),), -1, dtype=torch.int64, device="cuda")
+        items.append((dev(a), dev(b), None if x is None else dev(x), c))
+    prods = nttm.make_prods(items)
+    (nttm.mul_each if each else nttm.mul_batch)(prods)
+    out = []
+    for (a, b, x), it, pr in zip(host_ops, items, prods):
+        nc = len(a) + len(b) + (1 if x is not None else 0)
+        out.append((host(it[3])[:nc], pr.ncn))
+    return out
+
+
+def test_batch_equals_each_equals_oracle(nttm):
+    ops = make_batch(96, 11)
+    say("batch 96")
+    got_b = run_batch(nttm, ops)
+    say("each 96")
+    got_e = run_batch(nttm, ops, each=True)
+    for (a, b, x), (cb, nb_), (ce, ne_) in zip(ops, got_b, got_e):
+        want = O.mul_school(a, b, x) if len(a) * len(b) <= 4_000_000 else O.mul_ntt(a, b, [nttm.prime(i) for i in range(3)], x)
+        assert np.array_equal(cb, want) and np.array_equal(ce, want)
+        nz = np.nonzero(want)[0]
+        assert nb_ == ne_ == (int(nz[-1]) + 1 if len(nz) else 0)
+
+
+def test_batch_leaf_shapes(nttm):
+    """4096 leaf products with log-uniform lengths in [64, 65536] limbs, unequal pairs (the batched-leaf workload),
+    every product checked: GMP exactly where available, else the residue form."""
+    rng = np.random.default_rng(0x5EED0005)
+    ops = []
+    s = 0x5EED0006
+    for i in range(4096):
+        na = int(np.exp(rng.uniform(np.log(64), np.log(65536))))
+        nb = int(np.exp(rng.uniform(np.log(64), np.log(65536))))
+        a, s = O.gen_limbs(na, s)
+        b, s = O.gen_limbs(nb, s)
+        ops.append((a, b, None))
+    say("batch 4096 leaves")
+    got = run_batch(nttm, ops)
+    bad = 0
+    for (a, b, _), (c, ncn) in zip(ops, got):
+        ok = O.tref_res(a, b, c)
+        if ok and O.have_gmp() and len(a) + len(b) <= 20000:
+            ok = O.gmp_check(a, b, c)
+        bad += not ok
+        assert ncn in (len(a) + len(b), len(a) + len(b) - 1)
+    assert bad == 0
+
+
+@pytest.mark.skipif(not O.have_gmp(), reason="GMP absent")
+def test_plain_product_2e20_vs_gmp(nttm):
+    """2^20 x 2^20 limbs, seeds 0x5EED0001 / 0x5EED0002, transform length 2^21, exact against GMP."""
+    a, _ = O.gen_limbs(1 << 20, 0x5EED0001)
+    b, _ = O.gen_limbs(1 << 20, 0x5EED0002)
+    say("2^20 x 2^20")
+    c = host(nttm.mul(dev(a), dev(b)))
+    assert O.gmp_check(a, b, c)
+
+
+def test_large_product_2e27_residue_form(nttm):
+    """2^27 x 2^27 limbs (transform 2^28), seeds 0x5EED0003 / 0x5EED0004: limbs < B, length, residues mod three
+    62-bit primes and both 64-limb edges exactly (the reference checker's form above 2^21 limbs)."""
+    import torch
+    n = 1 << 27
+    a, _ = O.gen_limbs(n, 0x5EED0003)
+    b, _ = O.gen_limbs(n, 0x5EED0004)
+    say("2^27 x 2^27")
+    da, db = dev(a), dev(b)
+    c = nttm.mul(da, db)
+    torch.cuda.synchronize()
+    del da, db
+    ch = host(c)
+    del c
+    assert O.tref_res(a, b, ch)
+    if O.have_gmp():
+        assert O.tref_edges(a, b, ch)
