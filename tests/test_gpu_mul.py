@@ -110,7 +110,7 @@ def make_batch(nprod, seed, max_len=5000, with_x=True, shared_b=True):
             b = host_ops[-1][1]
         x = None
         if with_x and i % 3 == 0:
-            x, s = O.gen_limbs(int(rng.integers(1, na + nb + 1)), s)
+            x, s = O.gen_limbs(int(rng.integers(1, len(a) + len(b) + 1)), s)
         host_ops.append((a, b, x))
     return host_ops
 
