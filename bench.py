@@ -164,6 +164,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-leaves", action="store_true")
     ap.add_argument("--leaf-reps", type=int, default=3)
+    ap.add_argument("--tile-log", type=int, default=0, help="strided-pass tile, log2 points (12 or 13; 0 = default)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 1 GiB device to device once (the reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the product's multi-GPU forms are not built (run with --gpus 1)",
@@ -177,7 +180,14 @@ def main():
     print("[bench] generating 2 x 2^%d limbs" % args.log_limbs, file=sys.stderr, flush=True)
     a = gen_limbs(n, 0x5EED0003)
     b = gen_limbs(n, 0x5EED0004)
-    nttm.init(0)
+    nttm.init(0, tile_log=args.tile_log)
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     da, db = to_dev(a), to_dev(b)
     dc = torch.empty(2 * n, dtype=torch.int64, device="cuda")
     print("[bench] warmup %d" % args.warmup, file=sys.stderr, flush=True)
@@ -238,7 +248,7 @@ def main():
                        "residue_arithmetic": "exact integers below 2^50 carried in FP64 (Shoup/Barrett), CRT in u64",
                        "b_alg_bytes": b_alg, "product_hbm_GBs": round(b_alg / sec / 1e9, 1),
                        "product_hbm_frac": round(b_alg / sec / 1e9 / HBM_PEAK_GBS, 4),
-                       "phase_ms": {"load_fwd": round(phase[0], 3), "pw_inv": round(phase[1], 3),
+                       "phase_ms": {"b_forward": round(phase[0], 3), "a_forward_pw_inverse": round(phase[1], 3),
                                     "crt": round(phase[2], 3)},
                        "kernels_per_step": kernels, "deterministic": deterministic,
                        "vs_baseline_ref": "time ratio to 0.1263 s, 1 MI300A (results/D13b_strategy_e0.txt:26)",

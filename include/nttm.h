@@ -50,6 +50,7 @@ enum { NTTM_Y_FULL = 0, NTTM_Y_BCAST = 1, NTTM_Y_PAIR = 2 };
 typedef struct nttm_config {
     int nprimes;                /* 0: choose per product (3 while the bound allows, else 4); 3 or 4: fixed */
     size_t workspace_bytes;     /* 0: grow on demand; else the cap of the library's device workspace */
+    int tile_log;               /* points per tile of the strided transform passes, log2: 12 or 13 (0: default) */
 } nttm_config;
 
 /* one product of a batch: c = a b (+ x).  c has na + nb limbs, or na + nb + 1 when x is given (nx <= na + nb),
@@ -100,8 +101,9 @@ int nttm_load(int prime, uint64_t *dst, const uint64_t *src, size_t nlimbs, size
 int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, int np, size_t ncoef,
              const uint64_t *x, size_t nx, void *stream);
 
-/* device-side time of the phases of the last nttm_mul / nttm_mul_add (ms): load+forward, pointwise+inverse, CRT */
-int nttm_last_phase_ms(double *fwd_ms, double *inv_ms, double *crt_ms);
+/* device-side time of the phases of the last nttm_mul / nttm_mul_add (ms): the forward transforms of b (limb load
+ * fused), those of a with the fused pointwise product and the inverse transforms, the CRT */
+int nttm_last_phase_ms(double *b_fwd_ms, double *a_fwd_pw_inv_ms, double *crt_ms);
 
 /* ---- live kernel timing (no reference counterpart; the reference times phases with events, E/rns_mul.c:548-587) ---- */
 typedef struct nttm_kstat {
@@ -116,8 +118,10 @@ int nttm_profile(int enable);
 int nttm_profile_read(nttm_kstat *out, int cap);
 
 /* ---- host-only introspection (no device calls; the reference has no counterpart) ---- */
-/* the pass plan of a 2^logn transform batch: 4 ints (lo, m, logw, logh) per pass into out[0 .. cap); returns passes */
-int nttm_debug_plan(int logn, uint64_t batch, int *out, int cap);
+/* the pass plan of a 2^logn transform batch: 4 ints (lo, m, logw, logh) per pass into out[0 .. cap); returns passes.
+ * tile_log / last_log: log2 of points per tile of the strided passes / of the last (contiguous) pass (0: defaults;
+ * the product path uses the configured tile_log and 12) */
+int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_log, int *out, int cap);
 /* the global element indices of tile `tile` of a pass of that geometry; returns the tile's element count */
 int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out);
 

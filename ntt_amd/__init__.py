@@ -36,7 +36,7 @@ class KStat(C.Structure):
 
 
 class Config(C.Structure):
-    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t)]
+    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t), ("tile_log", C.c_int)]
 
 
 _lib = None
@@ -69,7 +69,7 @@ def lib():
         L.nttm_last_phase_ms.argtypes = [C.POINTER(C.c_double)] * 3; L.nttm_last_phase_ms.restype = i32
         L.nttm_profile.argtypes = [i32]; L.nttm_profile.restype = i32
         L.nttm_profile_read.argtypes = [C.POINTER(KStat), i32]; L.nttm_profile_read.restype = i32
-        L.nttm_debug_plan.argtypes = [i32, u64, C.POINTER(C.c_int), i32]; L.nttm_debug_plan.restype = i32
+        L.nttm_debug_plan.argtypes = [i32, u64, i32, i32, C.POINTER(C.c_int), i32]; L.nttm_debug_plan.restype = i32
         L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
         L.nttm_debug_tile_indices.restype = i32
         _lib = L
@@ -105,9 +105,9 @@ def max_terms(np_):
     return int(lib().nttm_max_terms(np_))
 
 
-def debug_plan(logn, batch=1):
+def debug_plan(logn, batch=1, last_log=0, tile_log=13):
     out = (C.c_int * 24)()
-    n = lib().nttm_debug_plan(logn, batch, out, 24)
+    n = lib().nttm_debug_plan(logn, batch, tile_log, last_log, out, 24)
     return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
 
 
@@ -118,8 +118,8 @@ def debug_tile_indices(lo, m, logw, logh, tile):
 
 
 # ---- device entry points ------------------------------------------------------------------------------------------------
-def init(device=0, nprimes=0, workspace_bytes=0):
-    cfg = Config(nprimes, workspace_bytes)
+def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0):
+    cfg = Config(nprimes, workspace_bytes, tile_log)
     _check(lib().nttm_init(device, C.byref(cfg)))
 
 

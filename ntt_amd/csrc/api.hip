@@ -11,6 +11,10 @@
 #include "plan.h"
 
 int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s);
+int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s);
+#define NTTM_FUSED_LOG 12          /* tile of the fused forward + pointwise + inverse pass (k_fused) */
+#define NTTM_TILE_LOG_DEFAULT 12   /* tile of the strided passes: 2^12 points (32 KiB of LDS, 4 blocks per CU) */
+#define NTTM_PRODUCT_LOGN_MIN 4    /* products use at least 2^4 points, so every fused tile has >= 16 */
 struct LoadArgs;
 int nttm_run_u64_to_f64(double *x, uint64_t count, hipStream_t s);
 int nttm_run_cvt_u64_f64(double *dst, const uint64_t *src, uint64_t count, hipStream_t s);
@@ -198,6 +202,8 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     if (cfg) g.cfg = *cfg;
     if (g.cfg.nprimes && g.cfg.nprimes != 3 && g.cfg.nprimes != 4)
         return nttm_fail(NTTM_EINVAL, "nttm_init: nprimes must be 0, 3 or 4");
+    if (!g.cfg.tile_log) g.cfg.tile_log = NTTM_TILE_LOG_DEFAULT;
+    if (g.cfg.tile_log != 12 && g.cfg.tile_log != 13) return nttm_fail(NTTM_EINVAL, "nttm_init: tile_log must be 12 or 13");
     int nd = 0;
     HIPCK(hipGetDeviceCount(&nd));
     if (device < 0 || device >= nd) return nttm_fail(NTTM_EINVAL, "nttm_init: device %d of %d", device, nd);
@@ -232,12 +238,12 @@ static PassArgs base_args(int prime, double *x, int logn, uint64_t batch, bool i
 {
     PassArgs a; memset(&a, 0, sizeof a);
     a.x = x; a.y = nullptr; a.logn = logn; a.batch = batch; a.ymode = NTTM_Y_FULL;
-    a.pk = g.pk[prime]; a.tw = inv ? g.twi[prime] : g.twf[prime];
+    a.pk = g.pk[prime]; a.tw = inv ? g.twi[prime] : g.twf[prime]; a.twi = g.twi[prime];
     return a;
 }
 static int fwd_planes(int prime, double *x, int logn, uint64_t batch, hipStream_t s)
 {
-    Plan pl; plan_make(&pl, logn, batch, NTTM_TILE_LOG);
+    Plan pl; plan_make(&pl, logn, batch, g.cfg.tile_log);
     PassArgs a = base_args(prime, x, logn, batch, false);
     for (int i = 0; i < pl.npass; i++) RCK(nttm_run_pass(false, a, pl.p[i], s));
     return 0;
@@ -245,7 +251,7 @@ static int fwd_planes(int prime, double *x, int logn, uint64_t batch, hipStream_
 /* inverse with the pointwise product by y (ymode layout) fused into the first pass; scale by n^-1 in the last pass */
 static int inv_planes(int prime, double *x, const double *y, int ymode, int logn, uint64_t batch, int scale, hipStream_t s)
 {
-    Plan pl; plan_make(&pl, logn, batch, NTTM_TILE_LOG);
+    Plan pl; plan_make(&pl, logn, batch, g.cfg.tile_log);
     PassArgs a = base_args(prime, x, logn, batch, true);
     for (int i = pl.npass - 1; i >= 0; i--) {
         PassArgs b = a;
@@ -267,10 +273,6 @@ struct CrtDesc {                 /* must match crt.hip */
     uint64_t *c; uint64_t nc;
     uint64_t first_block;
 };
-struct LoadDesc {                /* must match ntt.hip */
-    const uint64_t *a; uint64_t na;
-    const uint64_t *b; uint64_t nb;
-};
 
 static int choose_np(size_t mn, int *np)
 {
@@ -284,6 +286,54 @@ static int choose_np(size_t mn, int *np)
 }
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+/* the transforms of one length group of products (cnt transforms of 2^logn points per prime):
+ *   B: forward, the first pass reading b's limbs (times n^-1, folded here once);
+ *   A: forward up to its last pass, the first pass reading a's limbs;
+ *   A: the fused last forward pass + pointwise product with B + first inverse pass (k_fused);
+ *   A: the remaining inverse passes.
+ * Operand limbs come from (a, na, b, nb) when cnt == 1 and ldesc is null, else from the device descriptors ldesc.
+ * Result: plane i of A holds the cyclic convolution mod p_i. */
+/* the strided-pass tile for a 2^logn product transform: the configured one, or 2^13 where that saves a pass */
+static int product_tile_log(int logn)
+{
+    const int R = logn > NTTM_FUSED_LOG ? logn - NTTM_FUSED_LOG : 0;
+    const int tl = g.cfg.tile_log, k_cfg = (R + tl - 5) / (tl - 4), k_13 = (R + 8) / 9;
+    return k_13 < k_cfg ? 13 : tl;
+}
+
+static int product_transforms(int np, int logn, uint64_t cnt, double *A, double *Bp, const uint64_t *a, uint64_t na,
+                              const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s, hipEvent_t mid)
+{
+    Plan pl; plan_make2(&pl, logn, cnt, product_tile_log(logn), NTTM_FUSED_LOG);
+    const uint64_t n = (uint64_t)1 << logn;
+    for (int i = 0; i < np; i++) {                               /* B, every prime */
+        double *Bi = Bp + (size_t)i * cnt * n;
+        const uint64_t p = NTTM_PRIMES[i];
+        PassArgs fb = base_args(i, Bi, logn, cnt, false);
+        fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
+        fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
+        for (int j = 0; j < pl.npass; j++) {
+            RCK(nttm_run_pass(false, fb, pl.p[j], s));
+            fb.limbs = nullptr; fb.ldesc = nullptr; fb.do_in_scale = 0;
+        }
+    }
+    if (mid) HIPCK(hipEventRecord(mid, s));
+    for (int i = 0; i < np; i++) {                               /* A, every prime */
+        double *Ai = A + (size_t)i * cnt * n, *Bi = Bp + (size_t)i * cnt * n;
+        PassArgs fa = base_args(i, Ai, logn, cnt, false);
+        fa.limbs = ldesc ? nullptr : a; fa.nlimbs = na; fa.ldesc = ldesc; fa.which = 0;
+        for (int j = 0; j + 1 < pl.npass; j++) {
+            RCK(nttm_run_pass(false, fa, pl.p[j], s));
+            fa.limbs = nullptr; fa.ldesc = nullptr;
+        }
+        fa.y = Bi; fa.ymode = NTTM_Y_FULL;
+        RCK(nttm_run_fused(fa, pl.p[pl.npass - 1], s));
+        PassArgs ia = base_args(i, Ai, logn, cnt, true);
+        for (int j = pl.npass - 2; j >= 0; j--) RCK(nttm_run_pass(true, ia, pl.p[j], s));
+    }
+    return 0;
+}
 
 /* c = a b (+ x), device pointers; synchronous (returns after the stream has drained) */
 static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
@@ -302,7 +352,7 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     }
     int np; RCK(choose_np(na < nb ? na : nb, &np));
     const uint64_t ncoef = na + nb - 1;
-    int logn = ceil_log2(ncoef); if (logn < 1) logn = 1;
+    int logn = ceil_log2(ncoef); if (logn < NTTM_PRODUCT_LOGN_MIN) logn = NTTM_PRODUCT_LOGN_MIN;
     if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", logn, NTTM_LOGN_MAX);
     const uint64_t n = (uint64_t)1 << logn;
     const size_t planes_bytes = (size_t)2 * np * n * 8;
@@ -313,17 +363,8 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     CrtDesc *ddesc = (CrtDesc *)(g.ws + off_desc);
     uint32_t *scratch = (uint32_t *)(g.ws + off_crt);
 
-    double2 sc[NTTM_MAX_PRIMES];
-    for (int i = 0; i < np; i++) { const uint64_t p = NTTM_PRIMES[i]; sc[i] = h_shoup(h_inv(n % p, p), p); }
     HIPCK(hipEventRecord(g.ev[0], s));
-    RCK(nttm_run_load_planes(A, n, a, na, n, np, g.pk, nullptr, 0, s));
-    RCK(nttm_run_load_planes(Bp, n, b, nb, n, np, g.pk, sc, 0, s));
-    for (int i = 0; i < np; i++) {
-        RCK(fwd_planes(i, A + (size_t)i * n, logn, 1, s));
-        RCK(fwd_planes(i, Bp + (size_t)i * n, logn, 1, s));
-    }
-    HIPCK(hipEventRecord(g.ev[1], s));
-    for (int i = 0; i < np; i++) RCK(inv_planes(i, A + (size_t)i * n, Bp + (size_t)i * n, NTTM_Y_FULL, logn, 1, 0, s));
+    RCK(product_transforms(np, logn, 1, A, Bp, a, na, b, nb, nullptr, s, g.ev[1]));
     HIPCK(hipEventRecord(g.ev[2], s));
     CrtDesc hd; hd.pl = A; hd.plane = n; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
     memcpy(g.hpin, &hd, sizeof hd);
@@ -396,7 +437,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         if (!q.c || (q.na && !q.a) || (q.nb && !q.b)) return nttm_fail(NTTM_EINVAL, "product %zu: null pointer", i);
         if (!q.na || !q.nb) continue;
         size_t mn = q.na < q.nb ? q.na : q.nb; if (mn > mn_max) mn_max = mn;
-        int logn = ceil_log2(q.na + q.nb - 1); if (logn < 1) logn = 1;
+        int logn = ceil_log2(q.na + q.nb - 1); if (logn < NTTM_PRODUCT_LOGN_MIN) logn = NTTM_PRODUCT_LOGN_MIN;
         if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "product %zu: transform above 2^%d", i, NTTM_LOGN_MAX);
         groups[logn].push_back(i);
     }
@@ -438,15 +479,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         memcpy(hp + cnt * sizeof(CrtDesc), ld.data(), cnt * sizeof(LoadDesc));
         HIPCK(hipMemcpyAsync(g.ws + off_cd, hp, cnt * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + cnt * sizeof(CrtDesc), cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
-        double2 sc[NTTM_MAX_PRIMES];
-        for (int i = 0; i < np; i++) { const uint64_t p = NTTM_PRIMES[i]; sc[i] = h_shoup(h_inv(n % p, p), p); }
-        RCK(nttm_run_load_batch(A, cnt * n, g.ws + off_ld, 0, cnt, logn, np, g.pk, nullptr, s));
-        RCK(nttm_run_load_batch(Bp, cnt * n, g.ws + off_ld, 1, cnt, logn, np, g.pk, sc, s));
-        for (int i = 0; i < np; i++) {
-            RCK(fwd_planes(i, A + (size_t)i * cnt * n, logn, cnt, s));
-            RCK(fwd_planes(i, Bp + (size_t)i * cnt * n, logn, cnt, s));
-            RCK(inv_planes(i, A + (size_t)i * cnt * n, Bp + (size_t)i * cnt * n, NTTM_Y_FULL, logn, cnt, 0, s));
-        }
+        RCK(product_transforms(np, logn, cnt, A, Bp, nullptr, 0, nullptr, 0, (const LoadDesc *)(g.ws + off_ld), s, nullptr));
         double crt_bytes = 0;
         for (uint64_t j = 0; j < cnt; j++) crt_bytes += 8.0 * np * cd[j].ncoef + 8.0 * cd[j].nc + 8.0 * cd[j].nx;
         RCK(nttm_crt_launch_batch(g.ws + off_cd, cnt, nblk, np, g.cc, (uint32_t *)(g.ws + off_crt), s, crt_bytes));
@@ -627,9 +660,11 @@ extern "C" int nttm_profile_read(nttm_kstat *out, int cap)
 }
 
 /* ---- host-side introspection for tests: the pass plan and the tile -> element map ---------------------------------- */
-extern "C" int nttm_debug_plan(int logn, uint64_t batch, int *out, int cap)
+extern "C" int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_log, int *out, int cap)
 {
-    Plan pl; plan_make(&pl, logn, batch, NTTM_TILE_LOG);
+    if (tile_log <= 0) tile_log = NTTM_TILE_LOG_DEFAULT;
+    if (tile_log > NTTM_TILE_LOG || last_log > NTTM_TILE_LOG) return nttm_fail(NTTM_EINVAL, "tile logs above %d", NTTM_TILE_LOG);
+    Plan pl; plan_make2(&pl, logn, batch, tile_log, last_log > 0 ? last_log : tile_log);
     int k = 0;
     for (int i = 0; i < pl.npass && k + 4 <= cap; i++) { out[k++] = pl.p[i].lo; out[k++] = pl.p[i].m; out[k++] = pl.p[i].logw; out[k++] = pl.p[i].logh; }
     return pl.npass;

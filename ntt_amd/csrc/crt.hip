@@ -3,11 +3,13 @@
  * Coefficient k of the convolution is rebuilt from its np residues by Garner's mixed radix (modular steps in the
  * FP64 arithmetic of arith.h, the integer value in np words), split into np base-10^18 digits d_0..d_{np-1}, and
  *     limb_k = d_0[k] + d_1[k-1] + ... + d_{np-1}[k-np+1] + x_k + carry_k            (E/rns_mul.c:332-405)
- * The top digit is small (X < 2^(50 np) gives d_{np-1} < 2^31), so v_k < np B + 2^31 <= 4 B + 2^31, and the carry into
- * the next limb, f_k(carry_k) = floor((v_k + carry_k) / B), maps {0..7} into {0..4}: a map fits 8 x 3 bits.  Maps
- * compose, so the carries are a parallel scan:
- * inside a block by wave shuffles and LDS, across blocks by a decoupled look-back (blocks take consecutive tickets, a
- * block publishes its aggregate map before looking back, so every wait is on a block that is already running).
+ * The top digit is small (X < 2^(50 np) gives d_{np-1} < 2^31), so v_k = sum of the digits + x_k < 4 B + 2^31.  With
+ * q_k = floor(v_k / B), r_k = v_k mod B and w_k = r_k + q_{k-1} (<= B + 3), the carry into limb k+1 is q_k + o_k where
+ * the overflow bit o_k = [w_k + o_{k-1} >= B] = g_k | (p_k & o_{k-1}), g_k = [w_k >= B], p_k = [w_k = B - 1]: a binary
+ * carry-lookahead chain.  For 64 consecutive limbs it is resolved by one 64-bit addition of the ballot masks
+ * X = G, Y = G | P (the carries into the bits of X + Y + cin are the o's, the textbook adder identity); words chain
+ * inside a wave, waves inside a block, and blocks by a decoupled look-back over (cin -> cout) maps of one bit
+ * (blocks take consecutive tickets and publish their map before looking back, so every wait is on a running block).
  * The reference instead ripples each 256-coefficient chunk on one thread of a wave64 ballot scan, writes a 4-limb spill
  * per stripe and merges the spills serially on the host or on one thread per product (E/rns_mul.c:452-470;
  * E/dbig.c:1007-1052).  The final carry beyond limb nc-1 must be zero; a non-zero one sets an error flag.
@@ -17,8 +19,8 @@
 #define CRT_T 256
 #define CRT_PER 8
 #define CRT_CH (CRT_T * CRT_PER)
-#define CRT_HALO 3
-#define CRT_ID_MAP 0xFAC688u             /* the identity map c -> c, 3 bits per c (c = 0..7) */
+#define CRT_HALO 4                       /* digits of coefficients k0-4 .. k0-1: v_{k0-1} with four primes */
+#define CRT_ID_MAP 0x2u                  /* one-bit carry maps: bit c = cout(cin = c); the identity is 0b10 */
 #define CRT_ST_AGG 1u
 #define CRT_ST_INC 2u
 #define CRT_SPIN_MAX (1u << 22)
@@ -32,19 +34,24 @@ struct CrtDesc {                         /* one product; must match api.hip */
 };
 struct CrtArgs {
     const CrtDesc *desc; uint64_t nprod;
-    uint32_t *status;                    /* one word per block: state << 30 | carry << 24 | aggregate map */
+    uint32_t *status;                    /* one word per block: state << 30 | carry out << 2 | aggregate map (2 bits) */
     uint32_t *ctl;                       /* [0] ticket counter, [1] error flags */
     CrtConst cc;
 };
 
-__device__ __forceinline__ uint32_t map_apply(uint32_t f, uint32_t c) { return (f >> (3 * c)) & 7u; }
-/* (g o f)(c) = g(f(c)) */
+__device__ __forceinline__ uint32_t map_apply(uint32_t f, uint32_t c) { return (f >> c) & 1u; }
+/* (g o f)(c) = g(f(c)) for maps {0,1} -> {0,1} */
 __device__ __forceinline__ uint32_t map_compose(uint32_t g, uint32_t f)
 {
-    uint32_t h = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < 8; c++) h |= map_apply(g, (f >> (3 * c)) & 7u) << (3 * c);
-    return h;
+    return map_apply(g, f & 1u) | (map_apply(g, (f >> 1) & 1u) << 1);
+}
+/* carry out of X + Y + cin over 64 bits */
+__device__ __forceinline__ uint32_t add_cout(uint64_t X, uint64_t Y, uint32_t cin)
+{
+    const uint64_t s = X + Y;
+    const uint32_t c1 = s < X;
+    const uint64_t s2 = s + cin;
+    return c1 | (uint32_t)(s2 < s);
 }
 
 template <int NP>
@@ -81,12 +88,21 @@ __device__ __forceinline__ void crt_digits(const CrtConst &cc, const double *r, 
             X[w] = s2; carry = hi + c1;
         }
     }
-#pragma unroll
-    for (int k = 0; k < NP; k++) {                       /* long division by B, least significant digit first */
-        uint64_t rem = 0;
-#pragma unroll
-        for (int w = NP - 1; w >= 0; w--) { uint64_t q; div_b(rem, X[w], &q, &rem); X[w] = q; }
-        d[k] = rem;
+    /* base-10^18 digits: X < 2^(50 NP), so the top word is below 2^(50 NP - 64 (NP-1)) < B and every quotient of the
+     * chain below keeps its high word under B (div_b's precondition).  NP = 3: 3 divisions, NP = 4: 6. */
+    if (NP == 3) {
+        uint64_t q1, q0, r;
+        div_b(X[2], X[1], &q1, &r);                  /* X[2] < 2^22 */
+        div_b(r, X[0], &q0, &d[0]);
+        div_b(q1, q0, &d[2], &d[1]);                 /* q1 < 2^27: X / B = q1 2^64 + q0 < 2^91 */
+    } else {
+        uint64_t q2, q1, q0, r, p1, p0;
+        div_b(X[3], X[2], &q2, &r);                  /* X[3] < 2^8 */
+        div_b(r, X[1], &q1, &r);
+        div_b(r, X[0], &q0, &d[0]);
+        div_b(q2, q1, &p1, &r);                      /* q2 < 2^13 */
+        div_b(r, q0, &p0, &d[1]);
+        div_b(p1, p0, &d[3], &d[2]);                 /* p1 < 2^17 */
     }
 }
 
@@ -129,95 +145,122 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
     }
     __syncthreads();
 
-    /* per thread: 8 consecutive limb positions, their sums and carry maps */
-    uint64_t rr[CRT_PER]; uint32_t qq[CRT_PER];
-    uint32_t F = CRT_ID_MAP;
-    const uint64_t kb = k0 + (uint64_t)tid * CRT_PER;
+    /* limb k = k0 + wave 512 + i 64 + lane for i < 8: one ballot per i gives 64 consecutive limbs */
+    uint64_t wv[CRT_PER];
+    uint64_t GX[CRT_PER], GY[CRT_PER];
+    uint32_t qlast = 0;
 #pragma unroll
     for (int i = 0; i < CRT_PER; i++) {
-        const uint64_t k = kb + i;
-        const int li = CRT_HALO + tid * CRT_PER + i;
-        uint64_t v = 0;
+        const int off = wave * 64 * CRT_PER + i * 64 + lane;          /* offset of limb k inside the block */
+        const uint64_t k = k0 + off;
+        const int li = CRT_HALO + off;
+        uint64_t v = 0, vp = 0;                                      /* v_k and v_{k-1} */
         if (k < dsc.nc) {
 #pragma unroll
             for (int j = 0; j < NP; j++) v += D[j][li - j];
             if (k < dsc.nx) v += dsc.x[k];
         }
-        const uint32_t q = (v >= NTTM_B) + (v >= 2 * NTTM_B) + (v >= 3 * NTTM_B) + (v >= 4 * NTTM_B);
-        const uint64_t r = v - (uint64_t)q * NTTM_B;
-        rr[i] = r; qq[i] = q;
-        uint32_t f = 0;
+        if (k >= 1 && k - 1 < dsc.nc) {
 #pragma unroll
-        for (uint32_t c = 0; c < 8; c++) f |= (q + (r + c >= NTTM_B ? 1u : 0u)) << (3 * c);
+            for (int j = 0; j < NP; j++) vp += D[j][li - 1 - j];
+            if (k - 1 < dsc.nx) vp += dsc.x[k - 1];
+        }
+        const uint32_t q = (v >= NTTM_B) + (v >= 2 * NTTM_B) + (v >= 3 * NTTM_B) + (v >= 4 * NTTM_B);
+        const uint32_t qp = (vp >= NTTM_B) + (vp >= 2 * NTTM_B) + (vp >= 3 * NTTM_B) + (vp >= 4 * NTTM_B);
+        const uint64_t w = v - (uint64_t)q * NTTM_B + qp;            /* <= B + 3 */
+        wv[i] = w;
+        const uint64_t g = __ballot(w >= NTTM_B), pr = __ballot(w == NTTM_B - 1);
+        GX[i] = g; GY[i] = g | pr;
+        if (k + 1 == dsc.nc) qlast = q;
+    }
+    /* the wave's map cin -> cout over its 8 words */
+    uint32_t F = CRT_ID_MAP;
+#pragma unroll
+    for (int i = 0; i < CRT_PER; i++) {
+        const uint32_t f = add_cout(GX[i], GY[i], 0) | (add_cout(GX[i], GY[i], 1) << 1);
         F = map_compose(f, F);
     }
-
-    /* block scan of the maps: inclusive within the wave, then across the waves */
-    uint32_t I = F;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        uint32_t o = __shfl_up(I, dd, 64);
-        if (lane >= dd) I = map_compose(I, o);
-    }
-    uint32_t E = __shfl_up(I, 1, 64);
-    if (lane == 0) E = CRT_ID_MAP;
-    if (lane == 63) s_wave[wave] = I;
+    if (lane == 0) s_wave[wave] = F;
     __syncthreads();
     uint32_t P = CRT_ID_MAP, A = CRT_ID_MAP;
     for (int w = 0; w < CRT_T / 64; w++) {
         if (w < wave) P = map_compose(s_wave[w], P);
         A = map_compose(s_wave[w], A);
     }
-    E = map_compose(E, P);                                  /* maps of every earlier position of the block */
 
-    /* carry into the block: decoupled look-back over the earlier blocks */
-    if (tid == 0) {
+    /* carry into the block: decoupled look-back over the earlier blocks of the product, 64 at a time by wave 0.  The
+     * status word carries its whole payload (map and carry), so the atomics are relaxed at agent scope: no cache
+     * write-back or invalidate is needed around them.  Lane i
+     * reads block blk-1-i; the nearest block with an inclusive carry ends the walk, the aggregate maps of the blocks
+     * in between are composed (nearest outermost).  Blocks before the product's first act as an inclusive carry 0. */
+    if (wave == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&a.status[blk], local == 0 ? ((CRT_ST_INC << 30) | (map_apply(A, 0) << 2) | A)
+                                                          : ((CRT_ST_AGG << 30) | A),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint32_t cin = 0;
-        if (local == 0) {
-            __hip_atomic_store(&a.status[blk], (CRT_ST_INC << 30) | (map_apply(A, 0) << 24) | A, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&a.status[blk], (CRT_ST_AGG << 30) | A, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (local != 0) {
             uint32_t G = CRT_ID_MAP, spins = 0;
-            int64_t i = (int64_t)blk - 1;
-            while (i >= (int64_t)first) {
-                const uint32_t st = __hip_atomic_load(&a.status[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t base = (int64_t)blk - 1;
+            for (;;) {
+                const int64_t idx = base - lane;
+                const uint32_t st = idx >= (int64_t)first
+                    ? __hip_atomic_load(&a.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    : ((CRT_ST_INC << 30) | CRT_ID_MAP);
                 const uint32_t state = st >> 30;
-                if (state == CRT_ST_INC) { cin = map_apply(G, (st >> 24) & 7u); break; }
-                if (state == CRT_ST_AGG) { G = map_compose(G, st & 0xffffffu); i--; continue; }
-                if (++spins > CRT_SPIN_MAX) { atomicOr(&a.ctl[1], 2u); cin = 0; break; }
-                __builtin_amdgcn_s_sleep(1);
+                const uint64_t inc = __ballot(state == CRT_ST_INC), wait = __ballot(state == 0);
+                const int fi = inc ? __ffsll((long long)inc) - 1 : 64;
+                const int fw = wait ? __ffsll((long long)wait) - 1 : 64;
+                if (fw < fi) {                               /* a block between us and the nearest inclusive one */
+                    if (++spins > CRT_SPIN_MAX) { if (lane == 0) atomicOr(&a.ctl[1], 2u); cin = 0; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                uint32_t m = lane < fi ? (st & 3u) : CRT_ID_MAP;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {           /* m_0 o m_1 o ... o m_63 into lane 0 */
+                    const uint32_t o = __shfl_down(m, d, 64);
+                    if (lane + d < 64) m = map_compose(m, o);
+                }
+                m = __shfl(m, 0, 64);
+                G = map_compose(G, m);
+                if (fi < 64) {
+                    const uint32_t c_inc = (__shfl(st, fi, 64) >> 2) & 1u;
+                    cin = map_apply(G, c_inc);
+                    break;
+                }
+                base -= 64;
             }
-            __hip_atomic_store(&a.status[blk], (CRT_ST_INC << 30) | (map_apply(A, cin) << 24) | A, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&a.status[blk], (CRT_ST_INC << 30) | (map_apply(A, cin) << 2) | A, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_cin = cin;
+        if (lane == 0) s_cin = cin;
     }
     __syncthreads();
 
-    /* final limbs into LDS (plane 0's digit array is no longer read), then a coalesced store */
-    uint32_t c = map_apply(E, s_cin);
-    uint64_t *out = &D[0][0];
+    /* final limbs: out_k = w_k + o_{k-1} - B o_k, stored directly (lanes hold consecutive limbs) */
+    uint32_t c = map_apply(P, s_cin);                               /* the carry into this wave's first word */
 #pragma unroll
     for (int i = 0; i < CRT_PER; i++) {
-        const uint64_t s = rr[i] + c;
-        const uint32_t ge = s >= NTTM_B;
-        out[tid * CRT_PER + i] = ge ? s - NTTM_B : s;
-        c = qq[i] + ge;
-        if (kb + i + 1 == dsc.nc) {                           /* the last limb: nothing may remain beyond it */
-            uint64_t extra = c;
-            const int li = CRT_HALO + tid * CRT_PER + i;
+        const uint64_t S = GX[i] + GY[i] + c;
+        const uint32_t cout = add_cout(GX[i], GY[i], c);
+        const uint64_t CI = S ^ GX[i] ^ GY[i];                      /* bit b: the carry into bit b (o_{k-1}) */
+        const uint32_t oin = (uint32_t)(CI >> lane) & 1u;
+        const uint32_t oout = lane == 63 ? cout : (uint32_t)(CI >> (lane + 1)) & 1u;
+        const int off = wave * 64 * CRT_PER + i * 64 + lane;
+        const uint64_t k = k0 + off;
+        if (k < dsc.nc) dsc.c[k] = wv[i] + oin - (oout ? NTTM_B : 0);
+        if (k + 1 == dsc.nc) {                                     /* the last limb: nothing may remain beyond it */
+            uint64_t extra = qlast + oout;
+            const int li = CRT_HALO + off;
 #pragma unroll
             for (int e = 1; e < NP; e++)
 #pragma unroll
                 for (int j = e; j < NP; j++) extra += D[j][li + e - j];
             if (extra) atomicOr(&a.ctl[1], 1u);
         }
-    }
-    __syncthreads();
-    for (int i = 0; i < CRT_PER; i++) {
-        const uint64_t kk = k0 + (uint64_t)i * CRT_T + tid;
-        if (kk < dsc.nc) dsc.c[kk] = out[i * CRT_T + tid];
+        c = cout;
     }
 }
 

@@ -22,6 +22,12 @@ struct TwTables {                   /* device pointers, one set per prime and di
     const double2 *bh, *bl;         /* boundary tables, 2^NTTM_BT_LOG entries each */
 };
 
+/* operand limbs of the products of a batch (device array, one per transform) */
+struct LoadDesc {
+    const uint64_t *a; uint64_t na;
+    const uint64_t *b; uint64_t nb;
+};
+
 /* one pass over bits [lo, lo + m) of every transform of 2^logn points in a batch; a tile holds W low columns x 2^m
  * rows x H high blocks (log2 of each below) */
 struct PassArgs {
@@ -31,9 +37,14 @@ struct PassArgs {
     int ymode;
     uint64_t batch;
     PrimeK pk;
-    TwTables tw;
+    TwTables tw;                    /* forward or inverse tables of the pass's direction */
+    TwTables twi;                   /* inverse tables (the fused forward + inverse pass) */
     double2 scale;                  /* (w, wq) of a final multiply (n^-1), used when do_scale */
     int do_scale;
+    /* input from limbs instead of the plane (the first forward pass): transform b reads limbs[b][0 .. nl) */
+    const uint64_t *limbs; uint64_t nlimbs;     /* batch of one */
+    const LoadDesc *ldesc; int which;           /* batches: operand a (which = 0) or b of ldesc[b] */
+    double2 in_scale; int do_in_scale;          /* multiply loaded limbs by in_scale (n^-1 folded into one operand) */
 };
 
 struct Ctx;

@@ -9,93 +9,334 @@
 #include "internal.h"
 #include "plan.h"
 
-__device__ __forceinline__ uint32_t brev_m(uint32_t r, int m) { return m ? (__brev(r) >> (32 - m)) : 0u; }
 
-/* w_N^(e) for N = 2^(lo+m): exponent scaled to the order-2^32 tables and split 16/16 */
-__device__ __forceinline__ double twiddle_apply(double v, uint64_t e, int lom, const TwTables &tw, double p)
+/* ---- the pass kernel -------------------------------------------------------------------------------------------------
+ * A tile of T = W x 2^m x H points (T <= 2^13) is handled by T / 16 threads holding 16 points each in registers.  The m
+ * stages run in rounds of up to 4 stages (radix 2..16 in registers); between rounds the tile goes through LDS once.
+ * In a round over row bits [s_lo, s_lo + logR) (L = logw + s_lo), thread t owns G = 16 / R groups of R points; group g
+ * has the base element
+ *     eb = (tau mod 2^L) | (tau div 2^L) << (L + logR),   tau = g * nthr + t,
+ * and point j of the group is element eb + j 2^L.  For every point the lanes of a wave touch consecutive LDS words
+ * (and, in the first and last round, consecutive global addresses of a row).  Since the round's bits lie inside the
+ * row field, point j sits at global offset ob + j 2^(s_lo + lo) and at bit-reversed row kb + brev(j) 2^(m - s_lo - logR):
+ * addresses and boundary-twiddle exponents of a group are a base plus compile-time multiples.  Each thread reads and
+ * writes exactly the elements of its groups within a round, so one barrier separates consecutive rounds.
+ * Lazy reduction (bounds of arith.h): the DIF reduces its sum path at odd levels and at a round's last level (values
+ * entering a butterfly stay within [-2p, 2p]); the DIT reduces both outputs at a round's last level only (after levels
+ * 0..3 the outputs are within 1.75p, 2.5p, 3.5p, 4.5p + 2, and every mul_shoup input within 4p). */
+#define PASS_PTS 16               /* points per thread of full tiles; tiles below 16 points use one thread */
+
+template <int BITS>
+__device__ __forceinline__ constexpr uint32_t brev_c(uint32_t j)
 {
-    uint32_t E = (uint32_t)(e << (32 - lom));
-    double2 h = tw.bh[E >> NTTM_BT_LOG];
-    double2 l = tw.bl[E & ((1u << NTTM_BT_LOG) - 1)];
+    uint32_t r = 0;
+    for (int i = 0; i < BITS; i++) r |= ((j >> i) & 1u) << (BITS - 1 - i);
+    return r;
+}
+
+template <int LOGR>
+__device__ __forceinline__ void dif_group(double *v, int s_lo, uint32_t rlow, const double2 *__restrict__ tw, double p,
+                                          double pinv)
+{
+    constexpr int R = 1 << LOGR;
+#pragma unroll
+    for (int lev = 0; lev < LOGR; lev++) {
+        const int k = LOGR - 1 - lev, h = 1 << k;
+        const bool red = (lev & 1) || lev == LOGR - 1;
+        const uint32_t tbase = (1u << (s_lo + k)) + rlow;
+#pragma unroll
+        for (int jj = 0; jj < h; jj++) {
+            const double2 w = tw[tbase + ((uint32_t)jj << s_lo)];
+#pragma unroll
+            for (int j = jj; j < R; j += 2 * h) {
+                const double u = v[j], x = v[j + h];
+                const double sm = u + x;
+                v[j] = red ? reduce(sm, p, pinv) : sm;
+                v[j + h] = mul_shoup(u - x, w.x, w.y, p);
+            }
+        }
+    }
+}
+
+template <int LOGR>
+__device__ __forceinline__ void dit_group(double *v, int s_lo, uint32_t rlow, const double2 *__restrict__ tw, double p,
+                                          double pinv)
+{
+    constexpr int R = 1 << LOGR;
+#pragma unroll
+    for (int lev = 0; lev < LOGR; lev++) {
+        const int k = lev, h = 1 << k;
+        const bool red = lev == LOGR - 1;
+        const uint32_t tbase = (1u << (s_lo + k)) + rlow;
+#pragma unroll
+        for (int jj = 0; jj < h; jj++) {
+            const double2 w = tw[tbase + ((uint32_t)jj << s_lo)];
+#pragma unroll
+            for (int j = jj; j < R; j += 2 * h) {
+                const double u = v[j], x = mul_shoup(v[j + h], w.x, w.y, p);
+                const double s1 = u + x, s2 = u - x;
+                v[j] = red ? reduce(s1, p, pinv) : s1;
+                v[j + h] = red ? reduce(s2, p, pinv) : s2;
+            }
+        }
+    }
+}
+
+/* geometry of the pass seen by one thread */
+struct PassPos {
+    int m, logw, lo, nthr, t;
+};
+
+/* f(slot, e, o, c, k) for every slot of a round: e the LDS index, o the global offset in the tile, c the column inside
+ * the tile, k the bit-reversed row */
+template <int LOGR, int PTS, typename F>
+__device__ __forceinline__ void for_slots(const PassPos &pp, int s_lo, F f)
+{
+    constexpr int R = 1 << LOGR, G = PTS / R;
+    const int L = pp.logw + s_lo;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t tau = (uint32_t)(g * pp.nthr + pp.t);
+        const uint32_t eb = (tau & ((1u << L) - 1)) | ((tau >> L) << (L + LOGR));
+        const uint32_t c = eb & ((1u << pp.logw) - 1), r = (eb >> pp.logw) & ((1u << pp.m) - 1);
+        const uint32_t h = eb >> (pp.logw + pp.m);
+        const uint32_t ob = (h << (pp.lo + pp.m)) + (r << pp.lo) + c;
+        const uint32_t kb = pp.m ? (__brev(r) >> (32 - pp.m)) : 0u;
+        const int ksh = pp.m - s_lo - LOGR;
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            f(g * R + j, eb + ((uint32_t)j << L), ob + ((uint32_t)j << (s_lo + pp.lo)), c, kb + (brev_c<LOGR>(j) << ksh));
+    }
+}
+
+/* the radix step of one round on all PTS slots: PTS / R groups of R points */
+template <int LOGR, bool INV, int PTS>
+__device__ __forceinline__ void round_compute(double *v, const PassPos &pp, int s_lo, const double2 *tw, double p,
+                                              double pinv)
+{
+    constexpr int R = 1 << LOGR, G = PTS / R;
+    const int L = pp.logw + s_lo;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t tau = (uint32_t)(g * pp.nthr + pp.t);
+        const uint32_t rlow = (tau & ((1u << L) - 1)) >> pp.logw;      /* row bits below s_lo */
+        if (INV) dit_group<LOGR>(v + g * R, s_lo, rlow, tw, p, pinv);
+        else dif_group<LOGR>(v + g * R, s_lo, rlow, tw, p, pinv);
+    }
+}
+
+/* rounds of a pass: sizes of at most 4 bits, balanced; forward from the top bits down, inverse from bit 0 up */
+__device__ __forceinline__ void round_split(int m, int i, int *s_lo, int *logR, bool inv)
+{
+    const int nr = (m + 3) >> 2, q = m / nr, rem = m % nr;
+    int lo = 0;                                     /* rounds in ascending bit order: sizes q+1 first (rem of them) */
+    int k = inv ? i : nr - 1 - i;
+    for (int r = 0; r < k; r++) lo += q + (r < rem ? 1 : 0);
+    *s_lo = lo; *logR = q + (k < rem ? 1 : 0);
+}
+
+/* call body.template operator()<LOGR>() with the runtime round size */
+#define ROUND_SWITCH(logR, ...) \
+    switch (logR) { \
+    case 4: { constexpr int LR = 4; __VA_ARGS__ } break; \
+    case 3: { constexpr int LR = 3; __VA_ARGS__ } break; \
+    case 2: { constexpr int LR = 2; __VA_ARGS__ } break; \
+    default: { constexpr int LR = 1; __VA_ARGS__ } break; }
+
+/* the pass's input element: from the plane, or (the first forward pass) from the operand limbs, zero beyond them */
+__device__ __forceinline__ double input_limb(const PassArgs &a, uint64_t gi, double p, double pinv)
+{
+    const uint64_t b = gi >> a.logn, k = gi & (((uint64_t)1 << a.logn) - 1);
+    const uint64_t *src; uint64_t nl;
+    if (a.ldesc) { const LoadDesc &d = a.ldesc[b]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; }
+    else { src = a.limbs; nl = a.nlimbs; }
+    if (k >= nl) return 0.0;
+    double x = limb_to_res(src[k], p, pinv);
+    if (a.do_in_scale) x = mul_shoup(x, a.in_scale.x, a.in_scale.y, p);
+    return x;
+}
+__device__ __forceinline__ uint64_t y_index(const PassArgs &a, uint64_t gi)
+{
+    const uint64_t nmask = ((uint64_t)1 << a.logn) - 1;
+    return a.ymode == NTTM_Y_FULL ? gi : a.ymode == NTTM_Y_BCAST ? (gi & nmask)
+                                       : ((((gi >> a.logn) >> 1) << a.logn) | (gi & nmask));
+}
+
+enum { PM_FWD = 0, PM_FWD_LIMB = 1, PM_INV = 2, PM_INV_PW = 3 };
+
+/* w_N^(col k) for N = 2^lom with the exponent wrapped to 32 bits (only its residue mod 2^lom matters) */
+__device__ __forceinline__ double boundary_twiddle(double v, uint32_t col, uint32_t k, int lom, const TwTables &tw,
+                                                   double p)
+{
+    const uint32_t E = (col * k) << (32 - lom);
+    const double2 h = tw.bh[E >> NTTM_BT_LOG];
+    const double2 l = tw.bl[E & ((1u << NTTM_BT_LOG) - 1)];
     v = mul_shoup(v, h.x, h.y, p);
     return mul_shoup(v, l.x, l.y, p);
 }
 
-template <bool INV, bool PW>
-__global__ __launch_bounds__(NTTM_THREADS) void k_pass(PassArgs a)
+template <int MODE, int PTS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_pass(PassArgs a)
 {
-    __shared__ double t[1 << NTTM_TILE_LOG];
-    PassGeom g; g.lo = a.lo; g.m = a.m; g.logw = a.logw; g.logh = a.logh;
-    const int T = 1 << (g.m + g.logw + g.logh);
-    const int M = 1 << g.m;
-    const uint64_t tile = blockIdx.x;
+    constexpr bool INV = MODE >= PM_INV;
+    extern __shared__ double lds[];                             /* T doubles, sized at launch */
+    PassPos pp; pp.m = a.m; pp.logw = a.logw; pp.lo = a.lo;
+    const int T = 1 << (a.m + a.logw + a.logh);
+    pp.nthr = T / PTS; pp.t = threadIdx.x;
+    const bool active = pp.t < pp.nthr;
     const double p = a.pk.p, pinv = a.pk.pinv;
-    const uint64_t nmask = ((uint64_t)1 << a.logn) - 1;
-    const int lom = g.lo + g.m;
+    const int lom = a.lo + a.m, nr = (a.m + 3) >> 2;
+    /* the tile: column block cb (W columns below bit lo) of block group hb (H blocks of 2^lom points) */
+    const uint64_t tile = blockIdx.x;
+    const uint64_t cb = tile & ((((uint64_t)1 << a.lo) >> a.logw) - 1), hb = tile >> (a.lo - a.logw);
+    const uint64_t gbase = (hb << (lom + a.logh)) + (cb << a.logw);
+    const uint32_t colbase = (uint32_t)(cb << a.logw);
+    double *xt = a.x + gbase;
 
-    for (int e = threadIdx.x; e < T; e += NTTM_THREADS) {
-        const uint64_t gi = plan_gidx(g, tile, e);
-        double v = a.x[gi];
-        if (PW) {
-            uint64_t yi = a.ymode == NTTM_Y_FULL ? gi
-                        : a.ymode == NTTM_Y_BCAST ? (gi & nmask)
-                        : ((((gi >> a.logn) >> 1) << a.logn) | (gi & nmask));
-            v = mul_barrett(v, a.y[yi], p, pinv);
+    auto load_at = [&](uint32_t o, uint32_t c, uint32_t k) -> double {
+        double x = MODE == PM_FWD_LIMB ? input_limb(a, gbase + o, p, pinv) : xt[o];
+        if (MODE == PM_INV_PW) x = mul_barrett(x, a.y[y_index(a, gbase + o)], p, pinv);
+        if (INV && a.lo > 0) x = boundary_twiddle(x, colbase + c, k, lom, a.tw, p);
+        return x;
+    };
+    auto store_at = [&](uint32_t o, uint32_t c, uint32_t k, double x) {
+        if (!INV && a.lo > 0) x = boundary_twiddle(x, colbase + c, k, lom, a.tw, p);
+        if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
+        xt[o] = x;
+    };
+
+    double v[PTS];
+    int s_lo, logR;
+    /* first round: straight from global when its lanes cover runs of >= 16 consecutive elements; else (a contiguous
+     * pass, lo = 0, where the offset of element e is e) a coalesced linear load through LDS */
+    round_split(a.m, 0, &s_lo, &logR, INV);
+    if (a.logw + s_lo >= 4) {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t c,
+                                                                       uint32_t k) { v[sl] = load_at(o, c, k); });)
+    } else {
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; lds[e] = load_at(e, 0, 0); }
         }
-        if (INV && g.lo > 0) {
-            const uint32_t r = (e >> g.logw) & (M - 1);
-            v = twiddle_apply(v, plan_col(g, tile, e) * brev_m(r, g.m), lom, a.tw, p);
+        __syncthreads();
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { v[sl] = lds[e]; });)
+    }
+    for (int i = 0; i < nr; i++) {
+        round_split(a.m, i, &s_lo, &logR, INV);
+        if (i > 0) {
+            __syncthreads();
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                           uint32_t) { v[sl] = lds[e]; });)
         }
-        t[e] = v;
+        if (active) ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
+        if (i + 1 < nr && active)
+            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                               uint32_t) { lds[e] = v[sl]; });)
+    }
+    /* last round: the same choice for the store (each thread rewrites only the LDS words it read in this round) */
+    if (a.logw + s_lo >= 4) {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t c,
+                                                                       uint32_t k) { store_at(o, c, k, v[sl]); });)
+    } else {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { lds[e] = v[sl]; });)
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; store_at(e, 0, 0, lds[e]); }
+        }
+    }
+}
+
+/* ---- the fused last forward pass + pointwise product + first inverse pass ------------------------------------------------
+ * For the contiguous low-bit pass (lo = 0, m + logh <= 12): the DIF rounds of the forward transform, the product with
+ * y in a linear pass over LDS (y read coalesced), the DIT rounds of the inverse, one read and one write of x.
+ * LIMB: x comes from the operand limbs (a transform that is a single pass). */
+#define FUSED_LOG 12
+template <bool LIMB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(PassArgs a)
+{
+    __shared__ double xs[1 << FUSED_LOG];
+    PassPos pp; pp.m = a.m; pp.logw = 0; pp.lo = 0;
+    const int T = 1 << (a.m + a.logh);
+    pp.nthr = T / PASS_PTS; pp.t = threadIdx.x;
+    const bool active = pp.t < pp.nthr;
+    const double p = a.pk.p, pinv = a.pk.pinv;
+    const uint64_t gbase = (uint64_t)blockIdx.x << (a.m + a.logh);  /* lo = 0, W = 1: the tile is contiguous */
+    double *xt = a.x + gbase;
+    const double *yt = a.y + gbase;                                   /* FULL layout (the product path) */
+    const int nr = (a.m + 3) >> 2;
+    double v[PASS_PTS];
+    int s_lo, logR;
+    auto in_at = [&](uint32_t o) -> double { return LIMB ? input_limb(a, gbase + o, p, pinv) : xt[o]; };
+
+    round_split(a.m, 0, &s_lo, &logR, false);
+    if (s_lo >= 4) {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
+                                                                            uint32_t) { v[sl] = in_at(o); });)
+    } else {
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < PASS_PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; xs[e] = in_at(e); }
+        }
+        __syncthreads();
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { v[sl] = xs[e]; });)
+    }
+    for (int i = 0; i < nr; i++) {                               /* forward, top bits first */
+        round_split(a.m, i, &s_lo, &logR, false);
+        if (i > 0) {
+            __syncthreads();
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t,
+                                                                                uint32_t, uint32_t) { v[sl] = xs[e]; });)
+        }
+        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PASS_PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { xs[e] = v[sl]; });)
     }
     __syncthreads();
-
-    const int half_count = T >> 1;
-    if (!INV) {
-        for (int s = g.m - 1; s >= 0; --s) {
-            const int sh = g.logw + s, hs = 1 << s;
-            for (int j = threadIdx.x; j < half_count; j += NTTM_THREADS) {
-                const int e1 = ((j >> sh) << (sh + 1)) | (j & ((1 << sh) - 1)), e2 = e1 | (1 << sh);
-                const int r = (e1 >> g.logw) & (M - 1);
-                const double2 w = a.tw.stage[hs + (r & (hs - 1))];
-                const double u = t[e1], v = t[e2];
-                t[e1] = reduce(u + v, p, pinv);
-                t[e2] = mul_shoup(u - v, w.x, w.y, p);
-            }
-            __syncthreads();
-        }
-    } else {
-        for (int s = 0; s < g.m; ++s) {
-            const int sh = g.logw + s, hs = 1 << s;
-            for (int j = threadIdx.x; j < half_count; j += NTTM_THREADS) {
-                const int e1 = ((j >> sh) << (sh + 1)) | (j & ((1 << sh) - 1)), e2 = e1 | (1 << sh);
-                const int r = (e1 >> g.logw) & (M - 1);
-                const double2 w = a.tw.stage[hs + (r & (hs - 1))];
-                const double u = t[e1], v = mul_shoup(t[e2], w.x, w.y, p);
-                t[e1] = reduce(u + v, p, pinv);
-                t[e2] = reduce(u - v, p, pinv);
-            }
-            __syncthreads();
+    if (active) {                                                /* the pointwise product, linear: y read coalesced */
+#pragma unroll
+        for (int sl = 0; sl < PASS_PTS; sl++) {
+            const uint32_t e = sl * pp.nthr + pp.t;
+            xs[e] = mul_barrett(xs[e], yt[e], p, pinv);
         }
     }
-
-    for (int e = threadIdx.x; e < T; e += NTTM_THREADS) {
-        double v = t[e];
-        if (!INV && g.lo > 0) {
-            const uint32_t r = (e >> g.logw) & (M - 1);
-            v = twiddle_apply(v, plan_col(g, tile, e) * brev_m(r, g.m), lom, a.tw, p);
+    for (int i = 0; i < nr; i++) {                               /* inverse, bit 0 first */
+        round_split(a.m, i, &s_lo, &logR, true);
+        __syncthreads();
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { v[sl] = xs[e]; });)
+        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PASS_PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+        if (i + 1 < nr && active)
+            ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                    uint32_t) { xs[e] = v[sl]; });)
+    }
+    if (s_lo >= 4) {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
+                                                                            uint32_t) {
+            double x = v[sl];
+            if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
+            xt[o] = x; });)
+    } else {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { xs[e] = v[sl]; });)
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < PASS_PTS; sl++) {
+                const uint32_t e = sl * pp.nthr + pp.t;
+                double x = xs[e];
+                if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
+                xt[e] = x;
+            }
         }
-        if (a.do_scale) v = mul_shoup(v, a.scale.x, a.scale.y, p);
-        a.x[plan_gidx(g, tile, e)] = v;
     }
 }
 
 /* limbs -> residues of np primes, zero-extended to npoints; optional per-prime scale (n^-1 folded into one operand).
  * Batched form: product j of a length group takes points [j n, (j+1) n) of every plane, operand a or b of desc[j]. */
-struct LoadDesc {                /* must match api.hip */
-    const uint64_t *a; uint64_t na;
-    const uint64_t *b; uint64_t nb;
-};
 struct LoadArgs {
     double *dst;                 /* plane i at dst + i * plane */
     uint64_t plane;
@@ -165,6 +406,7 @@ static int pass_geom_ok(const PassGeom &g, int logn, uint64_t batch)
 {
     if (g.m < 1 || g.m > NTTM_TILE_LOG || g.logw < 0 || g.logh < 0 || g.m + g.logw + g.logh > NTTM_TILE_LOG) return 0;
     if (g.logw > g.lo || g.lo + g.m > logn) return 0;
+    if (g.lo + g.m + g.logh > 32) return 0;                       /* element offsets inside a tile fit 32 bits */
     const uint64_t total = batch << logn, T = (uint64_t)1 << (g.m + g.logw + g.logh);
     if (total % T) return 0;
     const uint64_t tiles = total / T;
@@ -179,14 +421,54 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     const uint64_t tiles = plan_tiles(g, a.logn, a.batch);
     if (tiles > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "internal: too many tiles");
     PassArgs b = a; b.lo = g.lo; b.m = g.m; b.logw = g.logw; b.logh = g.logh;
+    const bool limb = !inv && (b.limbs || b.ldesc);
+    const int mode = !inv ? (limb ? PM_FWD_LIMB : PM_FWD) : (b.y ? PM_INV_PW : PM_INV);
     const double pts = (double)(a.batch << a.logn);
-    /* algorithmic bytes: read and write the plane once (16 B per point), plus the y plane of a fused pointwise (8 B) */
-    prof_begin(!inv ? "k_pass<fwd>" : b.y ? "k_pass<inv,pw>" : "k_pass<inv>", pts * (b.y ? 24.0 : 16.0), s);
-    if (!inv) hipLaunchKernelGGL((k_pass<false, false>), dim3((unsigned)tiles), dim3(NTTM_THREADS), 0, s, b);
-    else if (b.y) hipLaunchKernelGGL((k_pass<true, true>), dim3((unsigned)tiles), dim3(NTTM_THREADS), 0, s, b);
-    else hipLaunchKernelGGL((k_pass<true, false>), dim3((unsigned)tiles), dim3(NTTM_THREADS), 0, s, b);
+    /* algorithmic bytes: read and write the plane once (16 B per point), plus the y plane of a fused pointwise (8 B);
+     * a limb-input pass reads the limbs (8 B per point of operand) instead of the plane */
+    const double bytes = mode == PM_FWD_LIMB ? pts * 8.0 + 8.0 * (double)(b.ldesc ? 0 : b.nlimbs)
+                       : pts * (mode == PM_INV_PW ? 24.0 : 16.0);
+    static const char *names[4] = {"k_pass<fwd>", "k_pass<fwd,limb>", "k_pass<inv>", "k_pass<inv,pw>"};
+    prof_begin(names[mode], bytes, s);
+    const int T = 1 << (g.m + g.logw + g.logh);
+    const unsigned nthr = (unsigned)(T / PASS_PTS < 64 ? 64 : T / PASS_PTS);
+    const dim3 grid((unsigned)tiles), blk(nthr);
+    const size_t lds = (size_t)T * sizeof(double);
+#define NTTM_PASS_LAUNCH(P) \
+    switch (mode) { \
+    case PM_FWD: hipLaunchKernelGGL((k_pass<PM_FWD, P>), grid, blk, lds, s, b); break; \
+    case PM_FWD_LIMB: hipLaunchKernelGGL((k_pass<PM_FWD_LIMB, P>), grid, blk, lds, s, b); break; \
+    case PM_INV: hipLaunchKernelGGL((k_pass<PM_INV, P>), grid, blk, lds, s, b); break; \
+    default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P>), grid, blk, lds, s, b); break; }
+    if (T >= PASS_PTS) { NTTM_PASS_LAUNCH(PASS_PTS) }
+    else if (T == 8) { NTTM_PASS_LAUNCH(8) }                   /* tiles of 2..8 points (transforms below 2^4, odd */
+    else if (T == 4) { NTTM_PASS_LAUNCH(4) }                   /* batch): one thread */
+    else { NTTM_PASS_LAUNCH(2) }
+#undef NTTM_PASS_LAUNCH
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_pass launch");
+}
+
+/* the fused last forward pass + pointwise + first inverse pass; g must be a contiguous pass (lo = 0) of at most 2^12
+ * points per tile and at least 16 */
+int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s)
+{
+    if (!pass_geom_ok(g, a.logn, a.batch) || g.lo != 0 || g.logw != 0 || g.m + g.logh > FUSED_LOG ||
+        g.m + g.logh < 4 || !a.y || a.ymode != NTTM_Y_FULL)
+        return nttm_fail(NTTM_EINVAL, "internal: bad fused pass geometry");
+    const uint64_t tiles = plan_tiles(g, a.logn, a.batch);
+    if (tiles > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "internal: too many tiles");
+    PassArgs b = a; b.lo = 0; b.m = g.m; b.logw = 0; b.logh = g.logh;
+    const bool limb = b.limbs || b.ldesc;
+    const double pts = (double)(a.batch << a.logn);
+    /* read x (or its limbs), read y, write x */
+    prof_begin(limb ? "k_fused<limb>" : "k_fused", pts * (limb ? 16.0 : 24.0), s);
+    const int T = 1 << (g.m + g.logh);
+    const unsigned nthr = (unsigned)(T / PASS_PTS < 64 ? 64 : T / PASS_PTS);
+    if (limb) hipLaunchKernelGGL((k_fused<true>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+    else hipLaunchKernelGGL((k_fused<false>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+    prof_end(s);
+    return nttm_hip_check(hipGetLastError(), "k_fused launch");
 }
 
 static void load_common(LoadArgs &a, int np, const PrimeK *pk, const double2 *scale)

@@ -18,21 +18,23 @@
 #endif
 
 #define NTTM_MAX_PASSES 6
-#define NTTM_STRIDED_MAX 9          /* bits of a strided pass: 2^9 rows x 16 columns fill a 2^13 tile */
+/* bits of a strided pass: at most tile_log - 4, so its tiles read runs of >= 16 columns (128 bytes) */
 
 struct PassGeom { int lo, m, logw, logh; };
 struct Plan { int logn, npass; PassGeom p[NTTM_MAX_PASSES]; };
 
 NTTM_PHD int plan_ctz64(uint64_t x) { int c = 0; if (!x) return 63; while (!(x & 1)) { x >>= 1; c++; } return c; }
 
-/* passes in forward order (highest bits first); tile_log = log2 of points per tile */
-NTTM_PHD void plan_make(Plan *pl, int logn, uint64_t batch, int tile_log)
+/* passes in forward order (highest bits first); tile_log = log2 of points per tile of the strided passes, last_log
+ * that of the last (contiguous) pass */
+NTTM_PHD void plan_make2(Plan *pl, int logn, uint64_t batch, int tile_log, int last_log)
 {
     int bz = plan_ctz64(batch);
     if (bz > 40) bz = 40;
-    int m0 = logn < tile_log ? logn : tile_log;
+    int m0 = logn < last_log ? logn : last_log;
     int R = logn - m0;
-    int k = (R + NTTM_STRIDED_MAX - 1) / NTTM_STRIDED_MAX;
+    const int smax = tile_log - 4;
+    int k = (R + smax - 1) / smax;
     pl->logn = logn; pl->npass = k + 1;
     int top = logn;
     for (int i = 0; i < k; i++) {
@@ -46,10 +48,11 @@ NTTM_PHD void plan_make(Plan *pl, int logn, uint64_t batch, int tile_log)
         top -= m;
     }
     PassGeom g; g.lo = 0; g.m = m0; g.logw = 0;
-    int room = tile_log - m0, blocks = (logn - m0) + bz;
+    int room = last_log - m0, blocks = (logn - m0) + bz;
     g.logh = room < blocks ? room : blocks;
     pl->p[k] = g;
 }
+NTTM_PHD void plan_make(Plan *pl, int logn, uint64_t batch, int tile_log) { plan_make2(pl, logn, batch, tile_log, tile_log); }
 
 NTTM_PHD uint64_t plan_tiles(const PassGeom &g, int logn, uint64_t batch)
 {

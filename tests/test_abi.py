@@ -69,13 +69,15 @@ def test_max_terms():
     assert ntt_amd.max_terms(3) >= 1 << 30                      # a 2^30-term product fits three primes
 
 
-def check_plan(logn, batch, full):
-    passes = ntt_amd.debug_plan(logn, batch)
+def check_plan(logn, batch, full, last_log=0, tile_log=13):
+    passes = ntt_amd.debug_plan(logn, batch, last_log, tile_log)
     assert sum(m for _, m, _, _ in passes) == logn
     top = logn
     total = batch << logn
     for lo, m, logw, logh in passes:                              # contiguous bit ranges, high to low
-        assert lo + m == top and m >= 1 and logw <= lo and m + logw + logh <= 13
+        assert lo + m == top and m >= 1 and logw <= lo and m + logw + logh <= (tile_log if lo else (last_log or tile_log))
+        if lo == 0 and last_log:
+            assert m + logh <= last_log
         if lo > 0 and logn >= 17:
             assert logw >= 4                                      # strided passes read >= 128-byte runs
         top = lo
@@ -98,10 +100,18 @@ def check_plan(logn, batch, full):
 
 @pytest.mark.parametrize("logn", list(range(1, 19)))
 @pytest.mark.parametrize("batch", [1, 2, 3, 8])
-def test_plan_covers_every_element_once(logn, batch):
-    check_plan(logn, batch, full=(batch << logn) <= (1 << 16))
+@pytest.mark.parametrize("tiles", [(13, 0), (13, 12), (12, 12)])
+def test_plan_covers_every_element_once(logn, batch, tiles):
+    check_plan(logn, batch, full=(batch << logn) <= (1 << 16), tile_log=tiles[0], last_log=tiles[1])
 
 
 @pytest.mark.parametrize("logn", [20, 21, 24, 27, 28, 30, 32])
-def test_plan_large_bounds(logn):
-    check_plan(logn, 1, full=False)
+@pytest.mark.parametrize("tiles", [(13, 0), (13, 12), (12, 12)])
+def test_plan_large_bounds(logn, tiles):
+    check_plan(logn, 1, full=False, tile_log=tiles[0], last_log=tiles[1])
+
+
+def test_product_plan_shape():
+    # the 2^28 transform of the 2^27-limb product: two strided 8-bit passes (32-column tiles) and a 12-bit last pass
+    assert ntt_amd.debug_plan(28, 1, 12, 13) == [(20, 8, 5, 0), (12, 8, 5, 0), (0, 12, 0, 0)]
+    assert ntt_amd.debug_plan(28, 1, 12, 12) == [(20, 8, 4, 0), (12, 8, 4, 0), (0, 12, 0, 0)]
