@@ -165,6 +165,7 @@ def main():
     ap.add_argument("--no-leaves", action="store_true")
     ap.add_argument("--leaf-reps", type=int, default=3)
     ap.add_argument("--tile-log", type=int, default=0, help="strided-pass tile, log2 points (12 or 13; 0 = default)")
+    ap.add_argument("--pts", type=int, default=0, help="transform points per thread (8 or 16; 0 = default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device to device once (the reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -180,7 +181,7 @@ def main():
     print("[bench] generating 2 x 2^%d limbs" % args.log_limbs, file=sys.stderr, flush=True)
     a = gen_limbs(n, 0x5EED0003)
     b = gen_limbs(n, 0x5EED0004)
-    nttm.init(0, tile_log=args.tile_log)
+    nttm.init(0, tile_log=args.tile_log, pts=args.pts)
     if args.copy_probe:
         src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)

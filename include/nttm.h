@@ -51,6 +51,7 @@ typedef struct nttm_config {
     int nprimes;                /* 0: choose per product (3 while the bound allows, else 4); 3 or 4: fixed */
     size_t workspace_bytes;     /* 0: grow on demand; else the cap of the library's device workspace */
     int tile_log;               /* points per tile of the strided transform passes, log2: 12 or 13 (0: default) */
+    int pts;                    /* transform points per thread: 8 or 16 (0: default) */
 } nttm_config;
 
 /* one product of a batch: c = a b (+ x).  c has na + nb limbs, or na + nb + 1 when x is given (nx <= na + nb),

@@ -36,7 +36,7 @@ class KStat(C.Structure):
 
 
 class Config(C.Structure):
-    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t), ("tile_log", C.c_int)]
+    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t), ("tile_log", C.c_int), ("pts", C.c_int)]
 
 
 _lib = None
@@ -118,8 +118,8 @@ def debug_tile_indices(lo, m, logw, logh, tile):
 
 
 # ---- device entry points ------------------------------------------------------------------------------------------------
-def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0):
-    cfg = Config(nprimes, workspace_bytes, tile_log)
+def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0, pts=0):
+    cfg = Config(nprimes, workspace_bytes, tile_log, pts)
     _check(lib().nttm_init(device, C.byref(cfg)))
 
 

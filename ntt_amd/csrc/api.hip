@@ -12,8 +12,11 @@
 
 int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s);
 int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s);
+int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hipStream_t s);
+void nttm_set_pts(int pts);
+void nttm_set_fixed_shapes(int on);
 #define NTTM_FUSED_LOG 12          /* tile of the fused forward + pointwise + inverse pass (k_fused) */
-#define NTTM_TILE_LOG_DEFAULT 12   /* tile of the strided passes: 2^12 points (32 KiB of LDS, 4 blocks per CU) */
+#define NTTM_TILE_LOG_DEFAULT 13   /* tile of the strided passes: 2^13 points (64 KiB of LDS), runs of 32 columns */
 #define NTTM_PRODUCT_LOGN_MIN 4    /* products use at least 2^4 points, so every fused tile has >= 16 */
 struct LoadArgs;
 int nttm_run_u64_to_f64(double *x, uint64_t count, hipStream_t s);
@@ -204,6 +207,12 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
         return nttm_fail(NTTM_EINVAL, "nttm_init: nprimes must be 0, 3 or 4");
     if (!g.cfg.tile_log) g.cfg.tile_log = NTTM_TILE_LOG_DEFAULT;
     if (g.cfg.tile_log != 12 && g.cfg.tile_log != 13) return nttm_fail(NTTM_EINVAL, "nttm_init: tile_log must be 12 or 13");
+    if (g.cfg.pts && g.cfg.pts != 8 && g.cfg.pts != 16) return nttm_fail(NTTM_EINVAL, "nttm_init: pts must be 0, 8 or 16");
+    nttm_set_pts(g.cfg.pts ? g.cfg.pts : 16);
+    {   /* NTTM_GENERIC_PASSES=1: runtime-geometry pass kernels only (for comparison) */
+        const char *e = getenv("NTTM_GENERIC_PASSES");
+        nttm_set_fixed_shapes(!(e && e[0] == '1'));
+    }
     int nd = 0;
     HIPCK(hipGetDeviceCount(&nd));
     if (device < 0 || device >= nd) return nttm_fail(NTTM_EINVAL, "nttm_init: device %d of %d", device, nd);
@@ -307,13 +316,14 @@ static int product_transforms(int np, int logn, uint64_t cnt, double *A, double 
 {
     Plan pl; plan_make2(&pl, logn, cnt, product_tile_log(logn), NTTM_FUSED_LOG);
     const uint64_t n = (uint64_t)1 << logn;
-    for (int i = 0; i < np; i++) {                               /* B, every prime */
+    const int last = pl.npass - 1;
+    for (int i = 0; i < np; i++) {                               /* B, every prime, all passes but the last */
         double *Bi = Bp + (size_t)i * cnt * n;
         const uint64_t p = NTTM_PRIMES[i];
         PassArgs fb = base_args(i, Bi, logn, cnt, false);
         fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
         fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
-        for (int j = 0; j < pl.npass; j++) {
+        for (int j = 0; j < last; j++) {
             RCK(nttm_run_pass(false, fb, pl.p[j], s));
             fb.limbs = nullptr; fb.ldesc = nullptr; fb.do_in_scale = 0;
         }
@@ -321,16 +331,22 @@ static int product_transforms(int np, int logn, uint64_t cnt, double *A, double 
     if (mid) HIPCK(hipEventRecord(mid, s));
     for (int i = 0; i < np; i++) {                               /* A, every prime */
         double *Ai = A + (size_t)i * cnt * n, *Bi = Bp + (size_t)i * cnt * n;
+        const uint64_t p = NTTM_PRIMES[i];
         PassArgs fa = base_args(i, Ai, logn, cnt, false);
         fa.limbs = ldesc ? nullptr : a; fa.nlimbs = na; fa.ldesc = ldesc; fa.which = 0;
-        for (int j = 0; j + 1 < pl.npass; j++) {
+        for (int j = 0; j < last; j++) {
             RCK(nttm_run_pass(false, fa, pl.p[j], s));
             fa.limbs = nullptr; fa.ldesc = nullptr;
         }
-        fa.y = Bi; fa.ymode = NTTM_Y_FULL;
-        RCK(nttm_run_fused(fa, pl.p[pl.npass - 1], s));
+        /* B's last forward pass, A's last forward pass, the pointwise product and A's first inverse pass */
+        PassArgs fb = base_args(i, Bi, logn, cnt, false);
+        if (last == 0) {                                         /* a single-pass transform reads B's limbs here */
+            fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
+            fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
+        }
+        RCK(nttm_run_fused2(fa, fb, pl.p[last], s));
         PassArgs ia = base_args(i, Ai, logn, cnt, true);
-        for (int j = pl.npass - 2; j >= 0; j--) RCK(nttm_run_pass(true, ia, pl.p[j], s));
+        for (int j = last - 1; j >= 0; j--) RCK(nttm_run_pass(true, ia, pl.p[j], s));
     }
     return 0;
 }

@@ -8,6 +8,8 @@
 #include <string.h>
 #include "internal.h"
 #include "plan.h"
+/* the round loops carry an unroll request that applies to the fixed-shape instantiations only */
+#pragma clang diagnostic ignored "-Wpass-failed"
 
 
 /* ---- the pass kernel -------------------------------------------------------------------------------------------------
@@ -125,10 +127,12 @@ __device__ __forceinline__ void round_compute(double *v, const PassPos &pp, int 
     }
 }
 
-/* rounds of a pass: sizes of at most 4 bits, balanced; forward from the top bits down, inverse from bit 0 up */
-__device__ __forceinline__ void round_split(int m, int i, int *s_lo, int *logR, bool inv)
+/* rounds of a pass: sizes of at most maxr bits (the log2 of the points per thread, at most 4), balanced; forward from
+ * the top bits down, inverse from bit 0 up */
+__device__ __forceinline__ int round_count(int m, int maxr) { return (m + maxr - 1) / maxr; }
+__device__ __forceinline__ void round_split(int m, int i, int *s_lo, int *logR, bool inv, int maxr)
 {
-    const int nr = (m + 3) >> 2, q = m / nr, rem = m % nr;
+    const int nr = round_count(m, maxr), q = m / nr, rem = m % nr;
     int lo = 0;                                     /* rounds in ascending bit order: sizes q+1 first (rem of them) */
     int k = inv ? i : nr - 1 - i;
     for (int r = 0; r < k; r++) lo += q + (r < rem ? 1 : 0);
@@ -175,22 +179,26 @@ __device__ __forceinline__ double boundary_twiddle(double v, uint32_t col, uint3
     return mul_shoup(v, l.x, l.y, p);
 }
 
-template <int MODE, int PTS>
+/* GM, GW, GH >= 0 fix the tile shape (m, logw, logh) at compile time: round splits, LDS strides and shifts fold to
+ * constants; -1 takes them from the arguments */
+template <int MODE, int PTS, int GM = -1, int GW = -1, int GH = -1>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_pass(PassArgs a)
 {
+    const int gm = GM >= 0 ? GM : a.m, gw = GW >= 0 ? GW : a.logw, gh = GH >= 0 ? GH : a.logh;
     constexpr bool INV = MODE >= PM_INV;
     extern __shared__ double lds[];                             /* T doubles, sized at launch */
-    PassPos pp; pp.m = a.m; pp.logw = a.logw; pp.lo = a.lo;
-    const int T = 1 << (a.m + a.logw + a.logh);
+    PassPos pp; pp.m = gm; pp.logw = gw; pp.lo = a.lo;
+    const int T = 1 << (gm + gw + gh);
     pp.nthr = T / PTS; pp.t = threadIdx.x;
     const bool active = pp.t < pp.nthr;
     const double p = a.pk.p, pinv = a.pk.pinv;
-    const int lom = a.lo + a.m, nr = (a.m + 3) >> 2;
+    constexpr int MAXR = PTS >= 16 ? 4 : PTS == 8 ? 3 : PTS == 4 ? 2 : 1;
+    const int lom = a.lo + gm, nr = round_count(gm, MAXR);
     /* the tile: column block cb (W columns below bit lo) of block group hb (H blocks of 2^lom points) */
     const uint64_t tile = blockIdx.x;
-    const uint64_t cb = tile & ((((uint64_t)1 << a.lo) >> a.logw) - 1), hb = tile >> (a.lo - a.logw);
-    const uint64_t gbase = (hb << (lom + a.logh)) + (cb << a.logw);
-    const uint32_t colbase = (uint32_t)(cb << a.logw);
+    const uint64_t cb = tile & ((((uint64_t)1 << a.lo) >> gw) - 1), hb = tile >> (a.lo - gw);
+    const uint64_t gbase = (hb << (lom + gh)) + (cb << gw);
+    const uint32_t colbase = (uint32_t)(cb << gw);
     double *xt = a.x + gbase;
 
     auto load_at = [&](uint32_t o, uint32_t c, uint32_t k) -> double {
@@ -209,8 +217,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     int s_lo, logR;
     /* first round: straight from global when its lanes cover runs of >= 16 consecutive elements; else (a contiguous
      * pass, lo = 0, where the offset of element e is e) a coalesced linear load through LDS */
-    round_split(a.m, 0, &s_lo, &logR, INV);
-    if (a.logw + s_lo >= 4) {
+    round_split(gm, 0, &s_lo, &logR, INV, MAXR);
+    if (gw + s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t c,
                                                                        uint32_t k) { v[sl] = load_at(o, c, k); });)
     } else {
@@ -222,8 +230,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                        uint32_t) { v[sl] = lds[e]; });)
     }
-    for (int i = 0; i < nr; i++) {
-        round_split(a.m, i, &s_lo, &logR, INV);
+#pragma unroll
+    for (int i = 0; i < (GM >= 0 ? (GM + MAXR - 1) / MAXR : nr); i++) {
+        round_split(gm, i, &s_lo, &logR, INV, MAXR);
         if (i > 0) {
             __syncthreads();
             if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
@@ -235,7 +244,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                                uint32_t) { lds[e] = v[sl]; });)
     }
     /* last round: the same choice for the store (each thread rewrites only the LDS words it read in this round) */
-    if (a.logw + s_lo >= 4) {
+    if (gw + s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t c,
                                                                        uint32_t k) { store_at(o, c, k, v[sl]); });)
     } else {
@@ -254,83 +263,182 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
  * y in a linear pass over LDS (y read coalesced), the DIT rounds of the inverse, one read and one write of x.
  * LIMB: x comes from the operand limbs (a transform that is a single pass). */
 #define FUSED_LOG 12
-template <bool LIMB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(PassArgs a)
+template <bool LIMB, int PTS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(PassArgs a)
 {
     __shared__ double xs[1 << FUSED_LOG];
+    constexpr int MAXR = PTS >= 16 ? 4 : 3;
     PassPos pp; pp.m = a.m; pp.logw = 0; pp.lo = 0;
     const int T = 1 << (a.m + a.logh);
-    pp.nthr = T / PASS_PTS; pp.t = threadIdx.x;
+    pp.nthr = T / PTS; pp.t = threadIdx.x;
     const bool active = pp.t < pp.nthr;
     const double p = a.pk.p, pinv = a.pk.pinv;
     const uint64_t gbase = (uint64_t)blockIdx.x << (a.m + a.logh);  /* lo = 0, W = 1: the tile is contiguous */
     double *xt = a.x + gbase;
     const double *yt = a.y + gbase;                                   /* FULL layout (the product path) */
-    const int nr = (a.m + 3) >> 2;
-    double v[PASS_PTS];
+    const int nr = round_count(a.m, MAXR);
+    double v[PTS];
     int s_lo, logR;
     auto in_at = [&](uint32_t o) -> double { return LIMB ? input_limb(a, gbase + o, p, pinv) : xt[o]; };
 
-    round_split(a.m, 0, &s_lo, &logR, false);
+    round_split(a.m, 0, &s_lo, &logR, false, MAXR);
     if (s_lo >= 4) {
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
                                                                             uint32_t) { v[sl] = in_at(o); });)
     } else {
         if (active) {
 #pragma unroll
-            for (int sl = 0; sl < PASS_PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; xs[e] = in_at(e); }
+            for (int sl = 0; sl < PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; xs[e] = in_at(e); }
         }
         __syncthreads();
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                             uint32_t) { v[sl] = xs[e]; });)
     }
     for (int i = 0; i < nr; i++) {                               /* forward, top bits first */
-        round_split(a.m, i, &s_lo, &logR, false);
+        round_split(a.m, i, &s_lo, &logR, false, MAXR);
         if (i > 0) {
             __syncthreads();
-            if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t,
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t,
                                                                                 uint32_t, uint32_t) { v[sl] = xs[e]; });)
         }
-        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PASS_PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                             uint32_t) { xs[e] = v[sl]; });)
     }
     __syncthreads();
     if (active) {                                                /* the pointwise product, linear: y read coalesced */
 #pragma unroll
-        for (int sl = 0; sl < PASS_PTS; sl++) {
+        for (int sl = 0; sl < PTS; sl++) {
             const uint32_t e = sl * pp.nthr + pp.t;
             xs[e] = mul_barrett(xs[e], yt[e], p, pinv);
         }
     }
     for (int i = 0; i < nr; i++) {                               /* inverse, bit 0 first */
-        round_split(a.m, i, &s_lo, &logR, true);
+        round_split(a.m, i, &s_lo, &logR, true, MAXR);
         __syncthreads();
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                             uint32_t) { v[sl] = xs[e]; });)
-        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PASS_PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
         if (i + 1 < nr && active)
-            ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                     uint32_t) { xs[e] = v[sl]; });)
     }
     if (s_lo >= 4) {
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
                                                                             uint32_t) {
             double x = v[sl];
             if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
             xt[o] = x; });)
     } else {
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PASS_PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                             uint32_t) { xs[e] = v[sl]; });)
         __syncthreads();
         if (active) {
 #pragma unroll
-            for (int sl = 0; sl < PASS_PTS; sl++) {
+            for (int sl = 0; sl < PTS; sl++) {
                 const uint32_t e = sl * pp.nthr + pp.t;
                 double x = xs[e];
                 if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
                 xt[e] = x;
             }
+        }
+    }
+}
+
+/* ---- the double-fused last pass: B's last forward pass + A's last forward pass + pointwise + A's first inverse pass ---
+ * Same geometry as k_fused (lo = 0, m + logh <= 12).  The forward's last round and the inverse's first round cover the
+ * same bits [0, logR) with the same slot layout, so B's transformed tile stays in registers (bv) and is never written;
+ * the product v * bv and the first inverse round follow A's last forward round without an LDS round trip.
+ * One read of A (or its limbs), one read of B (or its limbs), one write of A. */
+template <int PTS, int MAXR>
+__device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &pp, bool active, uint64_t gbase,
+                                             double *xs, double *v, double p, double pinv)
+{
+    const bool limb = q.limbs || q.ldesc;
+    const double *xt = q.x + gbase;
+    const int nr = round_count(q.m, MAXR);
+    int s_lo, logR;
+    round_split(q.m, 0, &s_lo, &logR, false, MAXR);
+    if (s_lo >= 4) {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
+                                                                       uint32_t) {
+            v[sl] = limb ? input_limb(q, gbase + o, p, pinv) : xt[o]; });)
+    } else {
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < PTS; sl++) {
+                const uint32_t e = sl * pp.nthr + pp.t;
+                xs[e] = limb ? input_limb(q, gbase + e, p, pinv) : xt[e];
+            }
+        }
+        __syncthreads();
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { v[sl] = xs[e]; });)
+    }
+#pragma unroll
+    for (int i = 0; i < nr; i++) {                               /* top bits first; the last round stays in v */
+        round_split(q.m, i, &s_lo, &logR, false, MAXR);
+        if (i > 0) {
+            __syncthreads();
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                           uint32_t) { v[sl] = xs[e]; });)
+        }
+        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, q.tw.stage, p, pinv);)
+        if (i + 1 < nr && active)
+            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                               uint32_t) { xs[e] = v[sl]; });)
+    }
+}
+
+template <int PTS, int GM = -1, int GH = -1>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused2(PassArgs a, PassArgs bq)
+{
+    __shared__ double xs[1 << FUSED_LOG];
+    constexpr int MAXR = PTS >= 16 ? 4 : 3;
+    if (GM >= 0) { a.m = GM; bq.m = GM; }
+    if (GH >= 0) { a.logh = GH; bq.logh = GH; }
+    PassPos pp; pp.m = a.m; pp.logw = 0; pp.lo = 0;
+    const int T = 1 << (a.m + a.logh);
+    pp.nthr = T / PTS; pp.t = threadIdx.x;
+    const bool active = pp.t < pp.nthr;
+    const double p = a.pk.p, pinv = a.pk.pinv;
+    const uint64_t gbase = (uint64_t)blockIdx.x << (a.m + a.logh);
+    const int nr = round_count(a.m, MAXR);
+    double *xt = a.x + gbase;
+    double v[PTS], bv[PTS];
+    int s_lo, logR;
+
+    tile_forward<PTS, MAXR>(bq, pp, active, gbase, xs, bv, p, pinv);
+    __syncthreads();                                             /* B's last LDS reads before A's first writes */
+    tile_forward<PTS, MAXR>(a, pp, active, gbase, xs, v, p, pinv);
+    if (active) {
+#pragma unroll
+        for (int sl = 0; sl < PTS; sl++) v[sl] = mul_barrett(v[sl], bv[sl], p, pinv);
+    }
+#pragma unroll
+    for (int i = 0; i < nr; i++) {                               /* inverse, bit 0 first; round 0 is the layout in v */
+        round_split(a.m, i, &s_lo, &logR, true, MAXR);
+        if (i > 0) {
+            __syncthreads();
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                           uint32_t) { v[sl] = xs[e]; });)
+        }
+        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+        if (i + 1 < nr && active)
+            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                               uint32_t) { xs[e] = v[sl]; });)
+    }
+    if (s_lo >= 4) {
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
+                                                                       uint32_t) { xt[o] = v[sl]; });)
+    } else {
+        /* each thread rewrites only the LDS words it read in this round */
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { xs[e] = v[sl]; });)
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int sl = 0; sl < PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; xt[e] = xs[e]; }
         }
     }
 }
@@ -415,6 +523,11 @@ static int pass_geom_ok(const PassGeom &g, int logn, uint64_t batch)
     return 1;
 }
 
+static int g_pts = PASS_PTS;                                   /* points per thread of full tiles: 16 or 8 */
+void nttm_set_pts(int pts) { g_pts = pts == 8 ? 8 : PASS_PTS; }
+static int g_fixed_shapes = 1;                                  /* use the fixed-shape pass kernels where they apply */
+void nttm_set_fixed_shapes(int on) { g_fixed_shapes = on != 0; }
+
 int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
 {
     if (!pass_geom_ok(g, a.logn, a.batch)) return nttm_fail(NTTM_EINVAL, "internal: bad pass geometry");
@@ -431,7 +544,10 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     static const char *names[4] = {"k_pass<fwd>", "k_pass<fwd,limb>", "k_pass<inv>", "k_pass<inv,pw>"};
     prof_begin(names[mode], bytes, s);
     const int T = 1 << (g.m + g.logw + g.logh);
-    const unsigned nthr = (unsigned)(T / PASS_PTS < 64 ? 64 : T / PASS_PTS);
+    /* 8 points per thread only where the tile needs between 64 and 512 threads (the kernel's launch bound) */
+    const int ppt = (T >= 64 * g_pts && T <= 512 * g_pts) ? g_pts : PASS_PTS;
+    const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
+    if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
     const dim3 grid((unsigned)tiles), blk(nthr);
     const size_t lds = (size_t)T * sizeof(double);
 #define NTTM_PASS_LAUNCH(P) \
@@ -440,11 +556,22 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     case PM_FWD_LIMB: hipLaunchKernelGGL((k_pass<PM_FWD_LIMB, P>), grid, blk, lds, s, b); break; \
     case PM_INV: hipLaunchKernelGGL((k_pass<PM_INV, P>), grid, blk, lds, s, b); break; \
     default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P>), grid, blk, lds, s, b); break; }
-    if (T >= PASS_PTS) { NTTM_PASS_LAUNCH(PASS_PTS) }
+#define NTTM_PASS_LAUNCH_G(P, M, W, H) \
+    switch (mode) { \
+    case PM_FWD: hipLaunchKernelGGL((k_pass<PM_FWD, P, M, W, H>), grid, blk, lds, s, b); break; \
+    case PM_FWD_LIMB: hipLaunchKernelGGL((k_pass<PM_FWD_LIMB, P, M, W, H>), grid, blk, lds, s, b); break; \
+    case PM_INV: hipLaunchKernelGGL((k_pass<PM_INV, P, M, W, H>), grid, blk, lds, s, b); break; \
+    default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P, M, W, H>), grid, blk, lds, s, b); break; }
+    /* the strided tile shapes of large transforms, compiled with their geometry fixed */
+    const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
+    if (fixed && ppt == PASS_PTS && g.m == 8 && g.logw == 5) { NTTM_PASS_LAUNCH_G(PASS_PTS, 8, 5, 0) }
+    else if (T >= PASS_PTS && ppt == 8) { NTTM_PASS_LAUNCH(8) }
+    else if (T >= PASS_PTS) { NTTM_PASS_LAUNCH(PASS_PTS) }
     else if (T == 8) { NTTM_PASS_LAUNCH(8) }                   /* tiles of 2..8 points (transforms below 2^4, odd */
     else if (T == 4) { NTTM_PASS_LAUNCH(4) }                   /* batch): one thread */
     else { NTTM_PASS_LAUNCH(2) }
 #undef NTTM_PASS_LAUNCH
+#undef NTTM_PASS_LAUNCH_G
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_pass launch");
 }
@@ -464,11 +591,45 @@ int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s)
     /* read x (or its limbs), read y, write x */
     prof_begin(limb ? "k_fused<limb>" : "k_fused", pts * (limb ? 16.0 : 24.0), s);
     const int T = 1 << (g.m + g.logh);
-    const unsigned nthr = (unsigned)(T / PASS_PTS < 64 ? 64 : T / PASS_PTS);
-    if (limb) hipLaunchKernelGGL((k_fused<true>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
-    else hipLaunchKernelGGL((k_fused<false>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+    const int ppt = (T >= 64 * g_pts && T <= 512 * g_pts) ? g_pts : PASS_PTS;
+    const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
+    if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
+    if (ppt == 8) {
+        if (limb) hipLaunchKernelGGL((k_fused<true, 8>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+        else hipLaunchKernelGGL((k_fused<false, 8>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+    } else {
+        if (limb) hipLaunchKernelGGL((k_fused<true, PASS_PTS>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+        else hipLaunchKernelGGL((k_fused<false, PASS_PTS>), dim3((unsigned)tiles), dim3(nthr), 0, s, b);
+    }
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_fused launch");
+}
+
+/* the double-fused last pass: a = A's pass arguments (x = A's plane), bq = B's (x = B's plane before its last pass) */
+int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hipStream_t s)
+{
+    if (!pass_geom_ok(g, a.logn, a.batch) || g.lo != 0 || g.logw != 0 || g.m + g.logh > FUSED_LOG ||
+        g.m + g.logh < 4 || !bq.x || bq.logn != a.logn || bq.batch != a.batch || a.do_scale)
+        return nttm_fail(NTTM_EINVAL, "internal: bad double-fused pass geometry");
+    const uint64_t tiles = plan_tiles(g, a.logn, a.batch);
+    if (tiles > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "internal: too many tiles");
+    PassArgs A = a; A.lo = 0; A.m = g.m; A.logw = 0; A.logh = g.logh;
+    PassArgs Bq = bq; Bq.lo = 0; Bq.m = g.m; Bq.logw = 0; Bq.logh = g.logh;
+    const double pts = (double)(a.batch << a.logn);
+    const bool alimb = A.limbs || A.ldesc, blimb = Bq.limbs || Bq.ldesc;
+    /* read A (or its limbs), read B (or its limbs), write A */
+    prof_begin("k_fused2", pts * (24.0 - (alimb ? 8.0 : 0.0) - (blimb ? 8.0 : 0.0)), s);
+    const int T = 1 << (g.m + g.logh);
+    const int ppt = (T >= 64 * g_pts && T <= 512 * g_pts) ? g_pts : PASS_PTS;
+    const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
+    if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
+    const bool fixed = g_fixed_shapes && g.m == FUSED_LOG && g.logh == 0;
+    if (ppt == 8 && fixed) hipLaunchKernelGGL((k_fused2<8, FUSED_LOG, 0>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    else if (ppt == 8) hipLaunchKernelGGL((k_fused2<8>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    else if (fixed) hipLaunchKernelGGL((k_fused2<PASS_PTS, FUSED_LOG, 0>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    else hipLaunchKernelGGL((k_fused2<PASS_PTS>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    prof_end(s);
+    return nttm_hip_check(hipGetLastError(), "k_fused2 launch");
 }
 
 static void load_common(LoadArgs &a, int np, const PrimeK *pk, const double2 *scale)

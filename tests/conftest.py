@@ -20,7 +20,7 @@ def nttm():
     import ntt_amd
     assert torch.cuda.is_available(), "GPU tests need a GPU"
     torch.cuda.init()
-    ntt_amd.init(0)
+    ntt_amd.init(0, tile_log=int(os.environ.get("NTTM_TEST_TILE_LOG", "0")), pts=int(os.environ.get("NTTM_TEST_PTS", "0")))
     yield ntt_amd
     torch.cuda.synchronize()
     ntt_amd.shutdown()
