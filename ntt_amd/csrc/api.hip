@@ -15,6 +15,7 @@ int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s);
 int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hipStream_t s);
 void nttm_set_pts(int pts);
 void nttm_set_fixed_shapes(int on);
+void nttm_set_stream(int on, int ncu);
 #define NTTM_FUSED_LOG 12          /* tile of the fused forward + pointwise + inverse pass (k_fused) */
 #define NTTM_TILE_LOG_DEFAULT 13   /* tile of the strided passes: 2^13 points (64 KiB of LDS), runs of 32 columns */
 #define NTTM_PRODUCT_LOGN_MIN 4    /* products use at least 2^4 points, so every fused tile has >= 16 */
@@ -145,7 +146,8 @@ static void build_crt_const(CrtConst &cc)
 static int build_tables(void)
 {
     const size_t nst = (size_t)1 << NTTM_STAGE_LOG, nbt = (size_t)1 << NTTM_BT_LOG;
-    const size_t per_dir = nst + 2 * nbt, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
+    const size_t nbq = (size_t)4 << NTTM_BQ_LOG;
+    const size_t per_dir = nst + 2 * nbt + nbq, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
     std::vector<double2> h(total);
     for (int i = 0; i < NTTM_MAX_PRIMES; i++) {
         const uint64_t p = NTTM_PRIMES[i];
@@ -153,7 +155,7 @@ static int build_tables(void)
         g.root32[i] = h_powmod(gen, (p - 1) >> 32, p);
         for (int dir = 0; dir < 2; dir++) {
             const uint64_t w32 = dir ? h_inv(g.root32[i], p) : g.root32[i];
-            double2 *st = &h[((size_t)i * 2 + dir) * per_dir], *bh = st + nst, *bl = bh + nbt;
+            double2 *st = &h[((size_t)i * 2 + dir) * per_dir], *bh = st + nst, *bl = bh + nbt, *bq = bl + nbt;
             st[0] = h_shoup(1, p);
             for (int s = 0; s + 1 < NTTM_STAGE_LOG + 1 && ((size_t)1 << s) < nst; s++) {
                 const uint64_t ws = h_powmod(w32, (uint64_t)1 << (32 - (s + 1)), p);   /* order 2^(s+1) */
@@ -166,6 +168,13 @@ static int build_tables(void)
                 bh[j] = h_shoup(vh, p); vh = h_mulmod(vh, w16, p);
                 bl[j] = h_shoup(vl, p); vl = h_mulmod(vl, w32, p);
             }
+            for (int k = 0; k < 4; k++) {                                /* bq[k 2^8 + x] = w32^(x 2^(8k)) */
+                const uint64_t wk = h_powmod(w32, (uint64_t)1 << (NTTM_BQ_LOG * k), p);
+                uint64_t v = 1;
+                for (size_t x = 0; x < ((size_t)1 << NTTM_BQ_LOG); x++) {
+                    bq[((size_t)k << NTTM_BQ_LOG) + x] = h_shoup(v, p); v = h_mulmod(v, wk, p);
+                }
+            }
         }
     }
     HIPCK(hipMalloc(&g.tables, total * sizeof(double2)));
@@ -173,7 +182,7 @@ static int build_tables(void)
     for (int i = 0; i < NTTM_MAX_PRIMES; i++)
         for (int dir = 0; dir < 2; dir++) {
             const double2 *st = g.tables + ((size_t)i * 2 + dir) * per_dir;
-            TwTables t; t.stage = st; t.bh = st + nst; t.bl = st + nst + nbt;
+            TwTables t; t.stage = st; t.bh = st + nst; t.bl = st + nst + nbt; t.bq = st + nst + 2 * nbt;
             if (dir) g.twi[i] = t; else g.twf[i] = t;
         }
     return 0;
@@ -218,6 +227,12 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     if (device < 0 || device >= nd) return nttm_fail(NTTM_EINVAL, "nttm_init: device %d of %d", device, nd);
     HIPCK(hipSetDevice(device));
     g.dev = device;
+    {   /* NTTM_STREAM=0: strided passes without the LDS-streaming kernel (for comparison); one streaming block per CU */
+        const char *st = getenv("NTTM_STREAM");
+        int ncu = 0;
+        HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        nttm_set_stream(!(st && st[0] == '0'), ncu);
+    }
     for (int i = 0; i < NTTM_MAX_PRIMES; i++) { g.pk[i].p = (double)NTTM_PRIMES[i]; g.pk[i].pinv = 1.0 / (double)NTTM_PRIMES[i]; g.pk[i].pu = NTTM_PRIMES[i]; }
     build_crt_const(g.cc);
     RCK(build_tables());

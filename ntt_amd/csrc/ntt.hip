@@ -258,6 +258,166 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
+/* ---- the streaming strided pass ---------------------------------------------------------------------------------------
+ * Strided passes (lo >= GW) of the fixed tile shape 2^GM rows x 2^GW columns (H = 1, 2^13 points, 16 per thread): one
+ * block per CU walks the tiles blockIdx.x, + gridDim.x, ...  The next tile is fetched straight into the other of two
+ * LDS buffers (global_load_lds, 16 bytes per lane, the tile in natural row-major order) while the current one is
+ * transformed, so HBM streams during the butterflies.  Vector-memory operations of a wave complete in issue order, so
+ * the loop issues no global load it must wait on besides the fetch:
+ *   - the stage twiddles (entries below 2^GM) and the four-level boundary table (4 x 2^8 entries) are copied into LDS
+ *     once per block;
+ *   - the boundary twiddles w^(c k_j) of a thread's 16 points share the column c and have k_j = kb + i_j 2^(GM-4)
+ *     (i_j = brev4(j)), so they are base * step^i with base = w^(c kb), step = w^(c 2^(GM-4)), each the product of four
+ *     LDS table entries, and the powers a chain of Shoup products by step (exact integers, no error growth);
+ *   - at the top of an iteration at most the previous tile's 16 stores may remain outstanding (vmcnt 16), which
+ *     leaves exactly the fetch of the tile about to be transformed complete.
+ * The boundary twiddle is applied as a Barrett product (both factors variable, |x|, |w| <= p), the forward on the
+ * store (after the last round, bits [0, 4)), the inverse on the load (before the first round, the same bits). */
+#define STREAM_T 512
+/* a workgroup barrier for LDS only: the wave's LDS operations complete, then s_barrier.  The compiler places a wait
+ * for every outstanding vector-memory operation (an LDS fetch included) before a barrier it sees; written as one asm
+ * statement (with a memory clobber, so no memory access moves across it) the fetch in flight stays in flight. */
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+/* w^E for a 32-bit exponent fraction E from the four-level table in LDS, balanced (|result| <= p/2 + 1) */
+__device__ __forceinline__ double bq_twiddle(const double2 *bq, uint32_t E, double p, double pinv)
+{
+    constexpr uint32_t M = (1u << NTTM_BQ_LOG) - 1;
+    double x = bq[(3 << NTTM_BQ_LOG) + (E >> 24)].x;
+    const double2 t2 = bq[(2 << NTTM_BQ_LOG) + ((E >> 16) & M)];
+    const double2 t1 = bq[(1 << NTTM_BQ_LOG) + ((E >> 8) & M)];
+    const double2 t0 = bq[E & M];
+    x = mul_shoup(x, t2.x, t2.y, p);
+    x = mul_shoup(x, t1.x, t1.y, p);
+    x = mul_shoup(x, t0.x, t0.y, p);
+    return reduce(x, p, pinv);
+}
+
+template <int MODE, int GM, int GW>
+__global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t ntiles)
+{
+    constexpr bool INV = MODE == PM_INV, LIMB = MODE == PM_FWD_LIMB;
+    constexpr int PTS = 16, T = 1 << (GM + GW), W = 1 << GW, NBQ = 4 << NTTM_BQ_LOG;
+    static_assert(T == PTS * STREAM_T, "2^13-point tiles, 16 points per thread");
+    static_assert(GM == 8, "two rounds of 4 levels");
+    __shared__ double buf0[T], buf1[T];
+    __shared__ double2 stw[1 << GM], bqs[NBQ];
+    const int t = threadIdx.x;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane((uint32_t)t >> 6);
+    const double p = a.pk.p, pinv = a.pk.pinv;
+    const int lo = a.lo, lom = a.lo + GM;
+    const uint32_t ncb_mask = (uint32_t)((((uint64_t)1 << lo) >> GW) - 1);
+    PassPos pp; pp.m = GM; pp.logw = GW; pp.lo = lo; pp.nthr = STREAM_T; pp.t = t;
+
+    /* the thread's group in the round over bits [0, 4): column c, bit-reversed row kb of slot 0 */
+    const uint32_t c_in = (uint32_t)t & (W - 1);
+    const uint32_t r0 = (((uint32_t)t >> GW) << 4) & ((1u << GM) - 1);
+    const uint32_t kb = __brev(r0) >> (32 - GM);
+
+    auto tile_base = [&](uint32_t tile, uint32_t *colbase) -> uint64_t {
+        const uint32_t cb = tile & ncb_mask;
+        const uint64_t hb = (uint64_t)(tile >> (lo - GW));
+        *colbase = cb << GW;
+        return (hb << lom) + ((uint64_t)cb << GW);
+    };
+    /* a limb-input tile (one operand, a.limbs, batch 1): its limbs, their count and the tile's first index (reading
+     * per-transform descriptors here would put a global load the loop must wait on after the fetch) */
+    auto limb_src = [&](uint64_t gbase, const uint64_t **src, uint64_t *nl) -> uint64_t {
+        *src = a.limbs; *nl = a.nlimbs;
+        return gbase;
+    };
+    auto fetch = [&](uint32_t tile, double *dst) {
+        uint32_t cbase;
+        if (LIMB) {                                 /* 4 bytes per lane: one wave instruction = one row of W limbs */
+            const uint64_t *src; uint64_t nl;
+            const uint64_t k0 = limb_src(tile_base(tile, &cbase), &src, &nl);
+            const uint32_t l = (uint32_t)t & 63;
+#pragma unroll
+            for (int i = 0; i < (1 << GM) / (STREAM_T / 64); i++) {
+                const uint32_t r = (uint32_t)i * (STREAM_T / 64) + wv;
+                const uint64_t k = k0 + ((uint64_t)r << lo) + (l >> 1);
+                if (k < nl)
+                    __builtin_amdgcn_global_load_lds((const void *)((const uint32_t *)(src + k) + (l & 1)),
+                                                     (__attribute__((address_space(3))) void *)(dst + r * W), 4, 0, 0);
+            }
+            return;
+        }
+        const double *src = a.x + tile_base(tile, &cbase);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t q = (uint32_t)(i * STREAM_T + t);           /* 16-byte chunk: elements 2q, 2q + 1 */
+            const double *g = src + ((uint64_t)(q >> (GW - 1)) << lo) + ((q & (W / 2 - 1)) << 1);
+            __builtin_amdgcn_global_load_lds((const void *)g,
+                                             (__attribute__((address_space(3))) void *)(dst + (i * STREAM_T + wv * 64) * 2),
+                                             16, 0, 0);
+        }
+    };
+
+    if (t < (1 << GM)) stw[t] = a.tw.stage[t];
+    for (int i = t; i < NBQ; i += STREAM_T) bqs[i] = a.tw.bq[i];
+    uint32_t tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile, buf0);
+    __builtin_amdgcn_s_waitcnt(0);                                     /* vmcnt(0) expcnt(0) lgkmcnt(0) */
+    for (uint32_t it = 0; tile < ntiles; it++, tile += gridDim.x) {
+        double *cur = (it & 1) ? buf1 : buf0, *nxt = (it & 1) ? buf0 : buf1;
+        /* this tile's fetch has landed (only the previous tile's 16 stores may be outstanding); every thread is past
+         * the previous tile's LDS reads of nxt */
+        __builtin_amdgcn_s_waitcnt(0x4F70);                             /* vmcnt(16) */
+        __syncthreads();
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x, nxt);
+        uint32_t colbase;
+        const uint64_t gbase = tile_base(tile, &colbase);
+        double *xt = a.x + gbase;
+
+        double v[PTS], w[PTS];
+        {   /* boundary twiddles of the bits-[0, 4) round: w[i] = base * step^i (a limb pass's n^-1 folded into base) */
+            const uint32_t col = colbase + c_in, sh = 32 - lom;
+            double base = bq_twiddle(bqs, (col * kb) << sh, p, pinv);
+            if (LIMB && a.do_in_scale) base = mul_shoup(base, a.in_scale.x, a.in_scale.y, p);
+            const double step = bq_twiddle(bqs, (col << (GM - 4)) << sh, p, pinv);
+            const double stepq = step / p;
+            w[0] = base;
+#pragma unroll
+            for (int i = 1; i < PTS; i++) w[i] = mul_shoup(w[i - 1], step, stepq, p);
+        }
+        int s_lo, logR;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            round_split(GM, i, &s_lo, &logR, INV, 4);
+            if (i > 0) lds_barrier();
+            if (LIMB && i == 0) {                   /* limbs to residues, zero beyond the operand */
+                const uint64_t *src; uint64_t nl;
+                const uint64_t k0 = limb_src(gbase, &src, &nl);
+                const uint64_t *cl = (const uint64_t *)cur;
+                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
+                    const uint64_t k = k0 + ((uint64_t)(e >> GW) << lo) + (e & (W - 1));
+                    v[sl] = k < nl ? limb_to_res(cl[e], p, pinv) : 0.0; });
+            } else
+                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { v[sl] = cur[e]; });
+            if (INV && i == 0) {
+#pragma unroll
+                for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
+            }
+            round_compute<4, INV, PTS>(v, pp, s_lo, stw, p, pinv);
+            if (i == 0)
+                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { cur[e] = v[sl]; });
+        }
+        if (!INV) {
+#pragma unroll
+            for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
+        }
+        if (INV && a.do_scale) {                                       /* the inverse's final n^-1 */
+#pragma unroll
+            for (int j = 0; j < PTS; j++) v[j] = mul_shoup(v[j], a.scale.x, a.scale.y, p);
+        }
+        for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t, uint32_t) { xt[o] = v[sl]; });
+    }
+    __builtin_amdgcn_s_waitcnt(0);                                     /* no LDS fetch in flight past the block's end */
+}
+
 /* ---- the fused last forward pass + pointwise product + first inverse pass ------------------------------------------------
  * For the contiguous low-bit pass (lo = 0, m + logh <= 12): the DIF rounds of the forward transform, the product with
  * y in a linear pass over LDS (y read coalesced), the DIT rounds of the inverse, one read and one write of x.
@@ -527,6 +687,8 @@ static int g_pts = PASS_PTS;                                   /* points per thr
 void nttm_set_pts(int pts) { g_pts = pts == 8 ? 8 : PASS_PTS; }
 static int g_fixed_shapes = 1;                                  /* use the fixed-shape pass kernels where they apply */
 void nttm_set_fixed_shapes(int on) { g_fixed_shapes = on != 0; }
+static int g_stream = 1, g_ncu = 256;                           /* streaming strided passes; CUs of the device */
+void nttm_set_stream(int on, int ncu) { g_stream = on != 0; g_ncu = ncu > 0 ? ncu : 256; }
 
 int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
 {
@@ -564,7 +726,16 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P, M, W, H>), grid, blk, lds, s, b); break; }
     /* the strided tile shapes of large transforms, compiled with their geometry fixed */
     const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
-    if (fixed && ppt == PASS_PTS && g.m == 8 && g.logw == 5) { NTTM_PASS_LAUNCH_G(PASS_PTS, 8, 5, 0) }
+    const bool stream = fixed && g_stream && g.m == 8 && g.logw == 5 && g.lo >= 5 && mode != PM_INV_PW &&
+                        (mode == PM_INV || !b.do_scale) && (mode != PM_FWD_LIMB || (!b.ldesc && b.batch == 1));
+    if (stream) {
+        const unsigned nb = (unsigned)(tiles < (uint64_t)g_ncu ? tiles : (uint64_t)g_ncu);
+        const uint32_t nt = (uint32_t)tiles;
+        if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+        else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+        else hipLaunchKernelGGL((k_pass_stream<PM_INV, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+    }
+    else if (fixed && ppt == PASS_PTS && g.m == 8 && g.logw == 5) { NTTM_PASS_LAUNCH_G(PASS_PTS, 8, 5, 0) }
     else if (T >= PASS_PTS && ppt == 8) { NTTM_PASS_LAUNCH(8) }
     else if (T >= PASS_PTS) { NTTM_PASS_LAUNCH(PASS_PTS) }
     else if (T == 8) { NTTM_PASS_LAUNCH(8) }                   /* tiles of 2..8 points (transforms below 2^4, odd */

@@ -30,11 +30,11 @@ def test_fwd_equals_dft(nttm, prime, logn):
             assert np.array_equal(got[b], O.dft_bitrev(xs[b], p, w)), (prime, logn, b)
 
 
-@pytest.mark.parametrize("logn", [10, 13, 14, 16, 17, 20, 22, 23])
+@pytest.mark.parametrize("logn", [10, 13, 14, 16, 17, 20, 21, 22, 23])
 def test_fwd_equals_host_ntt(nttm, logn):
     for prime in (0, 3):
         p, w = nttm.prime(prime), nttm.root(prime, logn)
-        batch = 2 if logn <= 20 else 1
+        batch = 2 if logn <= 21 else 1
         xs = [rand_res(1 << logn, p, 31 * logn + b) for b in range(batch)]
         say("fwd-host", prime, logn, batch)
         t = dev(np.concatenate(xs))
