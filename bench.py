@@ -121,8 +121,9 @@ def dominant(ks):
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        return json.load(open(path)).get(name)
-    except (OSError, ValueError):
+        rec = json.load(open(path)).get(name)
+        return rec.get("bytes") if rec else None
+    except (OSError, ValueError, AttributeError):
         return None
 
 
@@ -153,7 +154,72 @@ def bench_leaves(nttm, reps):
             "batch_equals_each_first64": bool(same)}
 
 
+PROF_NAMES = (("k_fused2", "k_fused2"), ("k_pass_stream<0", "k_pass<fwd>"), ("k_pass_stream<1", "k_pass<fwd,limb>"),
+              ("k_pass_stream<2", "k_pass<inv>"), ("k_crt<", "k_crt"))
+
+
+def trace_summary(db):
+    """per-kernel calls / total / average / min / max duration (us) of a rocprofv3 --kernel-trace results database"""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("""select name, count(*), sum(duration) / 1e3, avg(duration) / 1e3, min(duration) / 1e3,
+                          max(duration) / 1e3, max(lds_size), max(grid_x), max(workgroup_x)
+                          from kernels group by name order by sum(duration) desc""").fetchall()
+    tot = sum(r[2] for r in rows)
+    print("| kernel | calls | total us | avg us | min us | max us | % | lds B | grid | wg |")
+    print("|---|---|---|---|---|---|---|---|---|---|")
+    for r in rows:
+        n = r[0] if len(r[0]) < 60 else r[0][:57] + "..."
+        print("| %s | %d | %.1f | %.1f | %.1f | %.1f | %.1f | %s | %s | %s |"
+              % (n, r[1], r[2], r[3], r[4], r[5], 100 * r[2] / tot, r[6], r[7], r[8]))
+
+
+def traffic_summary(fetch_db, write_db):
+    """HBM bytes per dispatch of every kernel from a FETCH_SIZE and a WRITE_SIZE rocprofv3 --pmc run of
+    `bench.py --copy-probe`, each scaled by what its counter reads for the probe's 1 GiB device copy (the first
+    __amd_rocclr_copyBuffer dispatch of the run).  Prints a table and the JSON that bench.py reads back."""
+    import collections
+    import sqlite3
+
+    def load(db):
+        con = sqlite3.connect(db)
+        per, meta = collections.defaultdict(float), {}
+        for d, k, v in con.execute("select dispatch_id, kernel_name, value from counters_collection"):
+            per[(d, k)] += v
+            meta[d] = k
+        return per, meta
+
+    (fe, fm), (wr, wm) = load(fetch_db), load(write_db)
+    probe = min(d for d, k in fm.items() if k == "__amd_rocclr_copyBuffer")
+    cf, cw = fe[(probe, fm[probe])], wr[(probe, wm[probe])]
+    ff, fw = float(1 << 20) / cf, float(1 << 20) / cw
+    print("copy probe (1 GiB): FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB -> scale %.3f, %.3f" % (cf, cw, ff, fw))
+    avg = collections.defaultdict(lambda: [0.0, 0.0, 0])
+    for (d, k), v in fe.items():
+        avg[k][0] += v; avg[k][2] += 1
+    for (d, k), v in wr.items():
+        avg[k][1] += v
+    out = {}
+    print("| kernel | dispatches | fetch KiB raw | write KiB raw | HBM read GB | HBM write GB |")
+    print("|---|---|---|---|---|---|")
+    for k, (f, w, n) in sorted(avg.items(), key=lambda kv: -kv[1][0]):
+        f, w = f / n, w / n
+        rd, wt = f * ff * 1024, w * fw * 1024
+        print("| %s | %d | %.0f | %.0f | %.3f | %.3f |" % (k[:60], n, f, w, rd / 1e9, wt / 1e9))
+        tag = next((t for pat, t in PROF_NAMES if pat in k), None)
+        if tag:
+            out[tag] = {"bytes": round(rd + wt), "read_bytes": round(rd), "write_bytes": round(wt)}
+    out["_scale"] = {"fetch": ff, "write": fw, "probe_bytes": 1 << 30}
+    print(json.dumps(out))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--trace-summary":
+        trace_summary(sys.argv[2])
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
+        traffic_summary(sys.argv[2], sys.argv[3])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)

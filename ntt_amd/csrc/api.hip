@@ -134,6 +134,19 @@ static void build_crt_const(CrtConst &cc)
             for (int w = 0; w < NTTM_MAX_PRIMES; w++) { u128 s = (u128)M[w] * NTTM_PRIMES[j] + carry; M[w] = (uint64_t)s; carry = (uint64_t)(s >> 64); }
         }
         for (int w = 0; w < NTTM_MAX_PRIMES; w++) cc.mprod[i][w] = M[w];
+        {   /* the same in base 10^18 (digits j < i; the product is below 2^(50 i) < 10^(18 i)) */
+            uint64_t W[NTTM_MAX_PRIMES];
+            for (int w = 0; w < NTTM_MAX_PRIMES; w++) W[w] = M[w];
+            for (int j = 0; j < NTTM_MAX_PRIMES; j++) {
+                u128 rem = 0;
+                for (int w = NTTM_MAX_PRIMES - 1; w >= 0; w--) {
+                    const u128 cur = (rem << 64) | W[w];
+                    W[w] = (uint64_t)(cur / NTTM_B); rem = cur % NTTM_B;
+                }
+                cc.mb[i][j] = (uint64_t)rem;
+                cc.mbp[i][j] = (uint64_t)(((u128)rem << 64) / NTTM_B);
+            }
+        }
         for (int j = 1; j <= i; j++) {
             uint64_t m = 1;
             for (int k = 0; k < j; k++) m = h_mulmod(m, NTTM_PRIMES[k] % p, p);

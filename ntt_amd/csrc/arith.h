@@ -90,6 +90,27 @@ NTTM_HD uint64_t umulhi64(uint64_t a, uint64_t b)
 #endif
 }
 
+/* ---- the CRT's digit split: products modulo B = 10^18 -------------------------------------------------------------
+ * mulmod_b(t, M, Mp): t M = a B + b with 0 <= b < B, for t < 2^64 and M < B, Mp = floor(M 2^64 / B) (Shoup's
+ * precomputed quotient factor with B as the modulus): q = floor(t Mp / 2^64) satisfies q <= floor(t M / B) <= q + 1,
+ * since t M / B - t Mp / 2^64 = t (M 2^64 - Mp B) / (B 2^64) < t / 2^64 <= 1; b = t M - q B is below 2B < 2^64, so
+ * it is exact modulo 2^64, and one correction finishes. */
+NTTM_HD void mulmod_b(uint64_t t, uint64_t M, uint64_t Mp, uint64_t *a, uint64_t *b)
+{
+    uint64_t q = umulhi64(t, Mp);
+    uint64_t r = t * M - q * NTTM_B;
+    const int c = r >= NTTM_B;
+    *a = q + (uint64_t)c;
+    *b = c ? r - NTTM_B : r;
+}
+/* an integer-valued double in [0, 2^52) to its integer: the low 52 bits of 2^52 + v */
+NTTM_HD uint64_t f64_to_u52(double v)
+{
+    union { double d; uint64_t u; } x;
+    x.d = v + 4503599627370496.0;
+    return x.u & ((1ULL << 52) - 1);
+}
+
 /* division by B = 10^18 with a precomputed inverse of the normalized divisor D = 16 B (2^63 <= D < 2^64):
  * N. Moller and T. Granlund, "Improved division by invariant integers", IEEE Trans. Computers 60(2), 2011, Alg. 4.
  * v = floor((2^128 - 1) / D) - 2^64.  Requires hi < B; q = floor((hi 2^64 + lo) / B), r the remainder. */

@@ -58,52 +58,46 @@ template <int NP>
 __device__ __forceinline__ void crt_digits(const CrtConst &cc, const double *r, uint64_t *d)
 {
     uint64_t t[NP];
+    double td[NP];                                       /* the same digits as doubles (exact, below 2^50) */
     const double p0 = cc.pk[0].p, pinv0 = cc.pk[0].pinv;
-    { double v = reduce(r[0], p0, pinv0); v = v < 0.0 ? v + p0 : v; t[0] = (uint64_t)v; }
+    { double v = reduce(r[0], p0, pinv0); v = v < 0.0 ? v + p0 : v; td[0] = v; t[0] = f64_to_u52(v); }
 #pragma unroll
     for (int i = 1; i < NP; i++) {
         const double p = cc.pk[i].p, pinv = cc.pk[i].pinv;
-        double acc = reduce((double)t[0], p, pinv);
+        double acc = reduce(td[0], p, pinv);
 #pragma unroll
         for (int j = 1; j < i; j++)
-            acc = reduce(acc + mul_barrett(reduce((double)t[j], p, pinv), cc.mmod[i][j], p, pinv), p, pinv);
+            acc = reduce(acc + mul_barrett(reduce(td[j], p, pinv), cc.mmod[i][j], p, pinv), p, pinv);
         double dd = reduce(r[i] - acc, p, pinv);
         double ti = mul_barrett(dd, cc.cinv[i], p, pinv);
         ti = ti < 0.0 ? ti + p : ti;
-        t[i] = (uint64_t)ti;
+        td[i] = ti; t[i] = f64_to_u52(ti);
     }
-    uint64_t X[NP];
+    /* X = sum_i t_i M_i with M_i = p_0 ... p_{i-1} = sum_j m_ij B^j: every t_i m_ij = a_ij B + b_ij (mulmod_b), then
+     * base-B columns col_j = sum_i b_ij + sum_i a_i(j-1) (+ t_0 in column 0) with their small carries.  Column sums
+     * stay below NP B + NP 2^50 < 2^62; X < 2^(50 NP) bounds the top digit (NP = 3: < 2^31, NP = 4: < 2^21). */
+    uint64_t acc[NP + 1];
 #pragma unroll
-    for (int w = 0; w < NP; w++) X[w] = 0;
-    X[0] = t[0];
+    for (int j = 0; j <= NP; j++) acc[j] = 0;
+    acc[0] = t[0];
 #pragma unroll
-    for (int i = 1; i < NP; i++) {                       /* X += t_i * (p_0 ... p_{i-1}) */
-        uint64_t carry = 0;
+    for (int i = 1; i < NP; i++)
 #pragma unroll
-        for (int w = 0; w < NP; w++) {
-            const uint64_t mw = w < i ? cc.mprod[i][w] : 0;
-            const uint64_t lo = t[i] * mw, hi = umulhi64(t[i], mw);
-            uint64_t s = X[w] + lo; uint64_t c1 = s < lo;
-            uint64_t s2 = s + carry; c1 += s2 < s;
-            X[w] = s2; carry = hi + c1;
+        for (int j = 0; j < i; j++) {
+            uint64_t ah, bl;
+            mulmod_b(t[i], cc.mb[i][j], cc.mbp[i][j], &ah, &bl);
+            acc[j] += bl; acc[j + 1] += ah;
         }
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < NP - 1; j++) {                   /* columns below the top: carries of at most NP - 1 */
+        uint64_t v = acc[j] + c;
+        c = 0;
+#pragma unroll
+        for (int k = 1; k < NP; k++) { const int ge = v >= NTTM_B; v = ge ? v - NTTM_B : v; c += (uint64_t)ge; }
+        d[j] = v;
     }
-    /* base-10^18 digits: X < 2^(50 NP), so the top word is below 2^(50 NP - 64 (NP-1)) < B and every quotient of the
-     * chain below keeps its high word under B (div_b's precondition).  NP = 3: 3 divisions, NP = 4: 6. */
-    if (NP == 3) {
-        uint64_t q1, q0, r;
-        div_b(X[2], X[1], &q1, &r);                  /* X[2] < 2^22 */
-        div_b(r, X[0], &q0, &d[0]);
-        div_b(q1, q0, &d[2], &d[1]);                 /* q1 < 2^27: X / B = q1 2^64 + q0 < 2^91 */
-    } else {
-        uint64_t q2, q1, q0, r, p1, p0;
-        div_b(X[3], X[2], &q2, &r);                  /* X[3] < 2^8 */
-        div_b(r, X[1], &q1, &r);
-        div_b(r, X[0], &q0, &d[0]);
-        div_b(q2, q1, &p1, &r);                      /* q2 < 2^13 */
-        div_b(r, q0, &p0, &d[1]);
-        div_b(p1, p0, &d[3], &d[2]);                 /* p1 < 2^17 */
-    }
+    d[NP - 1] = acc[NP - 1] + c;
 }
 
 template <int NP>

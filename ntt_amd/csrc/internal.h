@@ -63,5 +63,7 @@ struct CrtConst {
     double cinv[NTTM_MAX_PRIMES];               /* (p_0 ... p_{i-1})^-1 mod p_i, balanced */
     double mmod[NTTM_MAX_PRIMES][NTTM_MAX_PRIMES];   /* (p_0 ... p_{j-1}) mod p_i, balanced (j < i) */
     uint64_t mprod[NTTM_MAX_PRIMES][NTTM_MAX_PRIMES];/* p_0 ... p_{i-1} as little-endian words */
+    uint64_t mb[NTTM_MAX_PRIMES][NTTM_MAX_PRIMES];   /* p_0 ... p_{i-1} in base 10^18, digit j < i */
+    uint64_t mbp[NTTM_MAX_PRIMES][NTTM_MAX_PRIMES];  /* floor(mb 2^64 / 10^18) (mulmod_b) */
 };
 #endif

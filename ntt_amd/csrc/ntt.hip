@@ -323,41 +323,33 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
         *colbase = cb << GW;
         return (hb << lom) + ((uint64_t)cb << GW);
     };
-    /* a limb-input tile (one operand, a.limbs, batch 1): its limbs, their count and the tile's first index (reading
-     * per-transform descriptors here would put a global load the loop must wait on after the fetch) */
-    auto limb_src = [&](uint64_t gbase, const uint64_t **src, uint64_t *nl) -> uint64_t {
-        *src = a.limbs; *nl = a.nlimbs;
-        return gbase;
+    /* a limb-input tile (one operand a.limbs, batch 1): the number of its elements, counted from the tile's first
+     * index, that lie inside the operand, saturated (element offsets within a tile are below 2^lom <= 2^31) */
+    auto limb_lim = [&](uint64_t gbase) -> uint32_t {
+        const uint64_t nl = a.nlimbs;
+        return nl <= gbase ? 0u : nl - gbase >= 0xffffffffULL ? 0xffffffffu : (uint32_t)(nl - gbase);
     };
     auto fetch = [&](uint32_t tile, double *dst) {
         uint32_t cbase;
-        if (LIMB) {                                 /* 4 bytes per lane: one wave instruction = one row of W limbs */
-            const uint64_t *src; uint64_t nl;
-            const uint64_t k0 = limb_src(tile_base(tile, &cbase), &src, &nl);
-            const uint32_t l = (uint32_t)t & 63;
-#pragma unroll
-            for (int i = 0; i < (1 << GM) / (STREAM_T / 64); i++) {
-                const uint32_t r = (uint32_t)i * (STREAM_T / 64) + wv;
-                const uint64_t k = k0 + ((uint64_t)r << lo) + (l >> 1);
-                if (k < nl)
-                    __builtin_amdgcn_global_load_lds((const void *)((const uint32_t *)(src + k) + (l & 1)),
-                                                     (__attribute__((address_space(3))) void *)(dst + r * W), 4, 0, 0);
-            }
-            return;
-        }
-        const double *src = a.x + tile_base(tile, &cbase);
+        const uint64_t gb = tile_base(tile, &cbase);
+        const uint32_t lim = LIMB ? limb_lim(gb) : 0u;
+        const double *src = LIMB ? (const double *)(a.limbs + gb) : a.x + gb;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const uint32_t q = (uint32_t)(i * STREAM_T + t);           /* 16-byte chunk: elements 2q, 2q + 1 */
-            const double *g = src + ((uint64_t)(q >> (GW - 1)) << lo) + ((q & (W / 2 - 1)) << 1);
-            __builtin_amdgcn_global_load_lds((const void *)g,
-                                             (__attribute__((address_space(3))) void *)(dst + (i * STREAM_T + wv * 64) * 2),
-                                             16, 0, 0);
+            const uint32_t off = ((q >> (GW - 1)) << lo) + ((q & (W / 2 - 1)) << 1);
+            /* a limb chunk is fetched when both limbs lie inside the operand; the one that straddles its end (odd
+             * length) is read from the preloaded last limb */
+            if (!LIMB || off + 1 < lim)
+                __builtin_amdgcn_global_load_lds((const void *)(src + off),
+                                                 (__attribute__((address_space(3))) void *)(dst + (i * STREAM_T + wv * 64) * 2),
+                                                 16, 0, 0);
         }
     };
 
     if (t < (1 << GM)) stw[t] = a.tw.stage[t];
     for (int i = t; i < NBQ; i += STREAM_T) bqs[i] = a.tw.bq[i];
+    const uint64_t last = LIMB && a.nlimbs ? a.limbs[a.nlimbs - 1] : 0;   /* waited for below, before the loop */
     uint32_t tile = blockIdx.x;
     if (tile < ntiles) fetch(tile, buf0);
     __builtin_amdgcn_s_waitcnt(0);                                     /* vmcnt(0) expcnt(0) lgkmcnt(0) */
@@ -389,12 +381,10 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
             round_split(GM, i, &s_lo, &logR, INV, 4);
             if (i > 0) lds_barrier();
             if (LIMB && i == 0) {                   /* limbs to residues, zero beyond the operand */
-                const uint64_t *src; uint64_t nl;
-                const uint64_t k0 = limb_src(gbase, &src, &nl);
+                const uint32_t lim = limb_lim(gbase);
                 const uint64_t *cl = (const uint64_t *)cur;
-                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
-                    const uint64_t k = k0 + ((uint64_t)(e >> GW) << lo) + (e & (W - 1));
-                    v[sl] = k < nl ? limb_to_res(cl[e], p, pinv) : 0.0; });
+                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
+                    v[sl] = o < lim ? limb_to_res(o + 1 == lim ? last : cl[e], p, pinv) : 0.0; });
             } else
                 for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { v[sl] = cur[e]; });
             if (INV && i == 0) {
@@ -727,7 +717,8 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     /* the strided tile shapes of large transforms, compiled with their geometry fixed */
     const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
     const bool stream = fixed && g_stream && g.m == 8 && g.logw == 5 && g.lo >= 5 && mode != PM_INV_PW &&
-                        (mode == PM_INV || !b.do_scale) && (mode != PM_FWD_LIMB || (!b.ldesc && b.batch == 1));
+                        (mode == PM_INV || !b.do_scale) &&
+                        (mode != PM_FWD_LIMB || (!b.ldesc && b.batch == 1 && g.lo + g.m < 32));
     if (stream) {
         const unsigned nb = (unsigned)(tiles < (uint64_t)g_ncu ? tiles : (uint64_t)g_ncu);
         const uint32_t nt = (uint32_t)tiles;

@@ -27,6 +27,9 @@ def A():
     L.t_limb_to_res.argtypes = [C.c_uint64, d, d]; L.t_limb_to_res.restype = d
     L.t_res_canon.argtypes = [d, d, d]; L.t_res_canon.restype = C.c_uint64
     L.t_div_b.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]; L.t_div_b.restype = None
+    L.t_mulmod_b.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.t_mulmod_b.restype = None
+    L.t_f64_to_u52.argtypes = [d]; L.t_f64_to_u52.restype = C.c_uint64
     L.t_prime.argtypes = [C.c_int]; L.t_prime.restype = C.c_uint64
     L.t_divb_v.argtypes = []; L.t_divb_v.restype = C.c_uint64
     return L
@@ -108,3 +111,26 @@ def test_div_b(A):
         A.t_div_b(hi, lo, C.byref(q), C.byref(r))
         n = (hi << 64) | lo
         assert (q.value, r.value) == (n // B, n % B), (hi, lo)
+
+
+def test_mulmod_b(A):
+    """t M = a B + b exactly, for the digit split of the CRT: t below 2^64 (the CRT uses t < 2^50), M below B, with
+    Mp = floor(M 2^64 / B); the extremes of both ranges and the products of the prime prefixes' base-B digits."""
+    rnd = random.Random(11)
+    a, b = C.c_uint64(), C.c_uint64()
+    ps = primes(A)
+    ms = [1, 2, B - 1, B - 2, 10 ** 17, 2 ** 59, ps[0], (ps[0] * ps[1]) % B, (ps[0] * ps[1]) // B,
+          (ps[0] * ps[1] * ps[2]) % B, ((ps[0] * ps[1] * ps[2]) // B) % B, (ps[0] * ps[1] * ps[2]) // B ** 2]
+    ms += [rnd.randrange(B) for _ in range(40)]
+    ts = [0, 1, 2 ** 50 - 1, ps[0] - 1, ps[3] - 1, 2 ** 52, 2 ** 63, 2 ** 64 - 1] + [rnd.randrange(2 ** 50) for _ in range(300)]
+    for m in ms:
+        mp = (m << 64) // B
+        for t in ts:
+            A.t_mulmod_b(t, m, mp, C.byref(a), C.byref(b))
+            assert (a.value, b.value) == divmod(t * m, B), (t, m)
+
+
+def test_f64_to_u52(A):
+    rnd = random.Random(12)
+    for v in [0, 1, 2, 2 ** 32 - 1, 2 ** 32, 2 ** 50 - 1, 2 ** 52 - 1] + [rnd.randrange(2 ** 52) for _ in range(5000)]:
+        assert A.t_f64_to_u52(float(v)) == v
