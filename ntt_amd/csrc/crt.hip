@@ -1,10 +1,12 @@
 /* crt.hip - Chinese remaindering of the residue planes into base-10^18 limbs, with the carry resolved on the device.
  *
  * Coefficient k of the convolution is rebuilt from its np residues by Garner's mixed radix (modular steps in the
- * FP64 arithmetic of arith.h, the integer value in np words), split into np base-10^18 digits d_0..d_{np-1}, and
+ * FP64 arithmetic of arith.h), and its value X = sum_i t_i p_0 ... p_{i-1} is split into np base-10^18 digits
+ * d_0..d_{np-1} from the products of the t_i with the base-10^18 digits of the prime prefixes (mulmod_b), and
  *     limb_k = d_0[k] + d_1[k-1] + ... + d_{np-1}[k-np+1] + x_k + carry_k            (E/rns_mul.c:332-405)
- * The top digit is small (X < 2^(50 np) gives d_{np-1} < 2^31), so v_k = sum of the digits + x_k < 4 B + 2^31.  With
- * q_k = floor(v_k / B), r_k = v_k mod B and w_k = r_k + q_{k-1} (<= B + 3), the carry into limb k+1 is q_k + o_k where
+ * The top digit is small (X < 2^(50 np) gives d_{np-1} < 2^31), so v_k = sum of the digits + x_k < np B + 2^31.  With
+ * q_k = floor(v_k / B) <= np, r_k = v_k mod B and w_k = r_k + q_{k-1} (<= B + np - 1), the carry into limb k+1 is
+ * q_k + o_k where
  * the overflow bit o_k = [w_k + o_{k-1} >= B] = g_k | (p_k & o_{k-1}), g_k = [w_k >= B], p_k = [w_k = B - 1]: a binary
  * carry-lookahead chain.  For 64 consecutive limbs it is resolved by one 64-bit addition of the ballot masks
  * X = G, Y = G | P (the carries into the bits of X + Y + cin are the o's, the textbook adder identity); words chain
@@ -57,18 +59,22 @@ __device__ __forceinline__ uint32_t add_cout(uint64_t X, uint64_t Y, uint32_t ci
 template <int NP>
 __device__ __forceinline__ void crt_digits(const CrtConst &cc, const double *r, uint64_t *d)
 {
+    /* Garner's digits t_i in [0, p_i), for residues |r_i| < p_i (balanced or canonical):
+     *   t_0 = r_0 mod p_0;  t_i = (r_i - t_0 - sum_{0<j<i} t_j (p_0 ... p_{j-1})) (p_0 ... p_{i-1})^-1 mod p_i.
+     * The primes differ by less than 2^40, so t_j < p_j <= 1.0007 p_i: each t_j m_ij (|m_ij| <= p_i / 2) stays within
+     * the Barrett product's bound (|result| < 0.63 p_i), the sum r_i - acc within 4 p_i (one reduce), and no
+     * intermediate reduction of t_j mod p_i is needed. */
     uint64_t t[NP];
     double td[NP];                                       /* the same digits as doubles (exact, below 2^50) */
-    const double p0 = cc.pk[0].p, pinv0 = cc.pk[0].pinv;
-    { double v = reduce(r[0], p0, pinv0); v = v < 0.0 ? v + p0 : v; td[0] = v; t[0] = f64_to_u52(v); }
+    const double p0 = cc.pk[0].p;
+    { const double v = r[0] < 0.0 ? r[0] + p0 : r[0]; td[0] = v; t[0] = f64_to_u52(v); }
 #pragma unroll
     for (int i = 1; i < NP; i++) {
         const double p = cc.pk[i].p, pinv = cc.pk[i].pinv;
-        double acc = reduce(td[0], p, pinv);
+        double acc = td[0];
 #pragma unroll
-        for (int j = 1; j < i; j++)
-            acc = reduce(acc + mul_barrett(reduce(td[j], p, pinv), cc.mmod[i][j], p, pinv), p, pinv);
-        double dd = reduce(r[i] - acc, p, pinv);
+        for (int j = 1; j < i; j++) acc += mul_barrett(td[j], cc.mmod[i][j], p, pinv);
+        const double dd = reduce(r[i] - acc, p, pinv);
         double ti = mul_barrett(dd, cc.cinv[i], p, pinv);
         ti = ti < 0.0 ? ti + p : ti;
         td[i] = ti; t[i] = f64_to_u52(ti);
@@ -90,11 +96,13 @@ __device__ __forceinline__ void crt_digits(const CrtConst &cc, const double *r, 
         }
     uint64_t c = 0;
 #pragma unroll
-    for (int j = 0; j < NP - 1; j++) {                   /* columns below the top: carries of at most NP - 1 */
+    for (int j = 0; j < NP - 1; j++) {
+        /* column j holds NP - 1 - j terms below B, the rest (a's, t_0, the carry in) below 2^52 < B: its carry is at
+         * most NP - 1 - j */
         uint64_t v = acc[j] + c;
         c = 0;
 #pragma unroll
-        for (int k = 1; k < NP; k++) { const int ge = v >= NTTM_B; v = ge ? v - NTTM_B : v; c += (uint64_t)ge; }
+        for (int k = 0; k < NP - 1 - j; k++) { const int ge = v >= NTTM_B; v = ge ? v - NTTM_B : v; c += (uint64_t)ge; }
         d[j] = v;
     }
     d[NP - 1] = acc[NP - 1] + c;
@@ -140,28 +148,38 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
     __syncthreads();
 
     /* limb k = k0 + wave 512 + i 64 + lane for i < 8: one ballot per i gives 64 consecutive limbs */
-    uint64_t wv[CRT_PER];
-    uint64_t GX[CRT_PER], GY[CRT_PER];
-    uint32_t qlast = 0;
-#pragma unroll
-    for (int i = 0; i < CRT_PER; i++) {
-        const int off = wave * 64 * CRT_PER + i * 64 + lane;          /* offset of limb k inside the block */
-        const uint64_t k = k0 + off;
-        const int li = CRT_HALO + off;
-        uint64_t v = 0, vp = 0;                                      /* v_k and v_{k-1} */
+    /* v_k = sum_j d_j[k - j] + x_k < NP B + 2^31: its quotient by B is at most NP */
+    auto vsum = [&](uint64_t k, int li) -> uint64_t {
+        uint64_t v = 0;
         if (k < dsc.nc) {
 #pragma unroll
             for (int j = 0; j < NP; j++) v += D[j][li - j];
             if (k < dsc.nx) v += dsc.x[k];
         }
-        if (k >= 1 && k - 1 < dsc.nc) {
+        return v;
+    };
+    auto quot = [](uint64_t v) -> uint32_t {
+        uint32_t q = 0;
 #pragma unroll
-            for (int j = 0; j < NP; j++) vp += D[j][li - 1 - j];
-            if (k - 1 < dsc.nx) vp += dsc.x[k - 1];
-        }
-        const uint32_t q = (v >= NTTM_B) + (v >= 2 * NTTM_B) + (v >= 3 * NTTM_B) + (v >= 4 * NTTM_B);
-        const uint32_t qp = (vp >= NTTM_B) + (vp >= 2 * NTTM_B) + (vp >= 3 * NTTM_B) + (vp >= 4 * NTTM_B);
-        const uint64_t w = v - (uint64_t)q * NTTM_B + qp;            /* <= B + 3 */
+        for (int c = 1; c <= NP; c++) q += v >= (uint64_t)c * NTTM_B;
+        return q;
+    };
+    uint64_t wv[CRT_PER];
+    uint64_t GX[CRT_PER], GY[CRT_PER];
+    uint32_t qlast = 0, qprev = 0;
+#pragma unroll
+    for (int i = 0; i < CRT_PER; i++) {
+        const int off = wave * 64 * CRT_PER + i * 64 + lane;          /* offset of limb k inside the block */
+        const uint64_t k = k0 + off;
+        const int li = CRT_HALO + off;
+        const uint64_t v = vsum(k, li);
+        const uint32_t q = quot(v);
+        /* q_{k-1}: from the lane below; lane 0 from the previous word's lane 63, or (first word) computed */
+        uint32_t qp = __shfl_up(q, 1, 64);
+        if (i > 0) { if (lane == 0) qp = qprev; }
+        else if (lane == 0) qp = k >= 1 ? quot(vsum(k - 1, li - 1)) : 0u;
+        qprev = __builtin_amdgcn_readlane(q, 63);
+        const uint64_t w = v - (uint64_t)q * NTTM_B + qp;            /* <= B + NP */
         wv[i] = w;
         const uint64_t g = __ballot(w >= NTTM_B), pr = __ballot(w == NTTM_B - 1);
         GX[i] = g; GY[i] = g | pr;
