@@ -25,7 +25,10 @@
  * writes exactly the elements of its groups within a round, so one barrier separates consecutive rounds.
  * Lazy reduction (bounds of arith.h): the DIF reduces its sum path at odd levels and at a round's last level (values
  * entering a butterfly stay within [-2p, 2p]); the DIT reduces both outputs at a round's last level only (after levels
- * 0..3 the outputs are within 1.75p, 2.5p, 3.5p, 4.5p + 2, and every mul_shoup input within 4p). */
+ * 0..3 the outputs are within 1.75p, 2.5p, 3.5p, 4.5p + 2, and every mul_shoup input within 4p; with the trivial
+ * first level of a lowest-bits round, 2p, 2.75p + 1, 3.75p + 2, 4.75p + 3, mul_shoup inputs within 3.75p + 2).
+ * Where the round's low row bits are known to be zero at compile time, twiddles w^0 = 1 are not multiplied: the DIF
+ * difference then follows the sum path's rule, the DIT skips them at its first level (where all of them are 1). */
 #define PASS_PTS 16               /* points per thread of full tiles; tiles below 16 points use one thread */
 
 template <int BITS>
@@ -35,6 +38,10 @@ __device__ __forceinline__ constexpr uint32_t brev_c(uint32_t j)
     for (int i = 0; i < BITS; i++) r |= ((j >> i) & 1u) << (BITS - 1 - i);
     return r;
 }
+
+/* rlow == 0 known at compile time (the round over the lowest bits of a fixed-shape tile): the butterflies with
+ * jj = 0 then have the twiddle w^0 = 1 */
+#define NTTM_TRIVIAL_W(rlow, jj) ((jj) == 0 && __builtin_constant_p(rlow) && (rlow) == 0)
 
 template <int LOGR>
 __device__ __forceinline__ void dif_group(double *v, int s_lo, uint32_t rlow, const double2 *__restrict__ tw, double p,
@@ -48,13 +55,15 @@ __device__ __forceinline__ void dif_group(double *v, int s_lo, uint32_t rlow, co
         const uint32_t tbase = (1u << (s_lo + k)) + rlow;
 #pragma unroll
         for (int jj = 0; jj < h; jj++) {
-            const double2 w = tw[tbase + ((uint32_t)jj << s_lo)];
+            /* a trivial twiddle: the difference follows the sum path's bound and reduction rule */
+            const bool one = NTTM_TRIVIAL_W(rlow, jj);
+            const double2 w = one ? make_double2(1.0, 0.0) : tw[tbase + ((uint32_t)jj << s_lo)];
 #pragma unroll
             for (int j = jj; j < R; j += 2 * h) {
                 const double u = v[j], x = v[j + h];
-                const double sm = u + x;
+                const double sm = u + x, df = u - x;
                 v[j] = red ? reduce(sm, p, pinv) : sm;
-                v[j + h] = mul_shoup(u - x, w.x, w.y, p);
+                v[j + h] = one ? (red ? reduce(df, p, pinv) : df) : mul_shoup(df, w.x, w.y, p);
             }
         }
     }
@@ -72,10 +81,13 @@ __device__ __forceinline__ void dit_group(double *v, int s_lo, uint32_t rlow, co
         const uint32_t tbase = (1u << (s_lo + k)) + rlow;
 #pragma unroll
         for (int jj = 0; jj < h; jj++) {
-            const double2 w = tw[tbase + ((uint32_t)jj << s_lo)];
+            /* a trivial twiddle at the first level only (inputs within p: outputs within 2p, and the later bounds
+             * below hold with 0.25p to spare) */
+            const bool one = lev == 0 && NTTM_TRIVIAL_W(rlow, jj);
+            const double2 w = one ? make_double2(1.0, 0.0) : tw[tbase + ((uint32_t)jj << s_lo)];
 #pragma unroll
             for (int j = jj; j < R; j += 2 * h) {
-                const double u = v[j], x = mul_shoup(v[j + h], w.x, w.y, p);
+                const double u = v[j], x = one ? v[j + h] : mul_shoup(v[j + h], w.x, w.y, p);
                 const double s1 = u + x, s2 = u - x;
                 v[j] = red ? reduce(s1, p, pinv) : s1;
                 v[j + h] = red ? reduce(s2, p, pinv) : s2;
@@ -500,11 +512,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
  * same bits [0, logR) with the same slot layout, so B's transformed tile stays in registers (bv) and is never written;
  * the product v * bv and the first inverse round follow A's last forward round without an LDS round trip.
  * One read of A (or its limbs), one read of B (or its limbs), one write of A. */
-template <int PTS, int MAXR>
+template <int PTS, int MAXR, bool LIMBS>
 __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &pp, bool active, uint64_t gbase,
                                              double *xs, double *v, double p, double pinv)
 {
-    const bool limb = q.limbs || q.ldesc;
+    const bool limb = LIMBS && (q.limbs || q.ldesc);
     const double *xt = q.x + gbase;
     const int nr = round_count(q.m, MAXR);
     int s_lo, logR;
@@ -540,7 +552,7 @@ __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &p
     }
 }
 
-template <int PTS, int GM = -1, int GH = -1>
+template <int PTS, int GM = -1, int GH = -1, bool LIMBS = true>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused2(PassArgs a, PassArgs bq)
 {
     __shared__ double xs[1 << FUSED_LOG];
@@ -558,9 +570,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     double v[PTS], bv[PTS];
     int s_lo, logR;
 
-    tile_forward<PTS, MAXR>(bq, pp, active, gbase, xs, bv, p, pinv);
+    tile_forward<PTS, MAXR, LIMBS>(bq, pp, active, gbase, xs, bv, p, pinv);
     __syncthreads();                                             /* B's last LDS reads before A's first writes */
-    tile_forward<PTS, MAXR>(a, pp, active, gbase, xs, v, p, pinv);
+    tile_forward<PTS, MAXR, LIMBS>(a, pp, active, gbase, xs, v, p, pinv);
     if (active) {
 #pragma unroll
         for (int sl = 0; sl < PTS; sl++) v[sl] = mul_barrett(v[sl], bv[sl], p, pinv);
@@ -785,10 +797,11 @@ int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hi
     const int ppt = (T >= 64 * g_pts && T <= 512 * g_pts) ? g_pts : PASS_PTS;
     const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
     if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
-    const bool fixed = g_fixed_shapes && g.m == FUSED_LOG && g.logh == 0;
-    if (ppt == 8 && fixed) hipLaunchKernelGGL((k_fused2<8, FUSED_LOG, 0>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    /* the fixed shape (a 4096-point last pass of a transform of more than one pass) reads no limbs */
+    const bool fixed = g_fixed_shapes && g.m == FUSED_LOG && g.logh == 0 && !alimb && !blimb;
+    if (ppt == 8 && fixed) hipLaunchKernelGGL((k_fused2<8, FUSED_LOG, 0, false>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
     else if (ppt == 8) hipLaunchKernelGGL((k_fused2<8>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
-    else if (fixed) hipLaunchKernelGGL((k_fused2<PASS_PTS, FUSED_LOG, 0>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    else if (fixed) hipLaunchKernelGGL((k_fused2<PASS_PTS, FUSED_LOG, 0, false>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
     else hipLaunchKernelGGL((k_fused2<PASS_PTS>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_fused2 launch");
