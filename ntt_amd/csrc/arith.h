@@ -4,15 +4,18 @@
  * lies in [-p, p] (|v| < 2^50) and sums of two stay below 2^52: FP64 add/sub/FMA on them are exact, and the full-rate
  * FP64 pipe of CDNA4 does the modular products.  Bounds below use u = 2^-53 and p < 2^50.
  *
+ *  rint_prod(a, b)       a b rounded once to the nearest integer, for |a b| < 2^51: fma(a, b, C) - C with C = 1.5 2^52
+ *                        (the FMA result lies in [2^52, 2^53), where the FP64 spacing is 1; the subtraction is exact).
+ *                        One FMA and one add instead of a product and a rint, and a single rounding.
  *  mul_shoup(a, w, wq)   Shoup's product with a precomputed quotient factor wq = fl(w / p) (V. Shoup; D. Harvey,
  *                        "Faster arithmetic for number-theoretic transforms", J. Symb. Comput. 60, 2014), in FP64:
- *                        hi + lo = a w exactly (FMA), q = rint(a wq), r = (hi - q p) + lo.  For |a| <= 4p, |w| <= p/2:
- *                        |a w / p| <= 2p < 2^51 and |fl(a wq) - a w / p| < 2^51 (2u + u^2) < 1/2 + 2^-50, so
- *                        |q - a w / p| < 1 + 2^-50 and the integer r has |r| <= p (|r| <= 3p/4 + 1 for |a| <= 2p).
+ *                        hi + lo = a w exactly (FMA), q = rint_prod(a, wq), r = (hi - q p) + lo.  For |a| <= 4p,
+ *                        |w| <= p/2: |a w / p| <= 2p < 2^51 and |a wq - a w / p| <= 2^51 u = 1/4, so
+ *                        |q - a w / p| <= 3/4 and the integer r has |r| <= 3p/4 (|r| <= 5p/8 for |a| <= 2p).
  *                        hi - q p = r - lo is an integer below 2^53, so the FMA and the final add are exact.
- *  mul_barrett(a, b)     both operands variable, |a|, |b| <= p: q = rint(fl(a b) fl(1/p)); |a b / p - y| < p 3u (1+u)^2
- *                        < 3/8, so |r| <= 7p/8.
- *  reduce(s)             |s| <= 4p -> s - p rint(s / p), |result| <= p/2 + 1.
+ *  mul_barrett(a, b)     both operands variable, |a|, |b| <= p: q = rint_prod(fl(a b), fl(1/p)); |fl(a b) fl(1/p) - a b / p|
+ *                        < p 2u (1+u) < 1/4, so |r| <= 3p/4.
+ *  reduce(s)             |s| <= 4p -> s - p rint_prod(s, 1/p), |result| <= p/2 + 1.
  *  The reference computes its modmul in FP64 as well, with 52-bit primes and a Barrett quotient of the rounded product
  *  (E/modarith.h:118-130); here twiddle products use Shoup's precomputed factor and every value stays balanced.
  */
@@ -44,30 +47,31 @@ static const uint64_t NTTM_PRIMES[NTTM_MAX_PRIMES] = {
 #define NTTM_LOGN_MAX 32   /* the smallest 2-adic order in the set */
 
 NTTM_HD double nttm_fma(double a, double b, double c) { return fma(a, b, c); }
-NTTM_HD double nttm_rint(double x) { return rint(x); }
+#define NTTM_RND 6755399441055744.0          /* 1.5 2^52 */
+NTTM_HD double rint_prod(double a, double b) { return nttm_fma(a, b, NTTM_RND) - NTTM_RND; }
 
 NTTM_HD double mul_shoup(double a, double w, double wq, double p)
 {
     double hi = a * w;
     double lo = nttm_fma(a, w, -hi);
-    double q = nttm_rint(a * wq);
+    double q = rint_prod(a, wq);
     return nttm_fma(-q, p, hi) + lo;
 }
 NTTM_HD double mul_barrett(double a, double b, double p, double pinv)
 {
     double hi = a * b;
     double lo = nttm_fma(a, b, -hi);
-    double q = nttm_rint(hi * pinv);
+    double q = rint_prod(hi, pinv);
     return nttm_fma(-q, p, hi) + lo;
 }
-NTTM_HD double reduce(double s, double p, double pinv) { return nttm_fma(-nttm_rint(s * pinv), p, s); }
+NTTM_HD double reduce(double s, double p, double pinv) { return nttm_fma(-rint_prod(s, pinv), p, s); }
 
 /* a base-10^18 limb (< 2^60) to a residue in [-p/2 - 1, p/2 + 1] */
 NTTM_HD double limb_to_res(uint64_t x, double p, double pinv)
 {
     double h = (double)(uint32_t)(x >> 32) * 4294967296.0;     /* exact: < 2^60 with 28 significant bits */
     double l = (double)(uint32_t)x;
-    double q = nttm_rint((h + l) * pinv);
+    double q = rint_prod(h + l, pinv);
     return nttm_fma(-q, p, h) + l;                               /* h - q p exact (|.| < p + 2^32), then + l exact */
 }
 /* [-p, p] -> canonical [0, p) */

@@ -64,10 +64,10 @@ def test_mul_shoup_bounds_and_congruence(A):
             wq = w / p
             for a in [x for x in edge_values(p, 4) if abs(x) <= 4 * p] + [rnd.randrange(-4 * p, 4 * p + 1) for _ in range(300)]:
                 r = A.t_mul_shoup(float(a), float(w), wq, float(p))
-                assert r == int(r) and abs(r) <= p, (a, w, r)
+                assert r == int(r) and abs(r) <= 3 * p // 4 + 1, (a, w, r)     # single-rounding quotient: 3p/4
                 assert (int(r) - a * w) % p == 0
                 if abs(a) <= 2 * p:
-                    assert abs(r) <= 3 * p // 4 + 1
+                    assert abs(r) <= 5 * p // 8 + 1
 
 
 def test_mul_barrett_bounds_and_congruence(A):
@@ -78,7 +78,7 @@ def test_mul_barrett_bounds_and_congruence(A):
         for a in vals:
             for b in vals[:40] + [rnd.randrange(-p, p + 1) for _ in range(10)]:
                 r = A.t_mul_barrett(float(a), float(b), float(p), 1.0 / p)
-                assert r == int(r) and abs(r) <= 7 * p // 8 + 1, (a, b, r)
+                assert r == int(r) and abs(r) <= 3 * p // 4 + 1, (a, b, r)
                 assert (int(r) - a * b) % p == 0
 
 
