@@ -103,7 +103,9 @@ struct PassPos {
 
 /* f(slot, e, o, c, k) for every slot of a round: e the LDS index, o the global offset in the tile, c the column inside
  * the tile, k the bit-reversed row */
-template <int LOGR, int PTS, typename F>
+/* PAD: e is the padded LDS index (SwzLds below) of the element; eb and the slot offsets j 2^L have disjoint bits, so
+ * pad(eb + j 2^L) = pad(eb) + pad(j 2^L), the second term a compile-time constant in the fixed-shape kernels */
+template <int LOGR, int PTS, bool PAD = false, typename F>
 __device__ __forceinline__ void for_slots(const PassPos &pp, int s_lo, F f)
 {
     constexpr int R = 1 << LOGR, G = PTS / R;
@@ -112,14 +114,18 @@ __device__ __forceinline__ void for_slots(const PassPos &pp, int s_lo, F f)
     for (int g = 0; g < G; g++) {
         const uint32_t tau = (uint32_t)(g * pp.nthr + pp.t);
         const uint32_t eb = (tau & ((1u << L) - 1)) | ((tau >> L) << (L + LOGR));
+        const uint32_t pe = PAD ? eb + (eb >> 5) : eb;
         const uint32_t c = eb & ((1u << pp.logw) - 1), r = (eb >> pp.logw) & ((1u << pp.m) - 1);
         const uint32_t h = eb >> (pp.logw + pp.m);
         const uint32_t ob = (h << (pp.lo + pp.m)) + (r << pp.lo) + c;
         const uint32_t kb = pp.m ? (__brev(r) >> (32 - pp.m)) : 0u;
         const int ksh = pp.m - s_lo - LOGR;
 #pragma unroll
-        for (int j = 0; j < R; j++)
-            f(g * R + j, eb + ((uint32_t)j << L), ob + ((uint32_t)j << (s_lo + pp.lo)), c, kb + (brev_c<LOGR>(j) << ksh));
+        for (int j = 0; j < R; j++) {
+            const uint32_t dj = (uint32_t)j << L;
+            f(g * R + j, pe + (PAD ? dj + (dj >> 5) : dj), ob + ((uint32_t)j << (s_lo + pp.lo)), c,
+              kb + (brev_c<LOGR>(j) << ksh));
+        }
     }
 }
 
@@ -178,6 +184,18 @@ __device__ __forceinline__ uint64_t y_index(const PassArgs &a, uint64_t gi)
                                        : ((((gi >> a.logn) >> 1) << a.logn) | (gi & nmask));
 }
 
+/* The tile in LDS, padded: element e at word e + e / 32 (LDS_WORDS(T) words for T elements).  A round over the lowest
+ * row bits of a contiguous tile has lanes 16 elements apart, which unpadded all hit the same bank pair; padded, every
+ * round, the linear staging and the strided tiles' rounds spread over all banks.  For a group's slots
+ * e = eb + j 2^L the padded index is pad(eb) + a compile-time multiple of j (L >= 5, or the slot bits below bit 5 with
+ * eb's bits clear), so the offsets stay immediate. */
+#define LDS_WORDS(T) ((T) + ((T) >> 5))
+struct SwzLds {
+    double *p;
+    __device__ __forceinline__ double &operator[](uint32_t e) const { return p[e + (e >> 5)]; }
+    __device__ __forceinline__ double &at(uint32_t pe) const { return p[pe]; }   /* an index padded already */
+};
+
 enum { PM_FWD = 0, PM_FWD_LIMB = 1, PM_INV = 2, PM_INV_PW = 3 };
 
 /* w_N^(col k) for N = 2^lom with the exponent wrapped to 32 bits (only its residue mod 2^lom matters) */
@@ -198,7 +216,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 {
     const int gm = GM >= 0 ? GM : a.m, gw = GW >= 0 ? GW : a.logw, gh = GH >= 0 ? GH : a.logh;
     constexpr bool INV = MODE >= PM_INV;
-    extern __shared__ double lds[];                             /* T doubles, sized at launch */
+    extern __shared__ double lds_raw[];                         /* LDS_WORDS(T) doubles, sized at launch */
+    const SwzLds lds{lds_raw};
     PassPos pp; pp.m = gm; pp.logw = gw; pp.lo = a.lo;
     const int T = 1 << (gm + gw + gh);
     pp.nthr = T / PTS; pp.t = threadIdx.x;
@@ -239,29 +258,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int sl = 0; sl < PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; lds[e] = load_at(e, 0, 0); }
         }
         __syncthreads();
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                       uint32_t) { v[sl] = lds[e]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { v[sl] = lds.at(e); });)
     }
 #pragma unroll
     for (int i = 0; i < (GM >= 0 ? (GM + MAXR - 1) / MAXR : nr); i++) {
         round_split(gm, i, &s_lo, &logR, INV, MAXR);
         if (i > 0) {
             __syncthreads();
-            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                           uint32_t) { v[sl] = lds[e]; });)
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                           uint32_t) { v[sl] = lds.at(e); });)
         }
         if (active) ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
         if (i + 1 < nr && active)
-            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                               uint32_t) { lds[e] = v[sl]; });)
+            ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                               uint32_t) { lds.at(e) = v[sl]; });)
     }
     /* last round: the same choice for the store (each thread rewrites only the LDS words it read in this round) */
     if (gw + s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t c,
                                                                        uint32_t k) { store_at(o, c, k, v[sl]); });)
     } else {
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                       uint32_t) { lds[e] = v[sl]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { lds.at(e) = v[sl]; });)
         __syncthreads();
         if (active) {
 #pragma unroll
@@ -428,7 +447,8 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
 template <bool LIMB, int PTS>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(PassArgs a)
 {
-    __shared__ double xs[1 << FUSED_LOG];
+    __shared__ double xs_raw[LDS_WORDS(1 << FUSED_LOG)];
+    const SwzLds xs{xs_raw};
     constexpr int MAXR = PTS >= 16 ? 4 : 3;
     PassPos pp; pp.m = a.m; pp.logw = 0; pp.lo = 0;
     const int T = 1 << (a.m + a.logh);
@@ -453,19 +473,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int sl = 0; sl < PTS; sl++) { const uint32_t e = sl * pp.nthr + pp.t; xs[e] = in_at(e); }
         }
         __syncthreads();
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                            uint32_t) { v[sl] = xs[e]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { v[sl] = xs.at(e); });)
     }
     for (int i = 0; i < nr; i++) {                               /* forward, top bits first */
         round_split(a.m, i, &s_lo, &logR, false, MAXR);
         if (i > 0) {
             __syncthreads();
-            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t,
-                                                                                uint32_t, uint32_t) { v[sl] = xs[e]; });)
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t,
+                                                                                uint32_t, uint32_t) { v[sl] = xs.at(e); });)
         }
         if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                            uint32_t) { xs[e] = v[sl]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { xs.at(e) = v[sl]; });)
     }
     __syncthreads();
     if (active) {                                                /* the pointwise product, linear: y read coalesced */
@@ -478,12 +498,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = 0; i < nr; i++) {                               /* inverse, bit 0 first */
         round_split(a.m, i, &s_lo, &logR, true, MAXR);
         __syncthreads();
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                            uint32_t) { v[sl] = xs[e]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { v[sl] = xs.at(e); });)
         if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
         if (i + 1 < nr && active)
-            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                    uint32_t) { xs[e] = v[sl]; });)
+            ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                    uint32_t) { xs.at(e) = v[sl]; });)
     }
     if (s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
@@ -492,8 +512,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
             xt[o] = x; });)
     } else {
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                            uint32_t) { xs[e] = v[sl]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                            uint32_t) { xs.at(e) = v[sl]; });)
         __syncthreads();
         if (active) {
 #pragma unroll
@@ -514,7 +534,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
  * One read of A (or its limbs), one read of B (or its limbs), one write of A. */
 template <int PTS, int MAXR, bool LIMBS>
 __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &pp, bool active, uint64_t gbase,
-                                             double *xs, double *v, double p, double pinv)
+                                             const SwzLds &xs, double *v, double p, double pinv)
 {
     const bool limb = LIMBS && (q.limbs || q.ldesc);
     const double *xt = q.x + gbase;
@@ -534,28 +554,29 @@ __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &p
             }
         }
         __syncthreads();
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                       uint32_t) { v[sl] = xs[e]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { v[sl] = xs.at(e); });)
     }
 #pragma unroll
     for (int i = 0; i < nr; i++) {                               /* top bits first; the last round stays in v */
         round_split(q.m, i, &s_lo, &logR, false, MAXR);
         if (i > 0) {
             __syncthreads();
-            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                           uint32_t) { v[sl] = xs[e]; });)
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                           uint32_t) { v[sl] = xs.at(e); });)
         }
         if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, q.tw.stage, p, pinv);)
         if (i + 1 < nr && active)
-            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                               uint32_t) { xs[e] = v[sl]; });)
+            ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                               uint32_t) { xs.at(e) = v[sl]; });)
     }
 }
 
 template <int PTS, int GM = -1, int GH = -1, bool LIMBS = true>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused2(PassArgs a, PassArgs bq)
 {
-    __shared__ double xs[1 << FUSED_LOG];
+    __shared__ double xs_raw[LDS_WORDS(1 << FUSED_LOG)];
+    const SwzLds xs{xs_raw};
     constexpr int MAXR = PTS >= 16 ? 4 : 3;
     if (GM >= 0) { a.m = GM; bq.m = GM; }
     if (GH >= 0) { a.logh = GH; bq.logh = GH; }
@@ -582,21 +603,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         round_split(a.m, i, &s_lo, &logR, true, MAXR);
         if (i > 0) {
             __syncthreads();
-            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                           uint32_t) { v[sl] = xs[e]; });)
+            if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                           uint32_t) { v[sl] = xs.at(e); });)
         }
         if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
         if (i + 1 < nr && active)
-            ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                               uint32_t) { xs[e] = v[sl]; });)
+            ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                               uint32_t) { xs.at(e) = v[sl]; });)
     }
     if (s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
                                                                        uint32_t) { xt[o] = v[sl]; });)
     } else {
         /* each thread rewrites only the LDS words it read in this round */
-        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
-                                                                       uint32_t) { xs[e] = v[sl]; });)
+        if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
+                                                                       uint32_t) { xs.at(e) = v[sl]; });)
         __syncthreads();
         if (active) {
 #pragma unroll
@@ -713,7 +734,7 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
     if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
     const dim3 grid((unsigned)tiles), blk(nthr);
-    const size_t lds = (size_t)T * sizeof(double);
+    const size_t lds = (size_t)LDS_WORDS(T) * sizeof(double);
 #define NTTM_PASS_LAUNCH(P) \
     switch (mode) { \
     case PM_FWD: hipLaunchKernelGGL((k_pass<PM_FWD, P>), grid, blk, lds, s, b); break; \
