@@ -127,34 +127,56 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
     const uint32_t first = (uint32_t)dsc.first_block, local = blk - first;
     const uint64_t k0 = (uint64_t)local * CRT_CH;
 
-    /* digits of coefficients k0 - HALO .. k0 + CH - 1 into LDS (coalesced plane reads) */
-    for (int i = -1; i < CRT_PER; i++) {
-        int li; uint64_t kk;
-        if (i < 0) { if (tid >= CRT_HALO) continue; li = tid; kk = k0 + tid - CRT_HALO; if (k0 < CRT_HALO - tid) { for (int j = 0; j < NP; j++) D[j][li] = 0; continue; } }
-        else { li = CRT_HALO + i * CRT_T + tid; kk = k0 + i * CRT_T + tid; }
-        uint64_t d[NP];
-        if (kk < dsc.ncoef) {
-            double r[NP];
+    /* digits of coefficients k0 - HALO .. k0 + CH - 1 into LDS (coalesced plane reads, all of a thread's residues
+     * requested before the first is used) */
+    uint64_t xk[CRT_PER];
+    {
+        const bool halo = tid < CRT_HALO && k0 + tid >= CRT_HALO;   /* coefficient k0 - HALO + tid exists */
+        /* the planes as global-space pointers: generic (flat) loads would also count against the LDS counter, whose
+         * limit of 15 outstanding operations would serialize the prefetch */
+        const __attribute__((address_space(1))) double *pl = (const __attribute__((address_space(1))) double *)dsc.pl;
+        const __attribute__((address_space(1))) uint64_t *xg = (const __attribute__((address_space(1))) uint64_t *)dsc.x;
+        double rr[CRT_PER + 1][NP];
 #pragma unroll
-            for (int j = 0; j < NP; j++) r[j] = dsc.pl[j * dsc.plane + kk];
-            crt_digits<NP>(a.cc, r, d);
-        } else {
+        for (int i = 0; i < CRT_PER; i++) {
+            const uint64_t kk = k0 + i * CRT_T + tid;
 #pragma unroll
-            for (int j = 0; j < NP; j++) d[j] = 0;
+            for (int j = 0; j < NP; j++) rr[i][j] = kk < dsc.ncoef ? pl[j * dsc.plane + kk] : 0.0;
+        }
+        {
+            const uint64_t kk = k0 + tid - CRT_HALO;
+#pragma unroll
+            for (int j = 0; j < NP; j++) rr[CRT_PER][j] = halo && kk < dsc.ncoef ? pl[j * dsc.plane + kk] : 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < NP; j++) D[j][li] = d[j];
+        for (int i = 0; i < CRT_PER; i++) {                          /* x at the limbs of the limb phase below */
+            const uint64_t k = k0 + wave * 64 * CRT_PER + i * 64 + lane;
+            xk[i] = k < dsc.nx && k < dsc.nc ? xg[k] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < CRT_PER; i++) {
+            uint64_t d[NP];
+            crt_digits<NP>(a.cc, rr[i], d);                          /* zero residues give zero digits */
+#pragma unroll
+            for (int j = 0; j < NP; j++) D[j][CRT_HALO + i * CRT_T + tid] = d[j];
+        }
+        if (tid < CRT_HALO) {
+            uint64_t d[NP];
+            crt_digits<NP>(a.cc, rr[CRT_PER], d);
+#pragma unroll
+            for (int j = 0; j < NP; j++) D[j][tid] = d[j];
+        }
     }
     __syncthreads();
 
     /* limb k = k0 + wave 512 + i 64 + lane for i < 8: one ballot per i gives 64 consecutive limbs */
     /* v_k = sum_j d_j[k - j] + x_k < NP B + 2^31: its quotient by B is at most NP */
-    auto vsum = [&](uint64_t k, int li) -> uint64_t {
+    auto vsum = [&](uint64_t k, int li, uint64_t xv) -> uint64_t {
         uint64_t v = 0;
         if (k < dsc.nc) {
 #pragma unroll
             for (int j = 0; j < NP; j++) v += D[j][li - j];
-            if (k < dsc.nx) v += dsc.x[k];
+            v += xv;
         }
         return v;
     };
@@ -172,12 +194,12 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         const int off = wave * 64 * CRT_PER + i * 64 + lane;          /* offset of limb k inside the block */
         const uint64_t k = k0 + off;
         const int li = CRT_HALO + off;
-        const uint64_t v = vsum(k, li);
+        const uint64_t v = vsum(k, li, xk[i]);
         const uint32_t q = quot(v);
         /* q_{k-1}: from the lane below; lane 0 from the previous word's lane 63, or (first word) computed */
         uint32_t qp = __shfl_up(q, 1, 64);
         if (i > 0) { if (lane == 0) qp = qprev; }
-        else if (lane == 0) qp = k >= 1 ? quot(vsum(k - 1, li - 1)) : 0u;
+        else if (lane == 0) qp = k >= 1 ? quot(vsum(k - 1, li - 1, k - 1 < dsc.nx ? dsc.x[k - 1] : 0)) : 0u;
         qprev = __builtin_amdgcn_readlane(q, 63);
         const uint64_t w = v - (uint64_t)q * NTTM_B + qp;            /* <= B + NP */
         wv[i] = w;
@@ -252,6 +274,7 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
     __syncthreads();
 
     /* final limbs: out_k = w_k + o_{k-1} - B o_k, stored directly (lanes hold consecutive limbs) */
+    __attribute__((address_space(1))) uint64_t *cg = (__attribute__((address_space(1))) uint64_t *)dsc.c;
     uint32_t c = map_apply(P, s_cin);                               /* the carry into this wave's first word */
 #pragma unroll
     for (int i = 0; i < CRT_PER; i++) {
@@ -262,7 +285,7 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         const uint32_t oout = lane == 63 ? cout : (uint32_t)(CI >> (lane + 1)) & 1u;
         const int off = wave * 64 * CRT_PER + i * 64 + lane;
         const uint64_t k = k0 + off;
-        if (k < dsc.nc) dsc.c[k] = wv[i] + oin - (oout ? NTTM_B : 0);
+        if (k < dsc.nc) cg[k] = wv[i] + oin - (oout ? NTTM_B : 0);
         if (k + 1 == dsc.nc) {                                     /* the last limb: nothing may remain beyond it */
             uint64_t extra = qlast + oout;
             const int li = CRT_HALO + off;
