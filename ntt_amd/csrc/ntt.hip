@@ -145,6 +145,55 @@ __device__ __forceinline__ void round_compute(double *v, const PassPos &pp, int 
     }
 }
 
+/* dit_group with the inverse twiddles taken from the FORWARD stage table: w^-j of stage s is the mirrored forward entry
+ * negated, w^-j = -w^(2^s - j) (w^(2^s) = -1), the negation folded into the butterfly (the product's sign flipped), and
+ * w^0 = 1 taken from entry 1 (stage 0).  Lets a kernel keep one small table in LDS for both directions. */
+template <int LOGR>
+__device__ __forceinline__ void dit_group_mirror(double *v, int s_lo, uint32_t rlow, const double2 *tw, double p,
+                                                 double pinv)
+{
+    constexpr int R = 1 << LOGR;
+#pragma unroll
+    for (int lev = 0; lev < LOGR; lev++) {
+        const int k = lev, h = 1 << k, s = s_lo + k;
+        const bool red = lev == LOGR - 1;
+#pragma unroll
+        for (int jj = 0; jj < h; jj++) {
+            const bool one = lev == 0 && NTTM_TRIVIAL_W(rlow, jj);
+            const uint32_t j = rlow + ((uint32_t)jj << s_lo);
+            double2 w;                                   /* (w, wq) negated for a mirrored entry */
+            if (jj == 0) {
+                const bool z = rlow == 0;
+                const double2 t = tw[z ? 1u : (2u << s) - j];
+                w = z ? t : make_double2(-t.x, -t.y);
+            } else {
+                const double2 t = tw[(2u << s) - j];
+                w = make_double2(-t.x, -t.y);
+            }
+#pragma unroll
+            for (int i = jj; i < R; i += 2 * h) {
+                const double u = v[i], x = one ? v[i + h] : mul_shoup(v[i + h], w.x, w.y, p);
+                const double s1 = u + x, s2 = u - x;
+                v[i] = red ? reduce(s1, p, pinv) : s1;
+                v[i + h] = red ? reduce(s2, p, pinv) : s2;
+            }
+        }
+    }
+}
+template <int LOGR, int PTS>
+__device__ __forceinline__ void round_compute_mirror(double *v, const PassPos &pp, int s_lo, const double2 *tw, double p,
+                                                     double pinv)
+{
+    constexpr int R = 1 << LOGR, G = PTS / R;
+    const int L = pp.logw + s_lo;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t tau = (uint32_t)(g * pp.nthr + pp.t);
+        const uint32_t rlow = (tau & ((1u << L) - 1)) >> pp.logw;
+        dit_group_mirror<LOGR>(v + g * R, s_lo, rlow, tw, p, pinv);
+    }
+}
+
 /* rounds of a pass: sizes of at most maxr bits (the log2 of the points per thread, at most 4), balanced; forward from
  * the top bits down, inverse from bit 0 up */
 __device__ __forceinline__ int round_count(int m, int maxr) { return (m + maxr - 1) / maxr; }
@@ -532,9 +581,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
  * same bits [0, logR) with the same slot layout, so B's transformed tile stays in registers (bv) and is never written;
  * the product v * bv and the first inverse round follow A's last forward round without an LDS round trip.
  * One read of A (or its limbs), one read of B (or its limbs), one write of A. */
+/* stw: the forward stage table's first 2^STW_LOG entries in LDS (nullptr: none); rounds whose stages all lie below
+ * STW_LOG read their twiddles there */
+#define STW_LOG 8
 template <int PTS, int MAXR, bool LIMBS>
 __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &pp, bool active, uint64_t gbase,
-                                             const SwzLds &xs, double *v, double p, double pinv)
+                                             const SwzLds &xs, double *v, double p, double pinv,
+                                             const double2 *stw = nullptr)
 {
     const bool limb = LIMBS && (q.limbs || q.ldesc);
     const double *xt = q.x + gbase;
@@ -565,7 +618,9 @@ __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &p
             if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                            uint32_t) { v[sl] = xs.at(e); });)
         }
-        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, q.tw.stage, p, pinv);)
+        /* with stw (the 4096-point tile in rounds of 4 bits), rounds 1 and 2 cover stages 7..0 */
+        const double2 *tw = stw && i > 0 ? stw : q.tw.stage;
+        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, tw, p, pinv);)
         if (i + 1 < nr && active)
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                uint32_t) { xs.at(e) = v[sl]; });)
@@ -576,8 +631,16 @@ template <int PTS, int GM = -1, int GH = -1, bool LIMBS = true>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused2(PassArgs a, PassArgs bq)
 {
     __shared__ double xs_raw[LDS_WORDS(1 << FUSED_LOG)];
+    /* the fixed shape keeps the stage table's low entries in LDS (the inverse mirrors them) */
+    constexpr bool LT = GM >= 0 && PTS >= 16;
+    static_assert(!LT || GM == 12, "the LDS stage table serves the rounds of 4 bits of a 4096-point tile");
+    __shared__ double2 stw_lds[LT ? 1 << STW_LOG : 1];
     const SwzLds xs{xs_raw};
     constexpr int MAXR = PTS >= 16 ? 4 : 3;
+    if (LT) {
+        for (int i = threadIdx.x; i < (1 << STW_LOG); i += blockDim.x) stw_lds[i] = a.tw.stage[i];
+        /* visible to the rounds after the first round's barrier; the first round's stages lie above STW_LOG */
+    }
     if (GM >= 0) { a.m = GM; bq.m = GM; }
     if (GH >= 0) { a.logh = GH; bq.logh = GH; }
     PassPos pp; pp.m = a.m; pp.logw = 0; pp.lo = 0;
@@ -591,9 +654,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     double v[PTS], bv[PTS];
     int s_lo, logR;
 
-    tile_forward<PTS, MAXR, LIMBS>(bq, pp, active, gbase, xs, bv, p, pinv);
+    const double2 *stw = LT ? (const double2 *)stw_lds : nullptr;
+    tile_forward<PTS, MAXR, LIMBS>(bq, pp, active, gbase, xs, bv, p, pinv, stw);
     __syncthreads();                                             /* B's last LDS reads before A's first writes */
-    tile_forward<PTS, MAXR, LIMBS>(a, pp, active, gbase, xs, v, p, pinv);
+    tile_forward<PTS, MAXR, LIMBS>(a, pp, active, gbase, xs, v, p, pinv, stw);
     if (active) {
 #pragma unroll
         for (int sl = 0; sl < PTS; sl++) v[sl] = mul_barrett(v[sl], bv[sl], p, pinv);
@@ -606,7 +670,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                            uint32_t) { v[sl] = xs.at(e); });)
         }
-        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+        if (LT && i < 2) {                                       /* inverse rounds 0 and 1: stages 0..7 */
+            if (active) ROUND_SWITCH(logR, round_compute_mirror<LR, PTS>(v, pp, s_lo, stw, p, pinv);)
+        } else {
+            if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+        }
         if (i + 1 < nr && active)
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                uint32_t) { xs.at(e) = v[sl]; });)
