@@ -463,8 +463,17 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
             if (LIMB && i == 0) {                   /* limbs to residues, zero beyond the operand */
                 const uint32_t lim = limb_lim(gbase);
                 const uint64_t *cl = (const uint64_t *)cur;
-                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
-                    v[sl] = o < lim ? limb_to_res(o + 1 == lim ? last : cl[e], p, pinv) : 0.0; });
+                /* slot j of the first round holds row bits 4..7 = j, so rows >= 2^(GM-1) are slots j >= 8; tiles
+                 * whose rows split into an all-inside and an all-outside half (the zero-padded upper half of a
+                 * product's operand) convert only the inside slots */
+                const uint32_t half = 1u << (GM - 1 + lo);
+                if (lim <= half && lim > half - (1u << lo) + W) {
+                    for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
+                        v[sl] = sl < PTS / 2 ? limb_to_res(cl[e], p, pinv) : 0.0; });
+                } else {
+                    for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
+                        v[sl] = o < lim ? limb_to_res(o + 1 == lim ? last : cl[e], p, pinv) : 0.0; });
+                }
             } else
                 for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { v[sl] = cur[e]; });
             if (INV && i == 0) {
