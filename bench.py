@@ -155,7 +155,8 @@ def bench_leaves(nttm, reps):
 
 
 PROF_NAMES = (("k_fused2", "k_fused2"), ("k_pass_stream<0", "k_pass<fwd>"), ("k_pass_stream<1", "k_pass<fwd,limb>"),
-              ("k_pass_stream<2", "k_pass<inv>"), ("k_crt<", "k_crt"))
+              ("k_pass_stream<2", "k_pass<inv>"), ("k_crt<", "k_crt"), ("k_crt_scan", "k_crt_scan"),
+              ("k_crt_fix", "k_crt_fix"))
 
 
 def trace_summary(db):

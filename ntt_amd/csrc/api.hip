@@ -31,6 +31,7 @@ int nttm_run_load_batch(double *dst, uint64_t plane, const void *desc, int which
 int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks, int np, const CrtConst &cc,
                           uint32_t *scratch, hipStream_t s, double alg_bytes);
 uint64_t nttm_crt_blocks(uint64_t nc);
+uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
 
 /* ---- errors ----------------------------------------------------------------------------------------------------- */
@@ -400,7 +401,7 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", logn, NTTM_LOGN_MAX);
     const uint64_t n = (uint64_t)1 << logn;
     const size_t planes_bytes = (size_t)2 * np * n * 8;
-    const size_t crt_words = nttm_crt_blocks(nc) + 2;
+    const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
     const size_t off_desc = align_up(planes_bytes, 256), off_crt = off_desc + 256;
     RCK(ws_need(off_crt + crt_words * 4 + 256));
     double *A = (double *)g.ws, *Bp = A + (size_t)np * n;
@@ -421,7 +422,6 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     float ms;
     for (int k = 0; k < 3; k++) { HIPCK(hipEventElapsedTime(&ms, g.ev[k], g.ev[k + 1])); g.last_ms[k] = ms; }
     const uint32_t flags = ((const uint32_t *)(g.hpin + 64))[1];
-    if (flags & 2) return nttm_fail(NTTM_ECHECK, "crt: carry look-back stalled");
     if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond the top limb (wrong residues)");
     return 0;
 }
@@ -513,7 +513,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         const size_t off_ld = align_up(off_cd + cnt * sizeof(CrtDesc), 256);
         const size_t off_ncn = align_up(off_ld + cnt * sizeof(LoadDesc), 256);
         const size_t off_crt = align_up(off_ncn + cnt * 8, 256);
-        RCK(ws_need(off_crt + (nblk + 2) * 4 + 256));
+        RCK(ws_need(off_crt + nttm_crt_scratch_words(nblk) * 4 + 256));
         RCK(hpin_need(cnt * (sizeof(CrtDesc) + sizeof(LoadDesc)) + 4096));
         double *A = (double *)g.ws, *Bp = A + (size_t)np * cnt * n;
         for (uint64_t j = 0; j < cnt; j++) cd[j].pl = A + j * n;
@@ -532,8 +532,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         HIPCK(hipMemcpyAsync(hp + cnt * 8, g.ws + off_crt, 8, hipMemcpyDeviceToHost, s));
         HIPCK(hipStreamSynchronize(s));
         const uint32_t flags = ((const uint32_t *)(hp + cnt * 8))[1];
-        if (flags & 2) return nttm_fail(NTTM_ECHECK, "crt: carry look-back stalled");
-        if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
+            if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
         for (uint64_t j = 0; j < cnt; j++) P[G[j]].ncn = ((const uint64_t *)hp)[j];
     }
     /* zero-operand products: normalized length of x */
@@ -614,7 +613,7 @@ extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, i
     if (!nc) return 0;
     hipStream_t s = (hipStream_t)stream;
     const size_t pbytes = align_up((size_t)np * ncoef * 8, 256);
-    const size_t crt_words = nttm_crt_blocks(nc) + 2;
+    const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
     RCK(ws_need(pbytes + 256 + crt_words * 4 + 256));
     double *pl = (double *)g.ws;
     for (int i = 0; i < np; i++) {
@@ -632,7 +631,6 @@ extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, i
     HIPCK(hipMemcpyAsync(g.hpin + 64, scratch, 8, hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
     const uint32_t flags = ((const uint32_t *)(g.hpin + 64))[1];
-    if (flags & 2) return nttm_fail(NTTM_ECHECK, "crt: carry look-back stalled");
     if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: the value needs more than nc limbs");
     return 0;
 }

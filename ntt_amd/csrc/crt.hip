@@ -9,9 +9,12 @@
  * q_k + o_k where
  * the overflow bit o_k = [w_k + o_{k-1} >= B] = g_k | (p_k & o_{k-1}), g_k = [w_k >= B], p_k = [w_k = B - 1]: a binary
  * carry-lookahead chain.  For 64 consecutive limbs it is resolved by one 64-bit addition of the ballot masks
- * X = G, Y = G | P (the carries into the bits of X + Y + cin are the o's, the textbook adder identity); words chain
- * inside a wave, waves inside a block, and blocks by a decoupled look-back over (cin -> cout) maps of one bit
- * (blocks take consecutive tickets and publish their map before looking back, so every wait is on a running block).
+ * X = G, Y = G | P (the carries into the bits of X + Y + cin are the o's, the textbook adder identity).  Words chain
+ * inside a wave and waves inside a block; each block then stores its limbs for a carry in of 0 together with its
+ * (cin -> cout) map of one bit.  One workgroup scans the maps of all blocks (restarting at every product's first
+ * block) into each block's carry in, and a last pass adds 1 to the limbs of the blocks that receive a carry: the
+ * limbs for cin = 1 are those for cin = 0 plus one, a ripple through the leading limbs equal to B - 1.  No block
+ * waits on another, and the scan reads one word per 2048 limbs.
  * The reference instead ripples each 256-coefficient chunk on one thread of a wave64 ballot scan, writes a 4-limb spill
  * per stripe and merges the spills serially on the host or on one thread per product (E/rns_mul.c:452-470;
  * E/dbig.c:1007-1052).  The final carry beyond limb nc-1 must be zero; a non-zero one sets an error flag.
@@ -23,9 +26,13 @@
 #define CRT_CH (CRT_T * CRT_PER)
 #define CRT_HALO 4                       /* digits of coefficients k0-4 .. k0-1: v_{k0-1} with four primes */
 #define CRT_ID_MAP 0x2u                  /* one-bit carry maps: bit c = cout(cin = c); the identity is 0b10 */
-#define CRT_ST_AGG 1u
-#define CRT_ST_INC 2u
-#define CRT_SPIN_MAX (1u << 22)
+#define CRT_CTL_WORDS 4                  /* control words before the block status words (16-byte aligned) */
+/* block status word: map (2 bits) | error for cin 0 << 2 | error for cin 1 << 3 | carry in << 4 | first of product << 5 */
+#define CRT_ST_E0 (1u << 2)
+#define CRT_ST_E1 (1u << 3)
+#define CRT_ST_CIN (1u << 4)
+#define CRT_ST_FIRST (1u << 5)
+#define CRT_SCAN_T 1024
 
 struct CrtDesc {                         /* one product; must match api.hip */
     const double *pl; uint64_t plane;    /* residues of prime j, coefficient k at pl[j * plane + k] */
@@ -36,8 +43,8 @@ struct CrtDesc {                         /* one product; must match api.hip */
 };
 struct CrtArgs {
     const CrtDesc *desc; uint64_t nprod;
-    uint32_t *status;                    /* one word per block: state << 30 | carry out << 2 | aggregate map (2 bits) */
-    uint32_t *ctl;                       /* [0] ticket counter, [1] error flags */
+    uint32_t *status;                    /* one word per block (CRT_ST_*) */
+    uint32_t *ctl;                       /* [1] error flags */
     CrtConst cc;
 };
 
@@ -112,17 +119,16 @@ template <int NP>
 __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
 {
     __shared__ uint64_t D[NP][CRT_CH + CRT_HALO];
-    __shared__ uint32_t s_blk, s_cin, s_prod;
+    __shared__ uint32_t s_prod, s_err;
     __shared__ uint32_t s_wave[CRT_T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t blk = blockIdx.x;
     if (tid == 0) {
-        const uint32_t b = atomicAdd(&a.ctl[0], 1u);
-        uint64_t lo = 0, hi = a.nprod - 1;                  /* the product whose block range holds ticket b */
-        while (lo < hi) { const uint64_t mid = (lo + hi + 1) >> 1; if (a.desc[mid].first_block <= b) lo = mid; else hi = mid - 1; }
-        s_blk = b; s_prod = (uint32_t)lo;
+        uint64_t lo = 0, hi = a.nprod - 1;                  /* the product whose block range holds blk */
+        while (lo < hi) { const uint64_t mid = (lo + hi + 1) >> 1; if (a.desc[mid].first_block <= blk) lo = mid; else hi = mid - 1; }
+        s_prod = (uint32_t)lo; s_err = 0;
     }
     __syncthreads();
-    const uint32_t blk = s_blk;
     const CrtDesc dsc = a.desc[s_prod];
     const uint32_t first = (uint32_t)dsc.first_block, local = blk - first;
     const uint64_t k0 = (uint64_t)local * CRT_CH;
@@ -222,60 +228,10 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         A = map_compose(s_wave[w], A);
     }
 
-    /* carry into the block: decoupled look-back over the earlier blocks of the product, 64 at a time by wave 0.  The
-     * status word carries its whole payload (map and carry), so the atomics are relaxed at agent scope: no cache
-     * write-back or invalidate is needed around them.  Lane i
-     * reads block blk-1-i; the nearest block with an inclusive carry ends the walk, the aggregate maps of the blocks
-     * in between are composed (nearest outermost).  Blocks before the product's first act as an inclusive carry 0. */
-    if (wave == 0) {
-        if (lane == 0)
-            __hip_atomic_store(&a.status[blk], local == 0 ? ((CRT_ST_INC << 30) | (map_apply(A, 0) << 2) | A)
-                                                          : ((CRT_ST_AGG << 30) | A),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t cin = 0;
-        if (local != 0) {
-            uint32_t G = CRT_ID_MAP, spins = 0;
-            int64_t base = (int64_t)blk - 1;
-            for (;;) {
-                const int64_t idx = base - lane;
-                const uint32_t st = idx >= (int64_t)first
-                    ? __hip_atomic_load(&a.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                    : ((CRT_ST_INC << 30) | CRT_ID_MAP);
-                const uint32_t state = st >> 30;
-                const uint64_t inc = __ballot(state == CRT_ST_INC), wait = __ballot(state == 0);
-                const int fi = inc ? __ffsll((long long)inc) - 1 : 64;
-                const int fw = wait ? __ffsll((long long)wait) - 1 : 64;
-                if (fw < fi) {                               /* a block between us and the nearest inclusive one */
-                    if (++spins > CRT_SPIN_MAX) { if (lane == 0) atomicOr(&a.ctl[1], 2u); cin = 0; break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                uint32_t m = lane < fi ? (st & 3u) : CRT_ID_MAP;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {           /* m_0 o m_1 o ... o m_63 into lane 0 */
-                    const uint32_t o = __shfl_down(m, d, 64);
-                    if (lane + d < 64) m = map_compose(m, o);
-                }
-                m = __shfl(m, 0, 64);
-                G = map_compose(G, m);
-                if (fi < 64) {
-                    const uint32_t c_inc = (__shfl(st, fi, 64) >> 2) & 1u;
-                    cin = map_apply(G, c_inc);
-                    break;
-                }
-                base -= 64;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&a.status[blk], (CRT_ST_INC << 30) | (map_apply(A, cin) << 2) | A, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (lane == 0) s_cin = cin;
-    }
-    __syncthreads();
-
-    /* final limbs: out_k = w_k + o_{k-1} - B o_k, stored directly (lanes hold consecutive limbs) */
+    /* final limbs for a carry of 0 into the block: out_k = w_k + o_{k-1} - B o_k, stored directly (lanes hold
+     * consecutive limbs).  c1 follows the carries for a carry of 1 into the block, for the check of the last limb. */
     __attribute__((address_space(1))) uint64_t *cg = (__attribute__((address_space(1))) uint64_t *)dsc.c;
-    uint32_t c = map_apply(P, s_cin);                               /* the carry into this wave's first word */
+    uint32_t c = map_apply(P, 0), c1 = map_apply(P, 1);              /* the carries into this wave's first word */
 #pragma unroll
     for (int i = 0; i < CRT_PER; i++) {
         const uint64_t S = GX[i] + GY[i] + c;
@@ -286,23 +242,149 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         const int off = wave * 64 * CRT_PER + i * 64 + lane;
         const uint64_t k = k0 + off;
         if (k < dsc.nc) cg[k] = wv[i] + oin - (oout ? NTTM_B : 0);
+        const uint32_t cout1 = add_cout(GX[i], GY[i], c1);
         if (k + 1 == dsc.nc) {                                     /* the last limb: nothing may remain beyond it */
-            uint64_t extra = qlast + oout;
+            const uint64_t CI1 = (GX[i] + GY[i] + c1) ^ GX[i] ^ GY[i];
+            const uint32_t oout1 = lane == 63 ? cout1 : (uint32_t)(CI1 >> (lane + 1)) & 1u;
+            uint64_t extra = qlast;
             const int li = CRT_HALO + off;
 #pragma unroll
             for (int e = 1; e < NP; e++)
 #pragma unroll
                 for (int j = e; j < NP; j++) extra += D[j][li + e - j];
-            if (extra) atomicOr(&a.ctl[1], 1u);
+            s_err = (extra + oout ? CRT_ST_E0 : 0u) | (extra + oout1 ? CRT_ST_E1 : 0u);
         }
-        c = cout;
+        c = cout; c1 = cout1;
     }
+    __syncthreads();
+    if (tid == 0) a.status[blk] = A | s_err | (local == 0 ? CRT_ST_FIRST : 0u);
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+/* carry in of every block: maps composed in block order, restarting at each product's first block.  One workgroup of
+ * 16 waves, each over a contiguous run of the status words (four per lane and load); the error flag of a product's
+ * last block is taken for its actual carry in. */
+__device__ __forceinline__ uint32_t crt_step_map(uint32_t st)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+    const uint32_t m = st & 3u;                                     /* a first block ignores the carry before it */
+    return (st & CRT_ST_FIRST) ? (m & 1u) * 3u : m;
+}
+__global__ __launch_bounds__(CRT_SCAN_T) void k_crt_scan(uint32_t *status, uint64_t n, uint32_t *ctl)
+{
+    __shared__ uint32_t s_map[CRT_SCAN_T / 64], s_err;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = CRT_SCAN_T / 64, U = 4;                      /* waves; loads of 4 words in flight per lane */
+    if (tid == 0) s_err = 0;
+    const uint64_t per = ((n + NW - 1) / NW + 256 * U - 1) / (256 * U) * (256 * U);
+    const uint64_t lo = (uint64_t)wave * per, hi = lo + per < n ? lo + per : n;
+    auto load4 = [&](uint64_t i, uint32_t *w) {                     /* words i..i+3 (i a multiple of 4) */
+        if (i + 4 <= hi) {
+            const uint4 v = *(const uint4 *)(status + i);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) w[j] = i + j < hi ? status[i + j] : CRT_ID_MAP;
+        }
+    };
+    /* pass 1: the map of the wave's run */
+    uint32_t F = CRT_ID_MAP;
+    for (uint64_t b = lo; b < hi; b += 256 * U) {
+        uint32_t w[U][4];
+#pragma unroll
+        for (int u = 0; u < U; u++) load4(b + u * 256 + lane * 4, w[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint32_t m = CRT_ID_MAP;
+#pragma unroll
+            for (int j = 0; j < 4; j++) m = map_compose(crt_step_map(w[u][j]), m);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {                      /* lane 0: m_63 o ... o m_0 */
+                const uint32_t o = __shfl_down(m, d, 64);
+                if ((lane & (2 * d - 1)) == 0) m = map_compose(o, m);
+            }
+            F = map_compose(__builtin_amdgcn_readfirstlane(m), F);
+        }
+    }
+    if (lane == 0) s_map[wave] = F;
+    __syncthreads();
+    uint32_t cw = 0;                                                /* the carry into the wave's run */
+    for (int v = 0; v < wave; v++) cw = map_apply(s_map[v], cw);
+    /* pass 2: carries into the blocks */
+    uint32_t err = 0;
+    for (uint64_t b = lo; b < hi; b += 256 * U) {
+        uint32_t w[U][4];
+#pragma unroll
+        for (int u = 0; u < U; u++) load4(b + u * 256 + lane * 4, w[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint32_t m = CRT_ID_MAP;
+#pragma unroll
+            for (int j = 0; j < 4; j++) m = map_compose(crt_step_map(w[u][j]), m);
+            uint32_t inc = m;                                       /* inclusive scan over the lanes */
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(inc, d, 64);
+                if (lane >= d) inc = map_compose(inc, o);
+            }
+            uint32_t ex = __shfl_up(inc, 1, 64);
+            if (lane == 0) ex = CRT_ID_MAP;
+            uint32_t ci = map_apply(ex, cw);
+            const uint64_t i0 = b + u * 256 + lane * 4;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t st = w[u][j];
+                if (st & CRT_ST_FIRST) ci = 0;
+                err |= st & (ci ? CRT_ST_E1 : CRT_ST_E0);
+                w[u][j] = (st & ~CRT_ST_CIN) | (ci ? CRT_ST_CIN : 0u);
+                ci = map_apply(st & 3u, ci);
+            }
+            if (i0 + 4 <= hi) *(uint4 *)(status + i0) = make_uint4(w[u][0], w[u][1], w[u][2], w[u][3]);
+            else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) if (i0 + j < hi) status[i0 + j] = w[u][j];
+            }
+            cw = map_apply(__builtin_amdgcn_readlane(inc, 63), cw);
+        }
+    }
+    if (__ballot(err != 0) && lane == 0) atomicOr(&s_err, 1u);
+    __syncthreads();
+    if (tid == 0) { ctl[0] = 0; ctl[1] = s_err; }
+}
+
+/* the limbs of a block whose carry in is 1: its stored limbs (for carry 0) plus one, i.e. the leading limbs equal to
+ * B - 1 become 0 and the first other one grows by 1.  A workgroup scouts 64 status words; each of its 4 waves takes
+ * every 4th block found and rewrites it (32 limbs per lane, all loads in flight at once). */
+#define CRT_FIX_W 64
+__global__ __launch_bounds__(256) void k_crt_fix(const CrtDesc *desc, uint64_t nprod, const uint32_t *status, uint64_t n)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t b0 = (uint64_t)blockIdx.x * CRT_FIX_W;
+    const uint32_t st = b0 + lane < n ? status[b0 + lane] : 0u;
+    uint64_t todo = __ballot((st & CRT_ST_CIN) != 0);
+    for (int r = 0; todo; r++) {
+        const int f = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        if ((r & 3) != wave) continue;
+        const uint64_t blk = b0 + f;
+        uint64_t lo = 0, hi = nprod - 1;
+        while (lo < hi) { const uint64_t mid = (lo + hi + 1) >> 1; if (desc[mid].first_block <= blk) lo = mid; else hi = mid - 1; }
+        const CrtDesc d = desc[lo];
+        const uint64_t k0 = (blk - d.first_block) * CRT_CH, end = k0 + CRT_CH < d.nc ? k0 + CRT_CH : d.nc;
+        uint64_t v[CRT_CH / 64];
+#pragma unroll
+        for (int j = 0; j < CRT_CH / 64; j++) { const uint64_t k = k0 + j * 64 + lane; v[j] = k < end ? d.c[k] : 0; }
+        int first_ne = CRT_CH;                                      /* offset of the first limb below B - 1 */
+#pragma unroll
+        for (int j = CRT_CH / 64 - 1; j >= 0; j--) {
+            const uint64_t ne = __ballot(k0 + j * 64 + lane < end && v[j] != NTTM_B - 1);
+            if (ne) first_ne = j * 64 + __ffsll((long long)ne) - 1;
+        }
+#pragma unroll
+        for (int j = 0; j < CRT_CH / 64; j++) {
+            const int o = j * 64 + lane;
+            const uint64_t k = k0 + o;
+            if (k < end && o <= first_ne) d.c[k] = o < first_ne ? 0 : v[j] + 1;
+        }
+    }
 }
 
 __global__ void k_norm(const CrtDesc *d, uint64_t nprod, uint64_t *ncn)
@@ -316,24 +398,36 @@ __global__ void k_norm(const CrtDesc *d, uint64_t nprod, uint64_t *ncn)
 
 uint64_t nttm_crt_blocks(uint64_t nc) { return (nc + CRT_CH - 1) / CRT_CH; }
 
-/* desc_dev: nprod descriptors in device memory whose block ranges tile [0, nblocks); scratch: nblocks + 2 words */
+uint64_t nttm_crt_scratch_words(uint64_t nblocks) { return nblocks + CRT_CTL_WORDS; }
+
+/* desc_dev: nprod descriptors in device memory whose block ranges tile [0, nblocks); scratch: 16-byte aligned,
+ * nttm_crt_scratch_words(nblocks) words, of which word 1 receives the error flags */
 int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks, int np, const CrtConst &cc,
                           uint32_t *scratch, hipStream_t s, double alg_bytes)
 {
     if (np < 3 || np > NTTM_MAX_PRIMES) return nttm_fail(NTTM_EINVAL, "crt: np = %d", np);
     if (!nprod || !nblocks) return 0;
     if (nblocks > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "crt: too many blocks");
+    if ((uintptr_t)scratch & 15) return nttm_fail(NTTM_EINVAL, "crt: scratch not 16-byte aligned");
     CrtArgs a;
     a.desc = (const CrtDesc *)desc_dev; a.nprod = nprod;
-    a.ctl = scratch; a.status = scratch + 2; a.cc = cc;
-    const uint64_t fill = nblocks + 2;
-    const unsigned fg = (unsigned)((fill + 255) / 256 < 4096 ? (fill + 255) / 256 : 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, s, scratch, fill, 0u);
+    a.ctl = scratch; a.status = scratch + CRT_CTL_WORDS; a.cc = cc;
     prof_begin("k_crt", alg_bytes, s);
     if (np == 3) hipLaunchKernelGGL(k_crt<3>, dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
     else hipLaunchKernelGGL(k_crt<4>, dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
     prof_end(s);
-    return nttm_hip_check(hipGetLastError(), "k_crt launch");
+    int rc = nttm_hip_check(hipGetLastError(), "k_crt launch");
+    if (rc) return rc;
+    prof_begin("k_crt_scan", 8.0 * (double)nblocks, s);
+    hipLaunchKernelGGL(k_crt_scan, dim3(1), dim3(CRT_SCAN_T), 0, s, a.status, nblocks, a.ctl);
+    prof_end(s);
+    rc = nttm_hip_check(hipGetLastError(), "k_crt_scan launch");
+    if (rc) return rc;
+    prof_begin("k_crt_fix", 4.0 * (double)nblocks, s);
+    hipLaunchKernelGGL(k_crt_fix, dim3((unsigned)((nblocks + CRT_FIX_W - 1) / CRT_FIX_W)), dim3(256), 0, s, a.desc, nprod,
+                       (const uint32_t *)a.status, nblocks);
+    prof_end(s);
+    return nttm_hip_check(hipGetLastError(), "k_crt_fix launch");
 }
 
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s)

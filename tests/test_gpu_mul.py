@@ -144,6 +144,25 @@ def test_batch_equals_each_equals_oracle(nttm):
         assert nb_ == ne_ == (int(nz[-1]) + 1 if len(nz) else 0)
 
 
+def test_batch_carry_chains(nttm):
+    """Products whose carries run through whole blocks of limbs, batched with random ones: (B^n - 1)^2 + x with x of
+    all limbs B - 1, so the carry chains of neighbouring products in one launch must not leak into each other."""
+    ops = []
+    s = 77
+    for i, n in enumerate((3000, 1, 2048, 4097, 1500, 2049)):
+        ones = np.full(n, B - 1, dtype=np.uint64)
+        if i % 2:
+            a, s = O.gen_limbs(n, s)
+            ops.append((a, ones, None))
+        else:
+            ops.append((ones, ones, np.full(2 * n, B - 1, dtype=np.uint64)))
+    say("batch carry chains")
+    got = run_batch(nttm, ops)
+    for (a, b, x), (c, ncn) in zip(ops, got):
+        want = to_int(a) * to_int(b) + (to_int(x) if x is not None else 0)
+        assert to_int(c) == want and (c < B).all()
+
+
 def test_batch_leaf_shapes(nttm):
     """4096 leaf products with log-uniform lengths in [64, 65536] limbs, unequal pairs (the batched-leaf workload),
     every product checked: GMP exactly where available, else the residue form."""

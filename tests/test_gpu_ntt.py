@@ -142,6 +142,28 @@ def test_crt_long_carry_ripple(nttm):
     assert not got[:n].any() and int(got[n]) == 1
 
 
+@pytest.mark.parametrize("hole", [False, True])
+def test_crt_carry_across_many_blocks(nttm, hole):
+    # 2^25 + 12345 coefficients of value B - 1 (more blocks than one pass of the block-carry scan covers) and x = 1;
+    # with a hole (value B - 2 at h) the value is B^n - B^h: limbs below h are 0, limbs h .. n-1 are B - 1
+    import torch
+    n, np_ = (1 << 25) + 12345, 3
+    h = n // 2 + 777 if hole else n
+    planes = []
+    for i in range(np_):
+        p = nttm.prime(i)
+        t = torch.full((n,), (B - 1) % p, dtype=torch.int64, device="cuda")
+        if hole:
+            t[h] = (B - 2) % p
+        planes.append(t)
+    x = torch.zeros(n, dtype=torch.int64, device="cuda"); x[0] = 1
+    say("crt many blocks", n, hole)
+    got = host(nttm.crt(planes, n, n + 1, x))
+    assert not got[:h].any()
+    assert (got[h:n] == B - 1).all()
+    assert int(got[n]) == (0 if hole else 1)
+
+
 def test_crt_overflow_is_reported(nttm):
     import ntt_amd
     planes = [np.full(10, nttm.prime(i) - 1, dtype=np.uint64) for i in range(3)]
