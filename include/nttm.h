@@ -19,6 +19,9 @@
  *   nttm_ntt_fwd / nttm_ntt_inv   ntt_fwd / ntt_inv                                E/ntt.h:45-46
  *   nttm_pw                       ntt_pw                                           E/ntt.h:47
  *   nttm_ntt_inv_pw               ntt_inv_pw_y (FULL / BCAST / PAIR)               E/ntt.h:52
+ *   nttm_ntt_fwd3 / _inv3         ntt_fwd3 / ntt_inv3 (3 2^k lengths)              E/ntt.h:53-54; E/ntt.c:671-779
+ *   nttm_ntt_inv3_pw              ntt_inv3_pw_y (FULL / BCAST / PAIR)              E/ntt.h:55-57
+ *   nttm_root3                    ec_root3                                         E/modarith.h:206-214
  *   nttm_load                     ntt_load                                         E/ntt.h:53
  *   nttm_crt                      k_crt_batch + spill merge for one product        E/rns_int.h:50-52; E/rns_mul.c:336
  *   nttm_max_terms                ec_np3_max_terms / ec_np_check                   E/modarith.h:40-43; E/crt.c:28-44
@@ -39,7 +42,7 @@ enum {
     NTTM_EHIP = -3,        /* a HIP runtime call failed */
     NTTM_EBOUND = -4,      /* the product exceeds what the prime set reconstructs, or the length exceeds 2^32 */
     NTTM_ESTATE = -5,      /* not initialised */
-    NTTM_ECHECK = -6       /* a device-side post-condition failed (non-zero final carry, CRT look-back stall) */
+    NTTM_ECHECK = -6       /* a device-side post-condition failed (non-zero carry beyond the top limb) */
 };
 
 #define NTTM_MAX_PRIMES 4
@@ -47,11 +50,17 @@ enum {
 /* y layouts of the fused inverse (E/ntt.h:31-35) */
 enum { NTTM_Y_FULL = 0, NTTM_Y_BCAST = 1, NTTM_Y_PAIR = 2 };
 
+/* transform lengths of the products (nttm_config.radix3): 2^k always, or 3 2^k where the product fits it */
+enum { NTTM_R3_AUTO = 0,       /* 3 2^k where it fits and moves fewer bytes than the 2^k transform */
+       NTTM_R3_NEVER = 1,      /* 2^k only */
+       NTTM_R3_FIT = 2 };      /* 3 2^k wherever it fits (the reference's rule, E/rns_mul.c:270-277) */
+
 typedef struct nttm_config {
     int nprimes;                /* 0: choose per product (3 while the bound allows, else 4); 3 or 4: fixed */
     size_t workspace_bytes;     /* 0: grow on demand; else the cap of the library's device workspace */
     int tile_log;               /* points per tile of the strided transform passes, log2: 12 or 13 (0: default) */
     int pts;                    /* transform points per thread: 8 or 16 (0: default) */
+    int radix3;                 /* NTTM_R3_*: when products use 3 2^k lengths (0: NTTM_R3_AUTO) */
 } nttm_config;
 
 /* one product of a batch: c = a b (+ x).  c has na + nb limbs, or na + nb + 1 when x is given (nx <= na + nb),
@@ -95,6 +104,17 @@ int nttm_ntt_inv(int prime, uint64_t *x, int logn, size_t batch, void *stream);
 int nttm_pw(int prime, uint64_t *x, const uint64_t *y, size_t count, void *stream);
 /* the pointwise product with y laid out per ymode, then the inverse: FULL y[t][k]; BCAST y[k]; PAIR y[t/2][k] */
 int nttm_ntt_inv_pw(int prime, uint64_t *x, const uint64_t *y, int ymode, int logn, size_t batch, void *stream);
+/* ---- transforms of 3 2^logk points (1 <= logk <= 32), batch transforms of 3 2^logk contiguous residues ----
+ * forward: natural order in; third r (points [r 2^logk, (r+1) 2^logk)), position brev_logk(t), holds X[3t + r] for
+ * the root nttm_root3(prime, logk) of order 3 2^logk.  Pointwise products need both operands in this order only. */
+int nttm_ntt_fwd3(int prime, uint64_t *x, int logk, size_t batch, void *stream);
+/* that order in -> natural out, times (3 2^logk)^-1 */
+int nttm_ntt_inv3(int prime, uint64_t *x, int logk, size_t batch, void *stream);
+/* the pointwise product with y (whole transforms of 3 2^logk points laid out per ymode), then the inverse */
+int nttm_ntt_inv3_pw(int prime, uint64_t *x, const uint64_t *y, int ymode, int logk, size_t batch, void *stream);
+/* the primitive 3 2^logk-th root: g^((p-1) / (3 2^logk)) for the smallest generator g */
+uint64_t nttm_root3(int i, int logk);
+
 /* dst[0 .. npoints) = src limbs mod p, zero-extended (nlimbs <= npoints) */
 int nttm_load(int prime, uint64_t *dst, const uint64_t *src, size_t nlimbs, size_t npoints, void *stream);
 /* limbs of sum_k CRT(planes[0..np)[k]) 10^(18 k) + x: planes hold ncoef canonical residues each (ncoef <= nc); c gets
@@ -123,6 +143,9 @@ int nttm_profile_read(nttm_kstat *out, int cap);
  * tile_log / last_log: log2 of points per tile of the strided passes / of the last (contiguous) pass (0: defaults;
  * the product path uses the configured tile_log and 12) */
 int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_log, int *out, int cap);
+/* the transform length the products take for ncoef = na + nb - 1 coefficients under a radix3 rule (NTTM_R3_*) and
+ * tile_log (0: default): 2^logn (r3 = 0) or 3 2^logn (r3 = 1) points */
+int nttm_debug_product_len(size_t ncoef, int radix3, int tile_log, int *logn, int *r3);
 /* the global element indices of tile `tile` of a pass of that geometry; returns the tile's element count */
 int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out);
 

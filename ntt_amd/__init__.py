@@ -35,8 +35,12 @@ class KStat(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
 
 
+R3_AUTO, R3_NEVER, R3_FIT = 0, 1, 2
+
+
 class Config(C.Structure):
-    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t), ("tile_log", C.c_int), ("pts", C.c_int)]
+    _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t), ("tile_log", C.c_int), ("pts", C.c_int),
+                ("radix3", C.c_int)]
 
 
 _lib = None
@@ -64,12 +68,18 @@ def lib():
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
         L.nttm_pw.argtypes = [i32, vp, vp, sz, vp]; L.nttm_pw.restype = i32
         L.nttm_ntt_inv_pw.argtypes = [i32, vp, vp, i32, i32, sz, vp]; L.nttm_ntt_inv_pw.restype = i32
+        L.nttm_ntt_fwd3.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd3.restype = i32
+        L.nttm_ntt_inv3.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv3.restype = i32
+        L.nttm_ntt_inv3_pw.argtypes = [i32, vp, vp, i32, i32, sz, vp]; L.nttm_ntt_inv3_pw.restype = i32
+        L.nttm_root3.argtypes = [i32, i32]; L.nttm_root3.restype = u64
         L.nttm_load.argtypes = [i32, vp, vp, sz, sz, vp]; L.nttm_load.restype = i32
         L.nttm_crt.argtypes = [vp, sz, C.POINTER(vp), i32, sz, vp, sz, vp]; L.nttm_crt.restype = i32
         L.nttm_last_phase_ms.argtypes = [C.POINTER(C.c_double)] * 3; L.nttm_last_phase_ms.restype = i32
         L.nttm_profile.argtypes = [i32]; L.nttm_profile.restype = i32
         L.nttm_profile_read.argtypes = [C.POINTER(KStat), i32]; L.nttm_profile_read.restype = i32
         L.nttm_debug_plan.argtypes = [i32, u64, i32, i32, C.POINTER(C.c_int), i32]; L.nttm_debug_plan.restype = i32
+        L.nttm_debug_product_len.argtypes = [sz, i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.nttm_debug_product_len.restype = i32
         L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
         L.nttm_debug_tile_indices.restype = i32
         _lib = L
@@ -101,6 +111,10 @@ def root(i, logn):
     return int(lib().nttm_root(i, logn))
 
 
+def root3(i, logk):
+    return int(lib().nttm_root3(i, logk))
+
+
 def max_terms(np_):
     return int(lib().nttm_max_terms(np_))
 
@@ -111,6 +125,13 @@ def debug_plan(logn, batch=1, last_log=0, tile_log=13):
     return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
 
 
+def debug_product_len(ncoef, radix3=R3_AUTO, tile_log=0):
+    """(logn, r3): the transform length a product of ncoef coefficients takes, 2^logn or 3 2^logn points."""
+    logn, r3 = C.c_int(), C.c_int()
+    _check(lib().nttm_debug_product_len(ncoef, radix3, tile_log, C.byref(logn), C.byref(r3)))
+    return logn.value, r3.value
+
+
 def debug_tile_indices(lo, m, logw, logh, tile):
     buf = (C.c_uint64 * (1 << (m + logw + logh)))()
     n = lib().nttm_debug_tile_indices(lo, m, logw, logh, tile, buf)
@@ -118,8 +139,8 @@ def debug_tile_indices(lo, m, logw, logh, tile):
 
 
 # ---- device entry points ------------------------------------------------------------------------------------------------
-def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0, pts=0):
-    cfg = Config(nprimes, workspace_bytes, tile_log, pts)
+def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0, pts=0, radix3=R3_AUTO):
+    cfg = Config(nprimes, workspace_bytes, tile_log, pts, radix3)
     _check(lib().nttm_init(device, C.byref(cfg)))
 
 
@@ -175,6 +196,19 @@ def ntt_inv(prime_idx, x, logn, batch=1, stream=None):
 
 def ntt_inv_pw(prime_idx, x, y, ymode, logn, batch=1, stream=None):
     _check(lib().nttm_ntt_inv_pw(prime_idx, _ptr(x), _ptr(y), ymode, logn, batch, _stream(stream)))
+
+
+def ntt_fwd3(prime_idx, x, logk, batch=1, stream=None):
+    """transforms of 3 2^logk points: third r, position brev(t) receives X[3t + r] (ntt_fwd3)."""
+    _check(lib().nttm_ntt_fwd3(prime_idx, _ptr(x), logk, batch, _stream(stream)))
+
+
+def ntt_inv3(prime_idx, x, logk, batch=1, stream=None):
+    _check(lib().nttm_ntt_inv3(prime_idx, _ptr(x), logk, batch, _stream(stream)))
+
+
+def ntt_inv3_pw(prime_idx, x, y, ymode, logk, batch=1, stream=None):
+    _check(lib().nttm_ntt_inv3_pw(prime_idx, _ptr(x), _ptr(y), ymode, logk, batch, _stream(stream)))
 
 
 def pw(prime_idx, x, y, stream=None):

@@ -33,6 +33,9 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
 uint64_t nttm_crt_blocks(uint64_t nc);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
+int nttm_run_r3(bool inv, double *x, uint64_t batch, int logk, const PrimeK &pk, const double2 *b3,
+                const uint64_t *limbs, uint64_t nlimbs, const LoadDesc *ldesc, int which, const double2 *scale,
+                hipStream_t s);
 
 /* ---- errors ----------------------------------------------------------------------------------------------------- */
 static char g_err[512] = "";
@@ -83,6 +86,7 @@ struct Ctx {
     nttm_config cfg;
     PrimeK pk[NTTM_MAX_PRIMES];
     uint64_t root32[NTTM_MAX_PRIMES];        /* root of order 2^32 */
+    uint64_t root3x32[NTTM_MAX_PRIMES];      /* root of order 3 2^32 */
     TwTables twf[NTTM_MAX_PRIMES], twi[NTTM_MAX_PRIMES];
     double2 *tables;
     CrtConst cc;
@@ -120,6 +124,12 @@ extern "C" uint64_t nttm_root(int i, int logn)
     if (i < 0 || i >= NTTM_MAX_PRIMES || logn < 0 || logn > NTTM_LOGN_MAX) return 0;
     const uint64_t p = NTTM_PRIMES[i];
     return h_powmod(h_generator(p), (p - 1) >> logn, p);
+}
+extern "C" uint64_t nttm_root3(int i, int logk)
+{
+    if (i < 0 || i >= NTTM_MAX_PRIMES || logk < 0 || logk > NTTM_LOGN_MAX) return 0;
+    const uint64_t p = NTTM_PRIMES[i];
+    return h_powmod(h_generator(p), ((p - 1) >> logk) / 3, p);
 }
 
 static void build_crt_const(CrtConst &cc)
@@ -160,16 +170,27 @@ static void build_crt_const(CrtConst &cc)
 static int build_tables(void)
 {
     const size_t nst = (size_t)1 << NTTM_STAGE_LOG, nbt = (size_t)1 << NTTM_BT_LOG;
-    const size_t nbq = (size_t)4 << NTTM_BQ_LOG;
-    const size_t per_dir = nst + 2 * nbt + nbq, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
+    const size_t nbq = (size_t)4 << NTTM_BQ_LOG, nb3 = nbq + 1;
+    const size_t per_dir = nst + 2 * nbt + nbq + nb3, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
     std::vector<double2> h(total);
     for (int i = 0; i < NTTM_MAX_PRIMES; i++) {
         const uint64_t p = NTTM_PRIMES[i];
         const uint64_t gen = h_generator(p);
         g.root32[i] = h_powmod(gen, (p - 1) >> 32, p);
+        g.root3x32[i] = h_powmod(gen, ((p - 1) >> 32) / 3, p);
         for (int dir = 0; dir < 2; dir++) {
             const uint64_t w32 = dir ? h_inv(g.root32[i], p) : g.root32[i];
             double2 *st = &h[((size_t)i * 2 + dir) * per_dir], *bh = st + nst, *bl = bh + nbt, *bq = bl + nbt;
+            double2 *b3 = bq + nbq;
+            const uint64_t W3 = dir ? h_inv(g.root3x32[i], p) : g.root3x32[i];
+            for (int k = 0; k < 4; k++) {                                /* b3[k 2^8 + x] = W3^(x 2^(8k)) */
+                const uint64_t wk = h_powmod(W3, (uint64_t)1 << (NTTM_BQ_LOG * k), p);
+                uint64_t v = 1;
+                for (size_t x = 0; x < ((size_t)1 << NTTM_BQ_LOG); x++) {
+                    b3[((size_t)k << NTTM_BQ_LOG) + x] = h_shoup(v, p); v = h_mulmod(v, wk, p);
+                }
+            }
+            b3[nbq] = h_shoup(h_powmod(g.root3x32[i], (uint64_t)1 << 32, p), p);   /* the forward cube root */
             st[0] = h_shoup(1, p);
             for (int s = 0; s + 1 < NTTM_STAGE_LOG + 1 && ((size_t)1 << s) < nst; s++) {
                 const uint64_t ws = h_powmod(w32, (uint64_t)1 << (32 - (s + 1)), p);   /* order 2^(s+1) */
@@ -197,6 +218,7 @@ static int build_tables(void)
         for (int dir = 0; dir < 2; dir++) {
             const double2 *st = g.tables + ((size_t)i * 2 + dir) * per_dir;
             TwTables t; t.stage = st; t.bh = st + nst; t.bl = st + nst + nbt; t.bq = st + nst + 2 * nbt;
+            t.b3 = t.bq + nbq;
             if (dir) g.twi[i] = t; else g.twf[i] = t;
         }
     return 0;
@@ -231,6 +253,8 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     if (!g.cfg.tile_log) g.cfg.tile_log = NTTM_TILE_LOG_DEFAULT;
     if (g.cfg.tile_log != 12 && g.cfg.tile_log != 13) return nttm_fail(NTTM_EINVAL, "nttm_init: tile_log must be 12 or 13");
     if (g.cfg.pts && g.cfg.pts != 8 && g.cfg.pts != 16) return nttm_fail(NTTM_EINVAL, "nttm_init: pts must be 0, 8 or 16");
+    if (g.cfg.radix3 < NTTM_R3_AUTO || g.cfg.radix3 > NTTM_R3_FIT)
+        return nttm_fail(NTTM_EINVAL, "nttm_init: radix3 must be 0, 1 or 2");
     nttm_set_pts(g.cfg.pts ? g.cfg.pts : 16);
     {   /* NTTM_GENERIC_PASSES=1: runtime-geometry pass kernels only (for comparison) */
         const char *e = getenv("NTTM_GENERIC_PASSES");
@@ -286,14 +310,16 @@ static int fwd_planes(int prime, double *x, int logn, uint64_t batch, hipStream_
     for (int i = 0; i < pl.npass; i++) RCK(nttm_run_pass(false, a, pl.p[i], s));
     return 0;
 }
-/* inverse with the pointwise product by y (ymode layout) fused into the first pass; scale by n^-1 in the last pass */
-static int inv_planes(int prime, double *x, const double *y, int ymode, int logn, uint64_t batch, int scale, hipStream_t s)
+/* inverse with the pointwise product by y (ymode layout) fused into the first pass; scale by n^-1 in the last pass.
+ * r3y: the batch is the thirds of transforms of 3 2^logn points, y holds whole such transforms */
+static int inv_planes(int prime, double *x, const double *y, int ymode, int logn, uint64_t batch, int scale, hipStream_t s,
+                      int r3y = 0)
 {
     Plan pl; plan_make(&pl, logn, batch, g.cfg.tile_log);
     PassArgs a = base_args(prime, x, logn, batch, true);
     for (int i = pl.npass - 1; i >= 0; i--) {
         PassArgs b = a;
-        if (i == pl.npass - 1) { b.y = y; b.ymode = ymode; }
+        if (i == pl.npass - 1) { b.y = y; b.ymode = ymode; b.r3y = r3y; }
         if (i == 0 && scale) {
             const uint64_t p = NTTM_PRIMES[prime];
             b.scale = h_shoup(h_inv(((uint64_t)1 << logn) % p, p), p); b.do_scale = 1;
@@ -333,25 +359,32 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
  * Operand limbs come from (a, na, b, nb) when cnt == 1 and ldesc is null, else from the device descriptors ldesc.
  * Result: plane i of A holds the cyclic convolution mod p_i. */
 /* the strided-pass tile for a 2^logn product transform: the configured one, or 2^13 where that saves a pass */
-static int product_tile_log(int logn)
+static int product_tile_log(int logn, int tl = 0)
 {
+    if (!tl) tl = g.cfg.tile_log;
     const int R = logn > NTTM_FUSED_LOG ? logn - NTTM_FUSED_LOG : 0;
-    const int tl = g.cfg.tile_log, k_cfg = (R + tl - 5) / (tl - 4), k_13 = (R + 8) / 9;
+    const int k_cfg = (R + tl - 5) / (tl - 4), k_13 = (R + 8) / 9;
     return k_13 < k_cfg ? 13 : tl;
 }
 
-static int product_transforms(int np, int logn, uint64_t cnt, double *A, double *Bp, const uint64_t *a, uint64_t na,
-                              const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s, hipEvent_t mid)
+/* the transforms of cnt products of one length: 2^logn points (r3 = 0), or 3 2^logn points (r3 = 1: a radix-3 layer,
+ * reading the limbs, and the 2^logn engine on the 3 cnt thirds).  Plane i of A / B holds the cnt transforms of prime
+ * i back to back.  After it, A holds the cyclic convolutions mod each prime, natural order. */
+static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A, double *Bp, const uint64_t *a,
+                              uint64_t na, const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s,
+                              hipEvent_t mid)
 {
-    Plan pl; plan_make2(&pl, logn, cnt, product_tile_log(logn), NTTM_FUSED_LOG);
-    const uint64_t n = (uint64_t)1 << logn;
+    const uint64_t tb = r3 ? 3 * cnt : cnt;                      /* engine transforms of 2^logn points */
+    Plan pl; plan_make2(&pl, logn, tb, product_tile_log(logn), NTTM_FUSED_LOG);
+    const uint64_t n = (uint64_t)1 << logn, plane = tb * n, ntot = r3 ? 3 * n : n;
     const int last = pl.npass - 1;
     for (int i = 0; i < np; i++) {                               /* B, every prime, all passes but the last */
-        double *Bi = Bp + (size_t)i * cnt * n;
+        double *Bi = Bp + (size_t)i * plane;
         const uint64_t p = NTTM_PRIMES[i];
-        PassArgs fb = base_args(i, Bi, logn, cnt, false);
-        fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
-        fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
+        const double2 ninv = h_shoup(h_inv(ntot % p, p), p);     /* the inverse's 1/length, folded into B */
+        PassArgs fb = base_args(i, Bi, logn, tb, false);
+        if (r3) RCK(nttm_run_r3(false, Bi, cnt, logn, g.pk[i], g.twf[i].b3, ldesc ? nullptr : b, nb, ldesc, 1, &ninv, s));
+        else { fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1; fb.in_scale = ninv; fb.do_in_scale = 1; }
         for (int j = 0; j < last; j++) {
             RCK(nttm_run_pass(false, fb, pl.p[j], s));
             fb.limbs = nullptr; fb.ldesc = nullptr; fb.do_in_scale = 0;
@@ -359,25 +392,52 @@ static int product_transforms(int np, int logn, uint64_t cnt, double *A, double 
     }
     if (mid) HIPCK(hipEventRecord(mid, s));
     for (int i = 0; i < np; i++) {                               /* A, every prime */
-        double *Ai = A + (size_t)i * cnt * n, *Bi = Bp + (size_t)i * cnt * n;
+        double *Ai = A + (size_t)i * plane, *Bi = Bp + (size_t)i * plane;
         const uint64_t p = NTTM_PRIMES[i];
-        PassArgs fa = base_args(i, Ai, logn, cnt, false);
-        fa.limbs = ldesc ? nullptr : a; fa.nlimbs = na; fa.ldesc = ldesc; fa.which = 0;
+        PassArgs fa = base_args(i, Ai, logn, tb, false);
+        if (r3) RCK(nttm_run_r3(false, Ai, cnt, logn, g.pk[i], g.twf[i].b3, ldesc ? nullptr : a, na, ldesc, 0, nullptr, s));
+        else { fa.limbs = ldesc ? nullptr : a; fa.nlimbs = na; fa.ldesc = ldesc; fa.which = 0; }
         for (int j = 0; j < last; j++) {
             RCK(nttm_run_pass(false, fa, pl.p[j], s));
             fa.limbs = nullptr; fa.ldesc = nullptr;
         }
         /* B's last forward pass, A's last forward pass, the pointwise product and A's first inverse pass */
-        PassArgs fb = base_args(i, Bi, logn, cnt, false);
-        if (last == 0) {                                         /* a single-pass transform reads B's limbs here */
+        PassArgs fb = base_args(i, Bi, logn, tb, false);
+        if (last == 0 && !r3) {                                  /* a single-pass transform reads B's limbs here */
             fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
             fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
         }
         RCK(nttm_run_fused2(fa, fb, pl.p[last], s));
-        PassArgs ia = base_args(i, Ai, logn, cnt, true);
+        PassArgs ia = base_args(i, Ai, logn, tb, true);
         for (int j = last - 1; j >= 0; j--) RCK(nttm_run_pass(true, ia, pl.p[j], s));
+        if (r3) RCK(nttm_run_r3(true, Ai, cnt, logn, g.pk[i], g.twi[i].b3, nullptr, 0, nullptr, 0, nullptr, s));
     }
     return 0;
+}
+
+/* HBM bytes per point of a 2^logn transform of one prime in the product schedule (both operands' forward
+ * transforms, the double-fused pass, the inverse; limb reads and the CRT excluded: equal for both lengths) */
+static double sched_bytes(int logn, int r3, int tile_log)
+{
+    Plan pl; plan_make2(&pl, logn, 1, product_tile_log(logn, tile_log), NTTM_FUSED_LOG);
+    const int P = pl.npass;
+    if (r3) return 3.0 * (2 * (8 + 16 * (P - 1)) + 24 + 16 * (P - 1) + 16);
+    return P >= 2 ? 2 * (8 + 16 * (P - 2)) + 24 + 16 * (P - 1) : 8;
+}
+/* the transform length of a product with ncoef coefficients: 2^logn, or 3 2^(logn - 2) when it fits and the rule
+ * (NTTM_R3_*) takes it (the reference takes it whenever it fits, E/rns_mul.c:270-277) */
+static void choose_len(uint64_t ncoef, int mode, int tile_log, int *logn_out, int *r3_out)
+{
+    int logn = ceil_log2(ncoef); if (logn < NTTM_PRODUCT_LOGN_MIN) logn = NTTM_PRODUCT_LOGN_MIN;
+    int r3 = 0;
+    if (mode != NTTM_R3_NEVER && logn - 2 >= NTTM_PRODUCT_LOGN_MIN && ncoef <= ((uint64_t)3 << (logn - 2)))
+        r3 = mode == NTTM_R3_FIT ||
+             sched_bytes(logn - 2, 1, tile_log) * ldexp(1.0, logn - 2) < sched_bytes(logn, 0, tile_log) * ldexp(1.0, logn);
+    *logn_out = r3 ? logn - 2 : logn; *r3_out = r3;
+}
+static void choose_len(uint64_t ncoef, int *logn_out, int *r3_out)
+{
+    choose_len(ncoef, g.cfg.radix3, g.cfg.tile_log, logn_out, r3_out);
 }
 
 /* c = a b (+ x), device pointers; synchronous (returns after the stream has drained) */
@@ -397,9 +457,10 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     }
     int np; RCK(choose_np(na < nb ? na : nb, &np));
     const uint64_t ncoef = na + nb - 1;
-    int logn = ceil_log2(ncoef); if (logn < NTTM_PRODUCT_LOGN_MIN) logn = NTTM_PRODUCT_LOGN_MIN;
+    int logn, r3;
+    choose_len(ncoef, &logn, &r3);
     if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", logn, NTTM_LOGN_MAX);
-    const uint64_t n = (uint64_t)1 << logn;
+    const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn;             /* points per plane */
     const size_t planes_bytes = (size_t)2 * np * n * 8;
     const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
     const size_t off_desc = align_up(planes_bytes, 256), off_crt = off_desc + 256;
@@ -409,7 +470,7 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     uint32_t *scratch = (uint32_t *)(g.ws + off_crt);
 
     HIPCK(hipEventRecord(g.ev[0], s));
-    RCK(product_transforms(np, logn, 1, A, Bp, a, na, b, nb, nullptr, s, g.ev[1]));
+    RCK(product_transforms(np, logn, r3, 1, A, Bp, a, na, b, nb, nullptr, s, g.ev[1]));
     HIPCK(hipEventRecord(g.ev[2], s));
     CrtDesc hd; hd.pl = A; hd.plane = n; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
     memcpy(g.hpin, &hd, sizeof hd);
@@ -474,16 +535,18 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
     if (!N) return 0;
     /* validate; zero-operand products are written directly */
     size_t mn_max = 0;
-    std::vector<std::vector<size_t>> groups(NTTM_LOGN_MAX + 1);
+    /* groups by transform length: 2^logn at [logn], 3 2^logn at [NTTM_LOGN_MAX + 1 + logn] */
+    std::vector<std::vector<size_t>> groups(2 * (NTTM_LOGN_MAX + 1));
     for (size_t i = 0; i < N; i++) {
         const nttm_prod &q = P[i];
         if (q.x && q.nx > q.na + q.nb) return nttm_fail(NTTM_EINVAL, "product %zu: nx exceeds na + nb", i);
         if (!q.c || (q.na && !q.a) || (q.nb && !q.b)) return nttm_fail(NTTM_EINVAL, "product %zu: null pointer", i);
         if (!q.na || !q.nb) continue;
         size_t mn = q.na < q.nb ? q.na : q.nb; if (mn > mn_max) mn_max = mn;
-        int logn = ceil_log2(q.na + q.nb - 1); if (logn < NTTM_PRODUCT_LOGN_MIN) logn = NTTM_PRODUCT_LOGN_MIN;
+        int logn, r3;
+        choose_len(q.na + q.nb - 1, &logn, &r3);
         if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "product %zu: transform above 2^%d", i, NTTM_LOGN_MAX);
-        groups[logn].push_back(i);
+        groups[r3 * (NTTM_LOGN_MAX + 1) + logn].push_back(i);
     }
     int np; RCK(choose_np(mn_max ? mn_max : 1, &np));
     for (size_t i = 0; i < N; i++) {
@@ -494,10 +557,11 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         if (q.x && q.nx) HIPCK(hipMemcpyAsync(q.c, q.x, q.nx * 8, hipMemcpyDeviceToDevice, s));
     }
     /* one group per transform length: batched load, transforms, CRT */
-    for (int logn = 1; logn <= NTTM_LOGN_MAX; logn++) {
-        const std::vector<size_t> &G = groups[logn];
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const std::vector<size_t> &G = groups[gi];
         if (G.empty()) continue;
-        const uint64_t n = (uint64_t)1 << logn, cnt = G.size();
+        const int r3 = gi > NTTM_LOGN_MAX, logn = (int)(gi % (NTTM_LOGN_MAX + 1));
+        const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn, cnt = G.size();      /* points per product plane */
         uint64_t nblk = 0;
         std::vector<CrtDesc> cd(cnt);
         std::vector<LoadDesc> ld(cnt);
@@ -523,7 +587,8 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         memcpy(hp + cnt * sizeof(CrtDesc), ld.data(), cnt * sizeof(LoadDesc));
         HIPCK(hipMemcpyAsync(g.ws + off_cd, hp, cnt * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + cnt * sizeof(CrtDesc), cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
-        RCK(product_transforms(np, logn, cnt, A, Bp, nullptr, 0, nullptr, 0, (const LoadDesc *)(g.ws + off_ld), s, nullptr));
+        RCK(product_transforms(np, logn, r3, cnt, A, Bp, nullptr, 0, nullptr, 0, (const LoadDesc *)(g.ws + off_ld), s,
+                               nullptr));
         double crt_bytes = 0;
         for (uint64_t j = 0; j < cnt; j++) crt_bytes += 8.0 * np * cd[j].ncoef + 8.0 * cd[j].nc + 8.0 * cd[j].nx;
         RCK(nttm_crt_launch_batch(g.ws + off_cd, cnt, nblk, np, g.cc, (uint32_t *)(g.ws + off_crt), s, crt_bytes));
@@ -532,7 +597,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         HIPCK(hipMemcpyAsync(hp + cnt * 8, g.ws + off_crt, 8, hipMemcpyDeviceToHost, s));
         HIPCK(hipStreamSynchronize(s));
         const uint32_t flags = ((const uint32_t *)(hp + cnt * 8))[1];
-            if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
+        if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
         for (uint64_t j = 0; j < cnt; j++) P[G[j]].ncn = ((const uint64_t *)hp)[j];
     }
     /* zero-operand products: normalized length of x */
@@ -587,6 +652,49 @@ extern "C" int nttm_ntt_inv_pw(int prime, uint64_t *x, const uint64_t *y, int ym
     RCK(nttm_run_cvt_u64_f64((double *)g.ws, y, ycnt, s));
     RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
     RCK(inv_planes(prime, (double *)x, (const double *)g.ws, ymode, logn, batch, 1, s));
+    return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+
+/* transforms of 3 2^logk points: the radix-3 layer (nttm_run_r3) and the 2^logk engine on the thirds (batch 3) */
+extern "C" int nttm_ntt_fwd3(int prime, uint64_t *x, int logk, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logk, batch));
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t cnt = 3 * ((uint64_t)batch << logk);
+    RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
+    RCK(nttm_run_r3(false, (double *)x, batch, logk, g.pk[prime], g.twf[prime].b3, nullptr, 0, nullptr, 0, nullptr, s));
+    RCK(fwd_planes(prime, (double *)x, logk, 3 * (uint64_t)batch, s));
+    return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+/* the inverse: the engine's inverses scale by 2^-logk, the radix-3 layer by 3^-1 */
+static int inv3_planes(int prime, double *x, const double *y, int ymode, int logk, uint64_t batch, hipStream_t s)
+{
+    const uint64_t p = NTTM_PRIMES[prime];
+    RCK(inv_planes(prime, x, y, ymode, logk, 3 * batch, 1, s, 1));
+    const double2 third = h_shoup(h_inv(3, p), p);
+    return nttm_run_r3(true, x, batch, logk, g.pk[prime], g.twi[prime].b3, nullptr, 0, nullptr, 0, &third, s);
+}
+extern "C" int nttm_ntt_inv3(int prime, uint64_t *x, int logk, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logk, batch));
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t cnt = 3 * ((uint64_t)batch << logk);
+    RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
+    RCK(inv3_planes(prime, (double *)x, nullptr, NTTM_Y_FULL, logk, batch, s));
+    return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+extern "C" int nttm_ntt_inv3_pw(int prime, uint64_t *x, const uint64_t *y, int ymode, int logk, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logk, batch));
+    if (!y) return nttm_fail(NTTM_EINVAL, "null y");
+    if (ymode < NTTM_Y_FULL || ymode > NTTM_Y_PAIR) return nttm_fail(NTTM_EINVAL, "ymode %d", ymode);
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t n3 = (uint64_t)3 << logk, cnt = n3 * batch;
+    const uint64_t ycnt = ymode == NTTM_Y_FULL ? cnt : ymode == NTTM_Y_BCAST ? n3 : (uint64_t)((batch + 1) / 2) * n3;
+    RCK(ws_need(ycnt * 8));
+    RCK(nttm_run_cvt_u64_f64((double *)g.ws, y, ycnt, s));
+    RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
+    RCK(inv3_planes(prime, (double *)x, (const double *)g.ws, ymode, logk, batch, s));
     return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
 }
 extern "C" int nttm_pw(int prime, uint64_t *x, const uint64_t *y, size_t count, void *stream)
@@ -710,6 +818,15 @@ extern "C" int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_
     int k = 0;
     for (int i = 0; i < pl.npass && k + 4 <= cap; i++) { out[k++] = pl.p[i].lo; out[k++] = pl.p[i].m; out[k++] = pl.p[i].logw; out[k++] = pl.p[i].logh; }
     return pl.npass;
+}
+extern "C" int nttm_debug_product_len(size_t ncoef, int radix3, int tile_log, int *logn, int *r3)
+{
+    if (!ncoef || !logn || !r3) return nttm_fail(NTTM_EINVAL, "nttm_debug_product_len arguments");
+    if (radix3 < NTTM_R3_AUTO || radix3 > NTTM_R3_FIT) return nttm_fail(NTTM_EINVAL, "radix3 mode %d", radix3);
+    if (tile_log <= 0) tile_log = NTTM_TILE_LOG_DEFAULT;
+    if (tile_log != 12 && tile_log != 13) return nttm_fail(NTTM_EINVAL, "tile_log %d", tile_log);
+    choose_len(ncoef, radix3, tile_log, logn, r3);
+    return 0;
 }
 extern "C" int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out)
 {

@@ -22,6 +22,8 @@ struct TwTables {                   /* device pointers, one set per prime and di
     const double2 *stage;           /* (w, wq) balanced, 2^NTTM_STAGE_LOG entries */
     const double2 *bh, *bl;         /* boundary tables, 2^NTTM_BT_LOG entries each */
     const double2 *bq;              /* four-level boundary table: bq[k 2^8 + x] = w32^(x 2^(8k)), k < 4 (16 KiB) */
+    const double2 *b3;              /* the same for the root W3 of order 3 2^32: b3[k 2^8 + x] = W3^(x 2^(8k)); entry
+                                     * 4 2^8 is the FORWARD cube root of unity w3 = W3^(2^32) in both directions */
 };
 
 /* operand limbs of the products of a batch (device array, one per transform) */
@@ -37,6 +39,7 @@ struct PassArgs {
     const double *y;                /* pointwise operand (inverse first pass), else null */
     int logn, lo, m, logw, logh;
     int ymode;
+    int r3y;                        /* y of a radix-3 length: the transforms of the batch are thirds (3 per y item) */
     uint64_t batch;
     PrimeK pk;
     TwTables tw;                    /* forward or inverse tables of the pass's direction */

@@ -229,6 +229,11 @@ __device__ __forceinline__ double input_limb(const PassArgs &a, uint64_t gi, dou
 __device__ __forceinline__ uint64_t y_index(const PassArgs &a, uint64_t gi)
 {
     const uint64_t nmask = ((uint64_t)1 << a.logn) - 1;
+    if (a.r3y) {                  /* transform t = 3 item + third: y holds whole items of 3 thirds */
+        const uint64_t t = gi >> a.logn, item = t / 3, third = t - 3 * item;
+        const uint64_t yt = a.ymode == NTTM_Y_FULL ? t : a.ymode == NTTM_Y_BCAST ? third : (item >> 1) * 3 + third;
+        return (yt << a.logn) | (gi & nmask);
+    }
     return a.ymode == NTTM_Y_FULL ? gi : a.ymode == NTTM_Y_BCAST ? (gi & nmask)
                                        : ((((gi >> a.logn) >> 1) << a.logn) | (gi & nmask));
 }
@@ -762,6 +767,88 @@ __global__ __launch_bounds__(NTTM_THREADS) void k_pw_u64(uint64_t *x, const uint
         x[i] = res_canon(mul_barrett((double)x[i], (double)y[i], pk.p, pk.pinv), pk.p, pk.pinv);
 }
 
+/* ---- transforms of 3 2^k points: the radix-3 layer -----------------------------------------------------------------
+ * A transform of n = 3m points (m = 2^logk) with the root w of order n (E/ntt.c:671-779, our own kernels):
+ * forward, one DIF radix-3 layer over the thirds x[j], x[j+m], x[j+2m]:
+ *     y_r[j] = (x[j] + w3^r x[j+m] + w3^(2r) x[j+2m]) w^(r j),     w3 = w^m (a cube root of unity),
+ * stored in third r, then the m-point transforms of the three thirds (the 2^k engine, batch 3, root w^3): third r,
+ * position brev_k(t), holds X[3t + r].  With 1 + w3 + w3^2 = 0: y_1 = (x0 - x2) + w3 (x1 - x2), y_2 = (x0 - x1) -
+ * w3 (x1 - x2), one product by w3 for the three outputs.  The inverse runs the m-point inverses on the thirds and
+ * then the mirrored layer: y'_r = y_r w^(-r j), x_0 = y'_0 + y'_1 + y'_2, x_1 = (y'_0 - y'_1) + w3 (y'_2 - y'_1),
+ * x_2 = (y'_0 - y'_2) - w3 (y'_2 - y'_1) (the forward w3 in both directions).  w^j comes from the four-level table of
+ * W3 (order 3 2^32): w^j = W3^E with E = j 2^(32 - logk); w^(2j) = (w^j)^2.  A thread handles one j of one transform. */
+struct R3Args {
+    double *x;                      /* batch transforms of 3m points, transform t at x + t 3m */
+    uint64_t batch;
+    int logk;
+    PrimeK pk;
+    const double2 *b3;              /* the direction's table (TwTables::b3) */
+    const uint64_t *limbs; uint64_t nlimbs;     /* forward from limbs: a single operand (batch 1) ... */
+    const LoadDesc *ldesc; int which;           /* ... or operand a / b of ldesc[t] */
+    double2 scale; int do_scale;                /* multiply every output by scale */
+};
+__device__ __forceinline__ double r3_input(const R3Args &a, uint64_t t, uint64_t k)
+{
+    const uint64_t *src; uint64_t nl;
+    if (a.ldesc) { const LoadDesc &d = a.ldesc[t]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; }
+    else { src = a.limbs; nl = a.nlimbs; }
+    return k < nl ? limb_to_res(src[k], a.pk.p, a.pk.pinv) : 0.0;
+}
+/* w^j and w^(2j) (times the scale when given), |.| <= p */
+__device__ __forceinline__ void r3_twiddles(const R3Args &a, uint64_t j, double *t1, double *t2)
+{
+    const double p = a.pk.p, pinv = a.pk.pinv;
+    const uint32_t E = (uint32_t)(j << (32 - a.logk));           /* logk >= 1 here, j < 2^logk */
+    double w1 = bq_twiddle(a.b3, E, p, pinv);
+    double w2 = mul_barrett(w1, w1, p, pinv);
+    if (a.do_scale) { w1 = mul_shoup(w1, a.scale.x, a.scale.y, p); w2 = mul_shoup(w2, a.scale.x, a.scale.y, p); }
+    *t1 = w1; *t2 = w2;
+}
+template <bool LIMB>
+__global__ __launch_bounds__(NTTM_THREADS) void k_r3_fwd(R3Args a)
+{
+    const uint64_t m = (uint64_t)1 << a.logk, total = a.batch << a.logk;
+    const uint64_t stride = (uint64_t)gridDim.x * NTTM_THREADS;
+    const double p = a.pk.p, pinv = a.pk.pinv;
+    const double2 w3 = a.b3[4 << NTTM_BQ_LOG];
+    for (uint64_t i = (uint64_t)blockIdx.x * NTTM_THREADS + threadIdx.x; i < total; i += stride) {
+        const uint64_t t = i >> a.logk, j = i & (m - 1);
+        double *v = a.x + t * 3 * m;
+        double x0, x1, x2;
+        if (LIMB) { x0 = r3_input(a, t, j); x1 = r3_input(a, t, j + m); x2 = r3_input(a, t, j + 2 * m); }
+        else { x0 = v[j]; x1 = v[j + m]; x2 = v[j + 2 * m]; }       /* |x| <= p */
+        const double u = mul_shoup(x1 - x2, w3.x, w3.y, p);         /* |u| <= 5p/8 */
+        double y0 = reduce(x0 + x1 + x2, p, pinv);
+        const double y1 = reduce((x0 - x2) + u, p, pinv), y2 = reduce((x0 - x1) - u, p, pinv);
+        double t1, t2;
+        r3_twiddles(a, j, &t1, &t2);
+        if (a.do_scale) y0 = mul_shoup(y0, a.scale.x, a.scale.y, p);
+        v[j] = y0;
+        v[j + m] = mul_barrett(y1, t1, p, pinv);
+        v[j + 2 * m] = mul_barrett(y2, t2, p, pinv);
+    }
+}
+__global__ __launch_bounds__(NTTM_THREADS) void k_r3_inv(R3Args a)
+{
+    const uint64_t m = (uint64_t)1 << a.logk, total = a.batch << a.logk;
+    const uint64_t stride = (uint64_t)gridDim.x * NTTM_THREADS;
+    const double p = a.pk.p, pinv = a.pk.pinv;
+    const double2 w3 = a.b3[4 << NTTM_BQ_LOG];
+    for (uint64_t i = (uint64_t)blockIdx.x * NTTM_THREADS + threadIdx.x; i < total; i += stride) {
+        const uint64_t t = i >> a.logk, j = i & (m - 1);
+        double *v = a.x + t * 3 * m;
+        double y0 = v[j];
+        double t1, t2;
+        r3_twiddles(a, j, &t1, &t2);                                 /* w^-j, w^-2j (scaled) */
+        if (a.do_scale) y0 = mul_shoup(y0, a.scale.x, a.scale.y, p);
+        const double y1 = mul_barrett(v[j + m], t1, p, pinv), y2 = mul_barrett(v[j + 2 * m], t2, p, pinv);
+        const double u = mul_shoup(y2 - y1, w3.x, w3.y, p);         /* |u| <= 5p/8 */
+        v[j] = reduce(y0 + y1 + y2, p, pinv);
+        v[j + m] = reduce((y0 - y1) + u, p, pinv);
+        v[j + 2 * m] = reduce((y0 - y2) - u, p, pinv);
+    }
+}
+
 /* ---- host launchers ------------------------------------------------------------------------------------------------ */
 static unsigned grid_for(uint64_t count)
 {
@@ -936,6 +1023,31 @@ int nttm_run_load_batch(double *dst, uint64_t plane, const void *desc, int which
     hipLaunchKernelGGL(k_load, dim3(grid_for(a.npoints)), dim3(NTTM_THREADS), 0, s, a);
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_load launch");
+}
+/* the radix-3 layer of batch transforms of 3 2^logk points (logk >= 1): forward (from the plane or from limbs) or
+ * inverse, in place */
+int nttm_run_r3(bool inv, double *x, uint64_t batch, int logk, const PrimeK &pk, const double2 *b3,
+                const uint64_t *limbs, uint64_t nlimbs, const LoadDesc *ldesc, int which, const double2 *scale,
+                hipStream_t s)
+{
+    if (logk < 1 || logk > NTTM_LOGN_MAX || !batch) return nttm_fail(NTTM_EINVAL, "internal: radix-3 layer of 3 2^%d", logk);
+    if (inv && (limbs || ldesc)) return nttm_fail(NTTM_EINVAL, "internal: inverse radix-3 layer from limbs");
+    if (limbs && batch != 1) return nttm_fail(NTTM_EINVAL, "internal: one limb operand for %llu transforms",
+                                              (unsigned long long)batch);
+    R3Args a; memset(&a, 0, sizeof a);
+    a.x = x; a.batch = batch; a.logk = logk; a.pk = pk; a.b3 = b3;
+    a.limbs = limbs; a.nlimbs = nlimbs; a.ldesc = ldesc; a.which = which;
+    a.do_scale = scale != nullptr; a.scale = scale ? *scale : make_double2(1.0, 0.0);
+    const bool limb = limbs || ldesc;
+    const uint64_t pts = 3 * (batch << logk);
+    prof_begin(inv ? "k_r3_inv" : limb ? "k_r3_fwd<limb>" : "k_r3_fwd",
+               limb ? 8.0 * (double)pts + 8.0 * (double)(ldesc ? 0 : nlimbs) : 16.0 * (double)pts, s);
+    const unsigned grid = grid_for(batch << logk);
+    if (inv) hipLaunchKernelGGL(k_r3_inv, dim3(grid), dim3(NTTM_THREADS), 0, s, a);
+    else if (limb) hipLaunchKernelGGL(k_r3_fwd<true>, dim3(grid), dim3(NTTM_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(k_r3_fwd<false>, dim3(grid), dim3(NTTM_THREADS), 0, s, a);
+    prof_end(s);
+    return nttm_hip_check(hipGetLastError(), "k_r3 launch");
 }
 int nttm_run_u64_to_f64(double *x, uint64_t count, hipStream_t s)
 {

@@ -55,6 +55,10 @@ def lib():
         for f in ("orc_ntt_fwd", "orc_ntt_inv"):
             getattr(L, f).argtypes = [_u64p, C.c_int, C.c_uint64, C.c_uint64]; getattr(L, f).restype = None
         L.orc_dft_bitrev.argtypes = [_u64p, _u64p, C.c_int, C.c_uint64, C.c_uint64]; L.orc_dft_bitrev.restype = None
+        L.orc_root3k.argtypes = [C.c_uint64, C.c_int]; L.orc_root3k.restype = C.c_uint64
+        L.orc_dft3.argtypes = [_u64p, _u64p, C.c_int, C.c_uint64, C.c_uint64]; L.orc_dft3.restype = None
+        for f in ("orc_ntt3_fwd", "orc_ntt3_inv"):
+            getattr(L, f).argtypes = [_u64p, C.c_int, C.c_uint64, C.c_uint64]; getattr(L, f).restype = None
         L.orc_garner.argtypes = [_u64p, _u64p, C.c_int, _u64p]; L.orc_garner.restype = None
         L.orc_div1e18.argtypes = [C.c_uint64, C.c_uint64, _u64p, _u64p]; L.orc_div1e18.restype = None
         L.orc_words_to_dec.argtypes = [_u64p, C.c_int, _u64p, C.c_int]; L.orc_words_to_dec.restype = None
@@ -152,6 +156,35 @@ def dft_bitrev(x, p, w):
     x = np.ascontiguousarray(x, dtype=np.uint64); X = np.zeros_like(x)
     lib().orc_dft_bitrev(_p(x), _p(X), int(len(x)).bit_length() - 1, p, w)
     return X
+
+
+def _logk3(n):
+    m = n // 3
+    assert n == 3 * m and m & (m - 1) == 0, "length must be 3 2^k"
+    return m.bit_length() - 1
+
+
+def root3k(p, logk):
+    return int(lib().orc_root3k(p, logk))
+
+
+def dft3(x, p, w):
+    """O(n^2) DFT of n = 3 2^k points; X[3t + r] at r 2^k + bitrev(t)."""
+    x = np.ascontiguousarray(x, dtype=np.uint64); X = np.zeros_like(x)
+    lib().orc_dft3(_p(x), _p(X), _logk3(len(x)), p, w)
+    return X
+
+
+def ntt3_fwd(x, p, w):
+    x = np.array(x, dtype=np.uint64)
+    lib().orc_ntt3_fwd(_p(x), _logk3(len(x)), p, w)
+    return x
+
+
+def ntt3_inv(x, p, w):
+    x = np.array(x, dtype=np.uint64)
+    lib().orc_ntt3_inv(_p(x), _logk3(len(x)), p, w)
+    return x
 
 
 def crt_carry(planes, primes, nc, x=None):

@@ -8,6 +8,7 @@
  *   - the base-10^18 schoolbook product with __int128 columns                        (E/bigint.c:77-110)
  *   - c = a b + x with the top carry limb                                            (E/rns_mul.h:48-51, E/rns_mul.c:403)
  *   - DIF forward (natural in, bit-reversed out) / DIT inverse (x n^-1) NTTs        (E/ntt.h:1-8, E/ntt.c:630-669)
+ *   - transforms of 3 2^k points: radix-3 layer over the thirds + 2^k NTTs          (E/ntt.c:671-779)
  *   - Garner for 3 or 4 primes                                                       (E/crt.c:114-144)
  *   - base-10^18 digits of a CRT value and the windowed limb sum                     (E/modarith.h:71-105, E/crt.c:146-199,
  *                                                                                     E/rns_mul.c:332-405)
@@ -56,6 +57,13 @@ void orc_ntt_fwd(uint64_t *x, int logn, uint64_t p, uint64_t w);      /* natural
 void orc_ntt_inv(uint64_t *x, int logn, uint64_t p, uint64_t w);      /* bit-reversed -> natural, x n^-1 (uses w^-1) */
 /* O(n^2) DFT X[k] = sum_i x[i] w^(ik), written at position bitrev(k) */
 void orc_dft_bitrev(const uint64_t *x, uint64_t *X, int logn, uint64_t p, uint64_t w);
+
+/* transforms of n = 3 2^logk points (E/ntt.c:671-779): w a primitive n-th root; output order: X[3t + r] at position
+ * r 2^logk + bitrev(t) */
+uint64_t orc_root3k(uint64_t p, int logk);           /* g^((p-1)/(3 2^logk)), 0 if 3 2^logk does not divide p-1 */
+void orc_dft3(const uint64_t *x, uint64_t *X, int logk, uint64_t p, uint64_t w);    /* O(n^2) definition */
+void orc_ntt3_fwd(uint64_t *x, int logk, uint64_t p, uint64_t w);                    /* radix-3 layer + 2^logk DIF */
+void orc_ntt3_inv(uint64_t *x, int logk, uint64_t p, uint64_t w);                    /* the inverse, x n^-1 */
 
 /* Garner: r[i] in [0,P[i]) -> the value < prod P as np little-endian words */
 void orc_garner(const uint64_t *r, const uint64_t *P, int np, uint64_t *out);

@@ -205,6 +205,61 @@ void orc_dft_bitrev(const uint64_t *x, uint64_t *X, int logn, uint64_t p, uint64
     }
 }
 
+/* ---- transforms of 3 2^k points (E/ntt.c:671-779; roots E/modarith.h:206-214) ------------------------------------------ */
+uint64_t orc_root3k(uint64_t p, int logk)
+{
+    if (logk < 0 || logk > 63 || ((p - 1) & ((1ULL << logk) - 1)) || ((p - 1) >> logk) % 3) return 0;
+    return orc_powmod(orc_generator(p), ((p - 1) >> logk) / 3, p);
+}
+
+/* the definition: X[k] = sum_i x[i] w^(ik) over n = 3 2^logk points, X[3t + r] written at r 2^logk + bitrev(t) */
+void orc_dft3(const uint64_t *x, uint64_t *X, int logk, uint64_t p, uint64_t w)
+{
+    size_t m = (size_t)1 << logk, n = 3 * m;
+    for (size_t k = 0; k < n; k++) {
+        uint64_t wk = orc_powmod(w, k, p), t = 1, acc = 0;
+        for (size_t i = 0; i < n; i++) { acc = (uint64_t)(((u128)x[i] * t + acc) % p); t = orc_mulmod(t, wk, p); }
+        X[(k % 3) * m + bitrev(k / 3, logk)] = acc;
+    }
+}
+
+/* the reference's decomposition: one DIF radix-3 layer over the thirds with the twiddles w^(r j), then the 2^logk
+ * DIF transforms of the thirds with the root w^3 (E/ntt.c:671-681, k_r3_fwd) */
+void orc_ntt3_fwd(uint64_t *x, int logk, uint64_t p, uint64_t w)
+{
+    size_t m = (size_t)1 << logk;
+    uint64_t w3 = orc_powmod(w, m, p), w3s = orc_mulmod(w3, w3, p), wj = 1;
+    for (size_t j = 0; j < m; j++) {
+        uint64_t a0 = x[j], a1 = x[j + m], a2 = x[j + 2 * m];
+        uint64_t y0 = (uint64_t)(((u128)a0 + a1 + a2) % p);
+        uint64_t y1 = (uint64_t)(((u128)a0 + orc_mulmod(a1, w3, p) + orc_mulmod(a2, w3s, p)) % p);
+        uint64_t y2 = (uint64_t)(((u128)a0 + orc_mulmod(a1, w3s, p) + orc_mulmod(a2, w3, p)) % p);
+        x[j] = y0; x[j + m] = orc_mulmod(y1, wj, p); x[j + 2 * m] = orc_mulmod(y2, orc_mulmod(wj, wj, p), p);
+        wj = orc_mulmod(wj, w, p);
+    }
+    uint64_t wm = orc_powmod(w, 3, p);
+    for (int r = 0; r < 3; r++) orc_ntt_fwd(x + r * m, logk, p, wm);
+}
+
+/* the inverse: the 2^logk inverses of the thirds (x 2^-logk), then the mirrored layer with w^(-r j), w3^(-r i) and
+ * 3^-1 (E/ntt.c:683-694, k_r3_inv) */
+void orc_ntt3_inv(uint64_t *x, int logk, uint64_t p, uint64_t w)
+{
+    size_t m = (size_t)1 << logk;
+    uint64_t wm = orc_powmod(w, 3, p);
+    for (int r = 0; r < 3; r++) orc_ntt_inv(x + r * m, logk, p, wm);
+    uint64_t wi = orc_invmod(w, p), w3i = orc_powmod(wi, m, p), w3is = orc_mulmod(w3i, w3i, p), inv3 = orc_invmod(3, p);
+    uint64_t wj = 1;
+    for (size_t j = 0; j < m; j++) {
+        uint64_t a0 = x[j], a1 = orc_mulmod(x[j + m], wj, p), a2 = orc_mulmod(x[j + 2 * m], orc_mulmod(wj, wj, p), p);
+        uint64_t x0 = (uint64_t)(((u128)a0 + a1 + a2) % p);
+        uint64_t x1 = (uint64_t)(((u128)a0 + orc_mulmod(a1, w3i, p) + orc_mulmod(a2, w3is, p)) % p);
+        uint64_t x2 = (uint64_t)(((u128)a0 + orc_mulmod(a1, w3is, p) + orc_mulmod(a2, w3i, p)) % p);
+        x[j] = orc_mulmod(x0, inv3, p); x[j + m] = orc_mulmod(x1, inv3, p); x[j + 2 * m] = orc_mulmod(x2, inv3, p);
+        wj = orc_mulmod(wj, wi, p);
+    }
+}
+
 /* ---- Garner (E/crt.c:114-144, any np) ---------------------------------------------------------------------------------- */
 /* w (nw words) mod p */
 static uint64_t words_mod(const uint64_t *w, int nw, uint64_t p)

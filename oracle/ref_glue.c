@@ -34,3 +34,4 @@ void ref_tables(uint64_t *gen, uint64_t *w33, uint64_t *w3x33, uint64_t *mu115)
     for (int i = 0; i < EC_NP; i++) { gen[i] = ec_G[i]; w33[i] = ec_W33[i]; w3x33[i] = ec_W3X33[i]; mu115[i] = ec_MU115[i]; }
 }
 uint64_t ref_root(int i, int logn) { return ec_root(i, logn); }
+uint64_t ref_root3(int i, int logk) { return ec_root3(i, logk); }

@@ -40,6 +40,10 @@ def main():
     out["generators"] = [int(v) for v in tabs[0]]
     out["w33"] = [int(v) for v in tabs[1]]
     out["roots"] = {str(k): [int(R.ref_root(i, k)) for i in range(4)] for k in (1, 2, 10, 20, 28, 33)}
+    # roots of order 3 2^k (E/modarith.h:63,206-214)
+    R.ref_root3.argtypes = [C.c_int, C.c_int]; R.ref_root3.restype = C.c_uint64
+    out["w3x33"] = [int(v) for v in tabs[2]]
+    out["roots3"] = {str(k): [int(R.ref_root3(i, k)) for i in range(4)] for k in (0, 1, 2, 10, 20, 33)}
     out["np3_max_terms"] = int(R.ref_np3_max_terms())
 
     # operand generator (E/tests/harness.h:55-81)

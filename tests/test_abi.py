@@ -53,6 +53,36 @@ def test_roots_exact_order_and_chain(i):
     assert ntt_amd.root(i, 33) == 0
 
 
+@pytest.mark.parametrize("i", range(4))
+def test_roots3_exact_order_and_chain(i):
+    p = ntt_amd.prime(i)
+    prev = None
+    for k in range(0, 33):
+        w = ntt_amd.root3(i, k)
+        n = 3 << k
+        assert w == O.root3k(p, k)
+        assert pow(w, n, p) == 1 and pow(w, n // 3, p) != 1 and (k == 0 or pow(w, n // 2, p) == p - 1)
+        assert pow(w, 3, p) == ntt_amd.root(i, k)                 # the thirds' transforms use w^3
+        if prev is not None:
+            assert w * w % p == prev
+        prev = w
+    assert ntt_amd.root3(i, 33) == 0
+
+
+def test_product_length_rules():
+    # the reference's rule (FIT): 3 2^(L-2) whenever ncoef fits it; AUTO takes it only where it moves fewer bytes
+    # (the 3 2^k transform's engine needs one pass fewer than the 2^L transform: L = 13, 14, 22, 23)
+    for L in range(6, 31):
+        n = 1 << L
+        for nc in (n // 2 + 1, 3 * n // 4, 3 * n // 4 + 1, n):
+            fits = nc <= 3 * n // 4 and L - 2 >= 4
+            assert ntt_amd.debug_product_len(nc, ntt_amd.R3_NEVER) == (L, 0)
+            assert ntt_amd.debug_product_len(nc, ntt_amd.R3_FIT) == ((L - 2, 1) if fits else (L, 0))
+            auto = ntt_amd.debug_product_len(nc, ntt_amd.R3_AUTO)
+            assert auto == ((L - 2, 1) if fits and L in (13, 14, 22, 23) else (L, 0)), (L, nc, auto)
+    assert ntt_amd.debug_product_len(1, ntt_amd.R3_FIT) == (4, 0)
+
+
 def test_max_terms():
     ps = [ntt_amd.prime(i) for i in range(4)]
     for np_ in (3, 4):

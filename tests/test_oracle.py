@@ -152,6 +152,51 @@ def test_ntt_roundtrip_and_convolution(logn):
         assert [int(v) for v in conv] == want
 
 
+# ---- transforms of 3 2^k points -----------------------------------------------------------------------------------------
+def test_reference_roots3_reproduced():
+    # the reference's W3X33 table and ec_root3 (E/modarith.h:63,206-214) follow from the smallest generator
+    for i, p in enumerate(REF_P):
+        assert O.root3k(p, 33) == GOLD["w3x33"][i]
+        for k, vals in GOLD["roots3"].items():
+            assert O.root3k(p, int(k)) == vals[i]
+        w = O.root3k(p, 20)
+        n = 3 << 20
+        assert pow(w, n, p) == 1 and pow(w, n // 2, p) != 1 and pow(w, n // 3, p) != 1
+
+
+def test_dft3_hand_computed():
+    # n = 3 mod 7 (w = 2, a cube root of unity): X = [6, 17, 57] mod 7 = [6, 3, 1], order X[3t + r] at r + 0
+    assert [int(v) for v in O.dft3(u64([1, 2, 3]), 7, 2)] == [6, 3, 1]
+    # n = 6 mod 7 (w = 3 of order 6), x = e_1: X[k] = 3^k = [1, 3, 2, 6, 4, 5]; position 2r + t holds X[3t + r]
+    assert [int(v) for v in O.dft3(u64([0, 1, 0, 0, 0, 0]), 7, 3)] == [1, 6, 3, 4, 2, 5]
+    # n = 12 mod 13 (w = 2 of order 12), x = e_1: X[k] = 2^k; position 4r + bitrev2(t) holds X[3t + r]
+    assert [int(v) for v in O.dft3(u64([0, 1] + [0] * 10), 13, 2)] == [1, 12, 8, 5, 2, 11, 3, 10, 4, 9, 6, 7]
+
+
+@pytest.mark.parametrize("logk", [0, 1, 2, 3, 6])
+def test_ntt3_equals_dft3(logk):
+    for p in (REF_P[2], 1125844072267777):
+        w = O.root3k(p, logk)
+        x, _ = O.gen_limbs(3 << logk, 77 + logk)
+        x = x % np.uint64(p)
+        assert np.array_equal(O.ntt3_fwd(x, p, w), O.dft3(x, p, w))
+
+
+@pytest.mark.parametrize("logk", [1, 3, 7, 12])
+def test_ntt3_roundtrip_and_convolution(logk):
+    p = 1125818302464001
+    w = O.root3k(p, logk)
+    n = 3 << logk
+    a, s = O.gen_limbs(n, 500 + logk); b, _ = O.gen_limbs(n, s)
+    a %= np.uint64(p); b %= np.uint64(p)
+    assert np.array_equal(O.ntt3_inv(O.ntt3_fwd(a, p, w), p, w), a)
+    if n <= 400:                                              # cyclic convolution of length 3 2^k by brute force
+        A, Bf = O.ntt3_fwd(a, p, w), O.ntt3_fwd(b, p, w)
+        conv = O.ntt3_inv(u64([(int(u) * int(v)) % p for u, v in zip(A, Bf)]), p, w)
+        ai, bi = [int(v) for v in a], [int(v) for v in b]
+        assert [int(v) for v in conv] == [sum(ai[i] * bi[(k - i) % n] for i in range(n)) % p for k in range(n)]
+
+
 # ---- Garner, digits, CRT ---------------------------------------------------------------------------------------------
 def test_garner_matches_reference_vectors():
     L = O.lib()
