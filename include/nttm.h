@@ -15,6 +15,7 @@
  *   nttm_init / nttm_shutdown     rns_init / rns_shutdown                         E/rns_mul.h:21,40
  *   nttm_mul, nttm_mul_add        rns_mul_dist_db on one device = rns_local_core   E/rns_mul.h:54,76; E/rns_mul.c:1057
  *   nttm_mul_batch                rns_mul_batch(rns_prod *, N)                     E/rns_mul.h:50-53; E/rns_mul.c:792
+ *   nttm_mul_cut                  rns_mul_low_db / _high_db / _band_db, rns_mul_local  E/rns_mul.h:66-73
  *   nttm_prod                     rns_prod                                         E/rns_mul.h:50-51
  *   nttm_ntt_fwd / nttm_ntt_inv   ntt_fwd / ntt_inv                                E/ntt.h:45-46
  *   nttm_pw                       ntt_pw                                           E/ntt.h:47
@@ -89,6 +90,11 @@ int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_
 /* c[0 .. na+nb+1) = a b + x, nx <= na + nb; the top limb is the final carry */
 int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
                  const uint64_t *x, size_t nx, void *stream);
+/* the cut product: c[0 .. min(na+nb, w)) = a b mod 10^(18 w) (w = SIZE_MAX: the whole product).  The operands are cut
+ * to their low w limbs first.  The caller reads only limbs at or above lowcut (lowcut <= min(na+nb, w)); a product
+ * that fits one transform is formed whole, so every limb written is exact.  No carry beyond limb w-1 is checked. */
+int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut, size_t w,
+                 void *stream);
 /* N independent products; P is HOST memory holding device pointers.  Products are grouped by transform length and
  * run batched; ncn is filled on return (the call synchronizes the stream). */
 int nttm_mul_batch(nttm_prod *P, size_t N, void *stream);

@@ -63,6 +63,7 @@ def lib():
         L.nttm_mul.argtypes = [vp, vp, sz, vp, sz, vp]; L.nttm_mul.restype = i32
         L.nttm_mul_add.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp]; L.nttm_mul_add.restype = i32
         L.nttm_mul_batch.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_batch.restype = i32
+        L.nttm_mul_cut.argtypes = [vp, vp, sz, vp, sz, sz, sz, vp]; L.nttm_mul_cut.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
@@ -164,6 +165,20 @@ def mul_add(a, b, x, out=None, stream=None):
     c = out if out is not None else torch.empty(na + nb + 1, dtype=torch.int64, device=a.device)
     _check(lib().nttm_mul_add(_ptr(c), _ptr(a), na, _ptr(b), nb, _ptr(x), x.numel(), _stream(stream)))
     return c[:na + nb + 1]
+
+
+WHOLE = 2 ** 64 - 1          # w of a cut product that keeps every limb (SIZE_MAX)
+
+
+def mul_cut(a, b, lowcut=0, w=WHOLE, out=None, stream=None):
+    """rns_mul_low_db / high_db / band_db analog: the low min(na + nb, w) limbs of a b (operands cut to w limbs)."""
+    import torch
+    na, nb = a.numel(), b.numel()
+    n = min(na + nb, w)
+    c = out if out is not None else torch.empty(max(n, 1), dtype=torch.int64, device=a.device)
+    _check(lib().nttm_mul_cut(_ptr(c), _ptr(a) if na else None, na, _ptr(b) if nb else None, nb, lowcut, w,
+                              _stream(stream)))
+    return c[:n]
 
 
 def make_prods(items):

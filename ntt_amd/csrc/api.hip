@@ -336,6 +336,7 @@ struct CrtDesc {                 /* must match crt.hip */
     const uint64_t *x; uint64_t nx;
     uint64_t *c; uint64_t nc;
     uint64_t first_block;
+    uint64_t trunc;
 };
 
 static int choose_np(size_t mn, int *np)
@@ -441,24 +442,26 @@ static void choose_len(uint64_t ncoef, int *logn_out, int *r3_out)
 }
 
 /* c = a b (+ x), device pointers; synchronous (returns after the stream has drained) */
+/* wcut: write only the low wcut limbs of the result (the value mod B^wcut; no check of what lies above) */
 static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
-                    hipStream_t s)
+                    hipStream_t s, size_t wcut = SIZE_MAX)
 {
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
-    const size_t nc = na + nb + (x ? 1 : 0);
+    const size_t nc_full = na + nb + (x ? 1 : 0), nc = nc_full < wcut ? nc_full : wcut;
     if (x && nx > na + nb) return nttm_fail(NTTM_EINVAL, "nx = %zu exceeds na + nb = %zu", nx, na + nb);
     if (!c && nc) return nttm_fail(NTTM_EINVAL, "null result pointer");
     if ((na && !a) || (nb && !b) || (nx && !x)) return nttm_fail(NTTM_EINVAL, "null operand pointer");
     if (!na || !nb) {
         if (nc) HIPCK(hipMemsetAsync(c, 0, nc * 8, s));
-        if (x && nx) HIPCK(hipMemcpyAsync(c, x, nx * 8, hipMemcpyDeviceToDevice, s));
+        if (x && nx && nc) HIPCK(hipMemcpyAsync(c, x, (nx < nc ? nx : nc) * 8, hipMemcpyDeviceToDevice, s));
         HIPCK(hipStreamSynchronize(s));
         return 0;
     }
+    if (!nc) return 0;
     int np; RCK(choose_np(na < nb ? na : nb, &np));
-    const uint64_t ncoef = na + nb - 1;
+    const uint64_t ncoef_full = na + nb - 1, ncoef = ncoef_full < nc ? ncoef_full : nc;   /* coefficients read */
     int logn, r3;
-    choose_len(ncoef, &logn, &r3);
+    choose_len(ncoef_full, &logn, &r3);
     if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", logn, NTTM_LOGN_MAX);
     const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn;             /* points per plane */
     const size_t planes_bytes = (size_t)2 * np * n * 8;
@@ -472,11 +475,13 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     HIPCK(hipEventRecord(g.ev[0], s));
     RCK(product_transforms(np, logn, r3, 1, A, Bp, a, na, b, nb, nullptr, s, g.ev[1]));
     HIPCK(hipEventRecord(g.ev[2], s));
-    CrtDesc hd; hd.pl = A; hd.plane = n; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
+    CrtDesc hd; memset(&hd, 0, sizeof hd);
+    hd.pl = A; hd.plane = n; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
+    hd.trunc = nc < nc_full;
     memcpy(g.hpin, &hd, sizeof hd);
     HIPCK(hipMemcpyAsync(ddesc, g.hpin, sizeof hd, hipMemcpyHostToDevice, s));
     RCK(nttm_crt_launch_batch(ddesc, 1, nttm_crt_blocks(nc), np, g.cc, scratch, s,
-                              8.0 * np * (double)ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? nx : 0)));
+                              8.0 * np * (double)ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? (nx < nc ? nx : nc) : 0)));
     HIPCK(hipEventRecord(g.ev[3], s));
     HIPCK(hipMemcpyAsync(g.hpin + 64, scratch, 8, hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
@@ -496,6 +501,16 @@ extern "C" int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uin
 {
     if (!x) return nttm_fail(NTTM_EINVAL, "nttm_mul_add: null x");
     return mul_core(c, a, na, b, nb, x, nx, (hipStream_t)stream);
+}
+/* the cut product (E/rns_dist.c:581-589,684-693; E/rns_mul.c rns_mul_local): the operands cut to their low w limbs,
+ * the low min(na + nb, w) limbs of the product written.  A product that fits one transform is formed whole (the
+ * reference skips no piece of it either), so lowcut only states which limbs the caller reads. */
+extern "C" int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut,
+                            size_t w, void *stream)
+{
+    const size_t nout = na + nb < w ? na + nb : w;
+    if (lowcut > nout) return nttm_fail(NTTM_EINVAL, "nttm_mul_cut: lowcut %zu above the %zu limbs written", lowcut, nout);
+    return mul_core(c, a, na < w ? na : w, b, nb < w ? nb : w, nullptr, 0, (hipStream_t)stream, w);
 }
 extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
 {
@@ -728,7 +743,8 @@ extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, i
         if (!planes[i] && ncoef) return nttm_fail(NTTM_EINVAL, "null plane %d", i);
         if (ncoef) RCK(nttm_run_cvt_u64_f64(pl + (size_t)i * ncoef, planes[i], ncoef, s));
     }
-    CrtDesc hd; hd.pl = pl; hd.plane = ncoef; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
+    CrtDesc hd; memset(&hd, 0, sizeof hd);
+    hd.pl = pl; hd.plane = ncoef; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
     HIPCK(hipStreamSynchronize(s));
     memcpy(g.hpin, &hd, sizeof hd);
     CrtDesc *dd = (CrtDesc *)(g.ws + pbytes);

@@ -40,6 +40,7 @@ struct CrtDesc {                         /* one product; must match api.hip */
     const uint64_t *x; uint64_t nx;
     uint64_t *c; uint64_t nc;
     uint64_t first_block;                /* the product's first block in the launch */
+    uint64_t trunc;                      /* nonzero: c is the value mod B^nc (nothing above it is checked) */
 };
 struct CrtArgs {
     const CrtDesc *desc; uint64_t nprod;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         const uint64_t k = k0 + off;
         if (k < dsc.nc) cg[k] = wv[i] + oin - (oout ? NTTM_B : 0);
         const uint32_t cout1 = add_cout(GX[i], GY[i], c1);
-        if (k + 1 == dsc.nc) {                                     /* the last limb: nothing may remain beyond it */
+        if (k + 1 == dsc.nc && !dsc.trunc) {                       /* the last limb: nothing may remain beyond it */
             const uint64_t CI1 = (GX[i] + GY[i] + c1) ^ GX[i] ^ GY[i];
             const uint32_t oout1 = lane == 63 ? cout1 : (uint32_t)(CI1 >> (lane + 1)) & 1u;
             uint64_t extra = qlast;

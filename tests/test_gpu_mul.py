@@ -85,6 +85,27 @@ def test_mul_add(nttm):
     assert to_int(c) == (B ** 3000 - 1) ** 2 + B ** 6000 - 1 and int(c[6000]) == 1
 
 
+def test_mul_cut(nttm):
+    """the cut product (rns_mul_low_db / high_db / band_db): the low min(na+nb, w) limbs of the FULL product"""
+    for na, nb in ((1, 1), (37, 100), (3000, 2500), (20000, 7)):
+        a, s = O.gen_limbs(na, na + 5 * nb)
+        b, _ = O.gen_limbs(nb, s)
+        full = O.mul_school(a, b)
+        da, db = dev(a), dev(b)
+        for w in sorted({1, 2, 7, nb, na, na + nb - 1, na + nb, na + nb + 9, nttm.WHOLE}):
+            for lowcut in (0, min(w, na + nb) // 2):
+                say("mul_cut", na, nb, lowcut, w)
+                c = host(nttm.mul_cut(da, db, lowcut, w))
+                assert np.array_equal(c, full[:min(w, na + nb)]), (na, nb, lowcut, w)
+    # huge operands, small window: only the low w limbs of each operand enter the transform
+    n = 1 << 22
+    a, s = O.gen_limbs(n, 91)
+    b, _ = O.gen_limbs(n, s)
+    say("mul_cut low 1000 of 2^22 x 2^22")
+    c = host(nttm.mul_cut(dev(a), dev(b), 0, 1000))
+    assert np.array_equal(c, O.mul_school(a[:1000], b[:1000])[:1000])
+
+
 def test_zero_operands(nttm):
     import torch
     a, _ = O.gen_limbs(10, 1)
