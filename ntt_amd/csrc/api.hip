@@ -6,6 +6,7 @@
 #include <string.h>
 #include <stdlib.h>
 #include <vector>
+#include <map>
 #include <algorithm>
 #include "internal.h"
 #include "plan.h"
@@ -369,31 +370,38 @@ static int product_tile_log(int logn, int tl = 0)
 }
 
 /* the transforms of cnt products of one length: 2^logn points (r3 = 0), or 3 2^logn points (r3 = 1: a radix-3 layer,
- * reading the limbs, and the 2^logn engine on the 3 cnt thirds).  Plane i of A / B holds the cnt transforms of prime
- * i back to back.  After it, A holds the cyclic convolutions mod each prime, natural order. */
+ * reading the limbs, and the 2^logn engine on the 3 cnt thirds).  Plane i of A holds the cnt transforms of prime i back
+ * to back.  After it, A holds the cyclic convolutions mod each prime, natural order.
+ * Shared B operands (bmap non-null): B's planes hold the cnt_b distinct operands of ldesc_b; A's engine transform t
+ * pairs with B's engine transform bmap[t] in the double-fused pass. */
 static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A, double *Bp, const uint64_t *a,
                               uint64_t na, const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s,
-                              hipEvent_t mid)
+                              hipEvent_t mid, uint64_t cnt_b = 0, const LoadDesc *ldesc_b = nullptr,
+                              const uint32_t *bmap = nullptr)
 {
-    const uint64_t tb = r3 ? 3 * cnt : cnt;                      /* engine transforms of 2^logn points */
+    if (!bmap) { cnt_b = cnt; ldesc_b = ldesc; }
+    const uint64_t tb = r3 ? 3 * cnt : cnt, tbb = r3 ? 3 * cnt_b : cnt_b;   /* engine transforms of 2^logn points */
     Plan pl; plan_make2(&pl, logn, tb, product_tile_log(logn), NTTM_FUSED_LOG);
-    const uint64_t n = (uint64_t)1 << logn, plane = tb * n, ntot = r3 ? 3 * n : n;
+    Plan plb; plan_make2(&plb, logn, tbb, product_tile_log(logn), NTTM_FUSED_LOG);
+    const uint64_t n = (uint64_t)1 << logn, plane = tb * n, plane_b = tbb * n, ntot = r3 ? 3 * n : n;
     const int last = pl.npass - 1;
+    if (bmap && (plb.npass != pl.npass || pl.p[last].m + pl.p[last].logh > logn || (last == 0 && !r3)))
+        return nttm_fail(NTTM_EINVAL, "internal: shared B operands on this length");
     for (int i = 0; i < np; i++) {                               /* B, every prime, all passes but the last */
-        double *Bi = Bp + (size_t)i * plane;
+        double *Bi = Bp + (size_t)i * plane_b;
         const uint64_t p = NTTM_PRIMES[i];
         const double2 ninv = h_shoup(h_inv(ntot % p, p), p);     /* the inverse's 1/length, folded into B */
-        PassArgs fb = base_args(i, Bi, logn, tb, false);
-        if (r3) RCK(nttm_run_r3(false, Bi, cnt, logn, g.pk[i], g.twf[i].b3, ldesc ? nullptr : b, nb, ldesc, 1, &ninv, s));
-        else { fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1; fb.in_scale = ninv; fb.do_in_scale = 1; }
+        PassArgs fb = base_args(i, Bi, logn, tbb, false);
+        if (r3) RCK(nttm_run_r3(false, Bi, cnt_b, logn, g.pk[i], g.twf[i].b3, ldesc_b ? nullptr : b, nb, ldesc_b, 1, &ninv, s));
+        else { fb.limbs = ldesc_b ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc_b; fb.which = 1; fb.in_scale = ninv; fb.do_in_scale = 1; }
         for (int j = 0; j < last; j++) {
-            RCK(nttm_run_pass(false, fb, pl.p[j], s));
+            RCK(nttm_run_pass(false, fb, plb.p[j], s));
             fb.limbs = nullptr; fb.ldesc = nullptr; fb.do_in_scale = 0;
         }
     }
     if (mid) HIPCK(hipEventRecord(mid, s));
     for (int i = 0; i < np; i++) {                               /* A, every prime */
-        double *Ai = A + (size_t)i * plane, *Bi = Bp + (size_t)i * plane;
+        double *Ai = A + (size_t)i * plane, *Bi = Bp + (size_t)i * plane_b;
         const uint64_t p = NTTM_PRIMES[i];
         PassArgs fa = base_args(i, Ai, logn, tb, false);
         if (r3) RCK(nttm_run_r3(false, Ai, cnt, logn, g.pk[i], g.twf[i].b3, ldesc ? nullptr : a, na, ldesc, 0, nullptr, s));
@@ -404,6 +412,7 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
         }
         /* B's last forward pass, A's last forward pass, the pointwise product and A's first inverse pass */
         PassArgs fb = base_args(i, Bi, logn, tb, false);
+        fb.bmap = bmap;
         if (last == 0 && !r3) {                                  /* a single-pass transform reads B's limbs here */
             fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
             fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
@@ -587,23 +596,56 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
             cd[j].c = q.c; cd[j].nc = q.na + q.nb + (q.x ? 1 : 0); cd[j].first_block = nblk;
             nblk += nttm_crt_blocks(cd[j].nc);
         }
-        const size_t planes_bytes = (size_t)2 * np * cnt * n * 8;
+        /* products that share their B operand (the same b and nb: the binary-splitting tree's P1 Q2 + P2 and Q1 Q2,
+         * E/rns_mul.c:476-495) transform it once, where B has planes (a transform of more than one pass, or a radix-3
+         * length) and the double-fused tiles lie inside one transform */
+        std::vector<uint32_t> slot(cnt);
+        std::vector<LoadDesc> ldb;
+        {
+            Plan pq; plan_make2(&pq, logn, r3 ? 3 * cnt : cnt, product_tile_log(logn), NTTM_FUSED_LOG);
+            const bool can = logn >= NTTM_FUSED_LOG && (pq.npass > 1 || r3);
+            std::map<std::pair<const uint64_t *, uint64_t>, uint32_t> seen;
+            for (uint64_t j = 0; j < cnt; j++) {
+                auto key = std::make_pair(ld[j].b, can ? ld[j].nb : j);       /* no sharing: every key distinct */
+                auto it = seen.find(key);
+                if (it == seen.end()) {
+                    it = seen.emplace(key, (uint32_t)ldb.size()).first;
+                    ldb.push_back(ld[j]);
+                }
+                slot[j] = it->second;
+            }
+        }
+        const uint64_t cnt_b = ldb.size(), tb = r3 ? 3 * cnt : cnt;
+        const bool share = cnt_b < cnt;
+        std::vector<uint32_t> bmap(share ? tb : 0);
+        for (uint64_t t = 0; share && t < tb; t++) bmap[t] = r3 ? 3 * slot[t / 3] + (uint32_t)(t % 3) : slot[t];
+        const size_t planes_bytes = (size_t)np * (cnt + cnt_b) * n * 8;
         const size_t off_cd = align_up(planes_bytes, 256);
         const size_t off_ld = align_up(off_cd + cnt * sizeof(CrtDesc), 256);
-        const size_t off_ncn = align_up(off_ld + cnt * sizeof(LoadDesc), 256);
+        const size_t off_ldb = align_up(off_ld + cnt * sizeof(LoadDesc), 256);
+        const size_t off_bmap = align_up(off_ldb + cnt_b * sizeof(LoadDesc), 256);
+        const size_t off_ncn = align_up(off_bmap + bmap.size() * 4, 256);
         const size_t off_crt = align_up(off_ncn + cnt * 8, 256);
         RCK(ws_need(off_crt + nttm_crt_scratch_words(nblk) * 4 + 256));
-        RCK(hpin_need(cnt * (sizeof(CrtDesc) + sizeof(LoadDesc)) + 4096));
+        const size_t h_ld = cnt * sizeof(CrtDesc), h_ldb = h_ld + cnt * sizeof(LoadDesc), h_bmap = h_ldb + cnt_b * sizeof(LoadDesc);
+        RCK(hpin_need(h_bmap + bmap.size() * 4 + 4096));
         double *A = (double *)g.ws, *Bp = A + (size_t)np * cnt * n;
         for (uint64_t j = 0; j < cnt; j++) cd[j].pl = A + j * n;
         char *hp = (char *)g.hpin;
         HIPCK(hipStreamSynchronize(s));                      /* the pinned staging is reused per group */
         memcpy(hp, cd.data(), cnt * sizeof(CrtDesc));
-        memcpy(hp + cnt * sizeof(CrtDesc), ld.data(), cnt * sizeof(LoadDesc));
+        memcpy(hp + h_ld, ld.data(), cnt * sizeof(LoadDesc));
+        memcpy(hp + h_ldb, ldb.data(), cnt_b * sizeof(LoadDesc));
+        if (share) memcpy(hp + h_bmap, bmap.data(), bmap.size() * 4);
         HIPCK(hipMemcpyAsync(g.ws + off_cd, hp, cnt * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
-        HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + cnt * sizeof(CrtDesc), cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + h_ld, cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
+        if (share) {
+            HIPCK(hipMemcpyAsync(g.ws + off_ldb, hp + h_ldb, cnt_b * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
+            HIPCK(hipMemcpyAsync(g.ws + off_bmap, hp + h_bmap, bmap.size() * 4, hipMemcpyHostToDevice, s));
+        }
         RCK(product_transforms(np, logn, r3, cnt, A, Bp, nullptr, 0, nullptr, 0, (const LoadDesc *)(g.ws + off_ld), s,
-                               nullptr));
+                               nullptr, cnt_b, share ? (const LoadDesc *)(g.ws + off_ldb) : nullptr,
+                               share ? (const uint32_t *)(g.ws + off_bmap) : nullptr));
         double crt_bytes = 0;
         for (uint64_t j = 0; j < cnt; j++) crt_bytes += 8.0 * np * cd[j].ncoef + 8.0 * cd[j].nc + 8.0 * cd[j].nx;
         RCK(nttm_crt_launch_batch(g.ws + off_cd, cnt, nblk, np, g.cc, (uint32_t *)(g.ws + off_crt), s, crt_bytes));

@@ -50,6 +50,9 @@ struct PassArgs {
     const uint64_t *limbs; uint64_t nlimbs;     /* batch of one */
     const LoadDesc *ldesc; int which;           /* batches: operand a (which = 0) or b of ldesc[b] */
     double2 in_scale; int do_in_scale;          /* multiply loaded limbs by in_scale (n^-1 folded into one operand) */
+    /* B of the double-fused pass shared between products: transform t reads plane transform bmap[t] (null: t); only
+     * for tiles inside one transform (m + logh <= logn) */
+    const uint32_t *bmap;
 };
 
 struct Ctx;

@@ -604,7 +604,9 @@ __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &p
                                              const double2 *stw = nullptr)
 {
     const bool limb = LIMBS && (q.limbs || q.ldesc);
-    const double *xt = q.x + gbase;
+    /* a shared B: the tile (inside one transform) is read from the transform the map names */
+    const double *xt = q.bmap ? q.x + (((uint64_t)q.bmap[gbase >> q.logn] << q.logn) | (gbase & (((uint64_t)1 << q.logn) - 1)))
+                              : q.x + gbase;
     const int nr = round_count(q.m, MAXR);
     int s_lo, logR;
     round_split(q.m, 0, &s_lo, &logR, false, MAXR);
@@ -968,7 +970,8 @@ int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s)
 int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hipStream_t s)
 {
     if (!pass_geom_ok(g, a.logn, a.batch) || g.lo != 0 || g.logw != 0 || g.m + g.logh > FUSED_LOG ||
-        g.m + g.logh < 4 || !bq.x || bq.logn != a.logn || bq.batch != a.batch || a.do_scale)
+        g.m + g.logh < 4 || !bq.x || bq.logn != a.logn || bq.batch != a.batch || a.do_scale || a.bmap ||
+        (bq.bmap && (g.m + g.logh > a.logn || bq.limbs || bq.ldesc)))
         return nttm_fail(NTTM_EINVAL, "internal: bad double-fused pass geometry");
     const uint64_t tiles = plan_tiles(g, a.logn, a.batch);
     if (tiles > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "internal: too many tiles");

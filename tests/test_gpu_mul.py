@@ -184,6 +184,53 @@ def test_batch_carry_chains(nttm):
         assert to_int(c) == want and (c < B).all()
 
 
+def binsplit_level(n_pairs, seed, lo, hi):
+    """one level of a binary-splitting tree: P = P1 Q2 + P2 and Q = Q1 Q2 for n_pairs nodes, the two products of a
+    node sharing Q2 (E/binsplit.c:1583-1700); P1 and Q1 of a node have one length, as in the tree"""
+    rng = np.random.default_rng(seed)
+    ops, s = [], seed
+    for _ in range(n_pairs):
+        l1, lq2, lp2 = (int(rng.integers(lo, hi)) for _ in range(3))
+        p1, s = O.gen_limbs(l1, s); q1, s = O.gen_limbs(l1, s)
+        q2, s = O.gen_limbs(lq2, s); p2, s = O.gen_limbs(lp2, s)
+        ops.append((p1, q2, p2))
+        ops.append((q1, q2, None))
+    return ops
+
+
+@pytest.mark.parametrize("radix3", [1, 2])
+def test_batch_shared_b(nttm, radix3):
+    """products sharing their B operand transform it once (half the B loads: the profile's bytes of the limb-reading
+    kernels count (products + distinct B) transforms); results equal the oracle.  2 x 6 products of 18000-20000
+    coefficients: 2^15 points (radix3 NEVER) or 3 2^13 (FIT), one group each."""
+    import torch
+    ops = binsplit_level(6, 17 + radix3, 9000, 10000)
+    items = []
+    for a, b, x in ops:
+        nc = len(a) + len(b) + (1 if x is not None else 0)
+        items.append((dev(a), None, None if x is None else dev(x), torch.full((nc,), -1, dtype=torch.int64, device="cuda")))
+    for k in range(0, len(items), 2):                             # the pair's products point at one device copy of Q2
+        qb = dev(ops[k][1])
+        items[k] = (items[k][0], qb, items[k][2], items[k][3])
+        items[k + 1] = (items[k + 1][0], qb, items[k + 1][2], items[k + 1][3])
+    nttm.shutdown(); nttm.init(0, radix3=radix3)
+    try:
+        say("shared-B batch", radix3)
+        prods = nttm.make_prods(items)
+        nttm.profile(True)
+        nttm.mul_batch(prods)
+        stats = nttm.profile_read()
+        nttm.profile(False)
+        for (a, b, x), it in zip(ops, items):
+            want = O.mul_ntt(a, b, [nttm.prime(i) for i in range(3)], x)
+            assert np.array_equal(host(it[3]), want)
+        cnt, cnt_b = len(ops), len(ops) // 2
+        name, pts = ("k_pass<fwd,limb>", 1 << 15) if radix3 == 1 else ("k_r3_fwd<limb>", 3 << 13)
+        assert stats[name][2] == 3 * (cnt + cnt_b) * pts * 8.0, stats
+    finally:
+        torch.cuda.synchronize(); nttm.shutdown(); nttm.init(0)
+
+
 def test_batch_leaf_shapes(nttm):
     """4096 leaf products with log-uniform lengths in [64, 65536] limbs, unequal pairs (the batched-leaf workload),
     every product checked: GMP exactly where available, else the residue form."""
