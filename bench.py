@@ -5,8 +5,9 @@ Default workload: one 2^27 x 2^27-limb product (base-10^18 uint64 limbs drawn by
 0x5EED0004, top limb nonzero), transform length 2^28, three primes, operands resident in HBM.  A step is one full
 product c = a b through the C ABI (load, 9 transforms with the fused pointwise product, CRT with the device carry scan).
 Also measured in the same run: the batched binary-splitting leaves (4096 products, log-uniform lengths in
-[64, 65536] limbs) in one batched call against one call per product, and, on the host, GMP mpz_mul on a bounded sample
-of the same operands as the CPU baseline.
+[64, 65536] limbs) in one batched call against one call per product; one binary-splitting level (256 nodes, each
+P = P1 Q2 + P2 and Q = Q1 Q2 sharing Q2) batched with Q2 shared and unshared; and, on the host, GMP mpz_mul on a
+bounded sample of the same operands as the CPU baseline.
 
 Prints one JSON line (rank 0).  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -154,6 +155,39 @@ def bench_leaves(nttm, reps):
             "batch_equals_each_first64": bool(same)}
 
 
+def bench_binsplit(nttm, reps, nodes=256, lo=12000, hi=16000, seed=0x5EED0007):
+    """One level of a binary-splitting tree as one batch: per node P = P1 Q2 + P2 and Q = Q1 Q2, the two products
+    sharing Q2 (E/binsplit.c:1583-1700).  Timed twice: Q2 shared (one device copy, transformed once per node) and
+    unshared (a copy per product)."""
+    import torch
+    lens = lo + (splitmix(seed, 3 * nodes) % np.uint64(hi - lo)).astype(np.int64)
+    shared, unshared = [], []
+    for i in range(nodes):
+        l1, lq2, lp2 = (int(v) for v in lens[3 * i:3 * i + 3])
+        p1, q1 = gen_limbs(l1, seed + 4 * i + 1), gen_limbs(l1, seed + 4 * i + 2)
+        q2, p2 = gen_limbs(lq2, seed + 4 * i + 3), gen_limbs(lp2, seed + 4 * i + 4)
+        dp1, dq1, dq2, dp2 = to_dev(p1), to_dev(q1), to_dev(q2), to_dev(p2)
+        cp = torch.empty(l1 + lq2 + 1, dtype=torch.int64, device="cuda")
+        cq = torch.empty(l1 + lq2, dtype=torch.int64, device="cuda")
+        shared += [(dp1, dq2, dp2, cp), (dq1, dq2, None, cq)]
+        unshared += [(dp1, dq2.clone(), dp2, cp.clone()), (dq1, dq2.clone(), None, cq.clone())]
+    out = {"nodes": nodes, "products": 2 * nodes, "limbs": [lo, hi]}
+    for tag, items in (("shared_b", shared), ("unshared_b", unshared)):
+        prods = nttm.make_prods(items)
+        nttm.mul_batch(prods)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            nttm.mul_batch(prods)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        out[tag + "_ms"] = round(1e3 * t, 3)
+        out[tag + "_products_per_s"] = round(2 * nodes / t, 1)
+    out["same_result"] = all(torch.equal(a[3], b[3]) for a, b in zip(shared, unshared))
+    out["speedup"] = round(out["unshared_b_ms"] / out["shared_b_ms"], 3)
+    return out
+
+
 PROF_NAMES = (("k_fused2", "k_fused2"), ("k_pass_stream<0", "k_pass<fwd>"), ("k_pass_stream<1", "k_pass<fwd,limb>"),
               ("k_pass_stream<2", "k_pass<inv>"), ("k_crt<", "k_crt"), ("k_crt_scan", "k_crt_scan"),
               ("k_crt_fix", "k_crt_fix"))
@@ -297,6 +331,8 @@ def main():
         del dc
         torch.cuda.empty_cache()
         leaves = bench_leaves(nttm, args.leaf_reps)
+        print("[bench] binary-splitting level", file=sys.stderr, flush=True)
+        leaves["binsplit_level"] = bench_binsplit(nttm, args.leaf_reps)
 
     cpu = None
     if not args.no_cpu_baseline:
