@@ -215,14 +215,19 @@ __device__ __forceinline__ void round_split(int m, int i, int *s_lo, int *logR, 
     default: { constexpr int LR = 1; __VA_ARGS__ } break; }
 
 /* the pass's input element: from the plane, or (the first forward pass) from the operand limbs, zero beyond them */
-__device__ __forceinline__ double input_limb(const PassArgs &a, uint64_t gi, double p, double pinv)
+__device__ __forceinline__ double input_limb_raw(const PassArgs &a, uint64_t gi, double p, double pinv)
 {
     const uint64_t b = gi >> a.logn, k = gi & (((uint64_t)1 << a.logn) - 1);
+    /* the descriptor is copied as values: selecting between addresses (a field or the argument's pointer) would put
+     * the kernel argument on the stack */
     const uint64_t *src; uint64_t nl;
-    if (a.ldesc) { const LoadDesc &d = a.ldesc[b]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; }
+    if (a.ldesc) { const LoadDesc d = a.ldesc[b]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; }
     else { src = a.limbs; nl = a.nlimbs; }
-    if (k >= nl) return 0.0;
-    double x = limb_to_res(src[k], p, pinv);
+    return k < nl ? limb_to_res(src[k], p, pinv) : 0.0;
+}
+__device__ __forceinline__ double input_limb(const PassArgs &a, uint64_t gi, double p, double pinv)
+{
+    double x = input_limb_raw(a, gi, p, pinv);
     if (a.do_in_scale) x = mul_shoup(x, a.in_scale.x, a.in_scale.y, p);
     return x;
 }
@@ -265,6 +270,50 @@ __device__ __forceinline__ double boundary_twiddle(double v, uint32_t col, uint3
 
 /* GM, GW, GH >= 0 fix the tile shape (m, logw, logh) at compile time: round splits, LDS strides and shifts fold to
  * constants; -1 takes them from the arguments */
+/* w^E for a 32-bit exponent fraction E from the four-level table (bq[k 2^8 + x] = w32^(x 2^(8k))), balanced
+ * (|result| <= p/2 + 1) */
+__device__ __forceinline__ double bq_twiddle(const double2 *bq, uint32_t E, double p, double pinv)
+{
+    constexpr uint32_t M = (1u << NTTM_BQ_LOG) - 1;
+    double x = bq[(3 << NTTM_BQ_LOG) + (E >> 24)].x;
+    const double2 t2 = bq[(2 << NTTM_BQ_LOG) + ((E >> 16) & M)];
+    const double2 t1 = bq[(1 << NTTM_BQ_LOG) + ((E >> 8) & M)];
+    const double2 t0 = bq[E & M];
+    x = mul_shoup(x, t2.x, t2.y, p);
+    x = mul_shoup(x, t1.x, t1.y, p);
+    x = mul_shoup(x, t0.x, t0.y, p);
+    return reduce(x, p, pinv);
+}
+
+/* the boundary twiddles w_N^(col k) (N = 2^lom) of the round over row bits [0, LOGR) (s_lo = 0), multiplied into v:
+ * group g of a thread has column c and bit-reversed rows k_j = kb + brev(j) 2^(m - LOGR), so its twiddles are
+ * base * step^brev(j) with base = w^(c kb), step = w^(c 2^(m - LOGR)) -- two table twiddles and a chain of Shoup
+ * products per group instead of a table lookup per point */
+template <int LOGR, int PTS>
+__device__ __forceinline__ void bound_chain(double *v, const PassPos &pp, int lom, uint32_t colbase, const double2 *bq,
+                                            double p, double pinv, bool do_scale = false,
+                                            double2 scale = make_double2(1.0, 0.0))
+{
+    constexpr int R = 1 << LOGR, G = PTS / R;
+    const int L = pp.logw, sh = 32 - lom, ksh = pp.m - LOGR;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t tau = (uint32_t)(g * pp.nthr + pp.t);
+        const uint32_t eb = (tau & ((1u << L) - 1)) | ((tau >> L) << (L + LOGR));
+        const uint32_t c = eb & ((1u << pp.logw) - 1), r = (eb >> pp.logw) & ((1u << pp.m) - 1);
+        const uint32_t kb = pp.m ? (__brev(r) >> (32 - pp.m)) : 0u, col = colbase + c;
+        double base = bq_twiddle(bq, (col * kb) << sh, p, pinv);
+        if (do_scale) base = mul_shoup(base, scale.x, scale.y, p);      /* a factor common to all points, folded in */
+        const double step = bq_twiddle(bq, (col << ksh) << sh, p, pinv), stepq = step / p;
+        double w[R];
+        w[0] = base;
+#pragma unroll
+        for (int i = 1; i < R; i++) w[i] = mul_shoup(w[i - 1], step, stepq, p);
+#pragma unroll
+        for (int j = 0; j < R; j++) v[g * R + j] = mul_barrett(v[g * R + j], w[brev_c<LOGR>(j)], p, pinv);
+    }
+}
+
 template <int MODE, int PTS, int GM = -1, int GW = -1, int GH = -1>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_pass(PassArgs a)
 {
@@ -285,15 +334,34 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint64_t gbase = (hb << (lom + gh)) + (cb << gw);
     const uint32_t colbase = (uint32_t)(cb << gw);
     double *xt = a.x + gbase;
-
-    auto load_at = [&](uint32_t o, uint32_t c, uint32_t k) -> double {
-        double x = MODE == PM_FWD_LIMB ? input_limb(a, gbase + o, p, pinv) : xt[o];
-        if (MODE == PM_INV_PW) x = mul_barrett(x, a.y[y_index(a, gbase + o)], p, pinv);
-        if (INV && a.lo > 0) x = boundary_twiddle(x, colbase + c, k, lom, a.tw, p);
+    /* a limb-input tile inside one transform (lo + m + logh <= logn): its operand, read once per tile */
+    const bool limb_one = MODE == PM_FWD_LIMB && a.lo + gm + gh <= a.logn;
+    const uint64_t *lsrc = nullptr;
+    uint64_t lnl = 0, lk0 = 0;
+    if (limb_one) {
+        const uint64_t tr = gbase >> a.logn;
+        lk0 = gbase & (((uint64_t)1 << a.logn) - 1);
+        if (a.ldesc) { const LoadDesc d = a.ldesc[tr]; lsrc = a.which ? d.b : d.a; lnl = a.which ? d.nb : d.na; }
+        else { lsrc = a.limbs; lnl = a.nlimbs; }
+    }
+    /* a strided limb pass folds the input scale into its boundary twiddles */
+    const bool scale_in_chain = MODE == PM_FWD_LIMB && a.lo > 0 && a.do_in_scale;
+    auto limb_at = [&](uint32_t o) -> double {
+        double x;
+        if (limb_one) { const uint64_t k = lk0 + o; x = k < lnl ? limb_to_res(lsrc[k], p, pinv) : 0.0; }
+        else x = input_limb_raw(a, gbase + o, p, pinv);
+        if (a.do_in_scale && !scale_in_chain) x = mul_shoup(x, a.in_scale.x, a.in_scale.y, p);
         return x;
     };
-    auto store_at = [&](uint32_t o, uint32_t c, uint32_t k, double x) {
-        if (!INV && a.lo > 0) x = boundary_twiddle(x, colbase + c, k, lom, a.tw, p);
+
+    /* the boundary twiddles of a strided pass (lo > 0) go with the round over row bits [0, logR) -- the last forward
+     * round, the first inverse one -- as base * step^i chains (bound_chain), applied to v between load and round */
+    auto load_at = [&](uint32_t o, uint32_t, uint32_t) -> double {
+        double x = MODE == PM_FWD_LIMB ? limb_at(o) : xt[o];
+        if (MODE == PM_INV_PW) x = mul_barrett(x, a.y[y_index(a, gbase + o)], p, pinv);
+        return x;
+    };
+    auto store_at = [&](uint32_t o, uint32_t, uint32_t, double x) {
         if (a.do_scale) x = mul_shoup(x, a.scale.x, a.scale.y, p);
         xt[o] = x;
     };
@@ -315,6 +383,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                        uint32_t) { v[sl] = lds.at(e); });)
     }
+    if (INV && a.lo > 0 && active) ROUND_SWITCH(logR, bound_chain<LR, PTS>(v, pp, lom, colbase, a.tw.bq, p, pinv);)
 #pragma unroll
     for (int i = 0; i < (GM >= 0 ? (GM + MAXR - 1) / MAXR : nr); i++) {
         round_split(gm, i, &s_lo, &logR, INV, MAXR);
@@ -328,6 +397,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                uint32_t) { lds.at(e) = v[sl]; });)
     }
+    if (!INV && a.lo > 0 && active)
+        ROUND_SWITCH(logR, bound_chain<LR, PTS>(v, pp, lom, colbase, a.tw.bq, p, pinv, scale_in_chain, a.in_scale);)
     /* last round: the same choice for the store (each thread rewrites only the LDS words it read in this round) */
     if (gw + s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t c,
@@ -367,19 +438,6 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-/* w^E for a 32-bit exponent fraction E from the four-level table in LDS, balanced (|result| <= p/2 + 1) */
-__device__ __forceinline__ double bq_twiddle(const double2 *bq, uint32_t E, double p, double pinv)
-{
-    constexpr uint32_t M = (1u << NTTM_BQ_LOG) - 1;
-    double x = bq[(3 << NTTM_BQ_LOG) + (E >> 24)].x;
-    const double2 t2 = bq[(2 << NTTM_BQ_LOG) + ((E >> 16) & M)];
-    const double2 t1 = bq[(1 << NTTM_BQ_LOG) + ((E >> 8) & M)];
-    const double2 t0 = bq[E & M];
-    x = mul_shoup(x, t2.x, t2.y, p);
-    x = mul_shoup(x, t1.x, t1.y, p);
-    x = mul_shoup(x, t0.x, t0.y, p);
-    return reduce(x, p, pinv);
-}
 
 template <int MODE, int GM, int GW>
 __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t ntiles)
@@ -792,7 +850,7 @@ struct R3Args {
 __device__ __forceinline__ double r3_input(const R3Args &a, uint64_t t, uint64_t k)
 {
     const uint64_t *src; uint64_t nl;
-    if (a.ldesc) { const LoadDesc &d = a.ldesc[t]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; }
+    if (a.ldesc) { const LoadDesc d = a.ldesc[t]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; }
     else { src = a.limbs; nl = a.nlimbs; }
     return k < nl ? limb_to_res(src[k], a.pk.p, a.pk.pinv) : 0.0;
 }
