@@ -170,9 +170,9 @@ static void build_crt_const(CrtConst &cc)
 
 static int build_tables(void)
 {
-    const size_t nst = (size_t)1 << NTTM_STAGE_LOG, nbt = (size_t)1 << NTTM_BT_LOG;
+    const size_t nst = (size_t)1 << NTTM_STAGE_LOG;
     const size_t nbq = (size_t)4 << NTTM_BQ_LOG, nb3 = nbq + 1;
-    const size_t per_dir = nst + 2 * nbt + nbq + nb3, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
+    const size_t per_dir = nst + nbq + nb3, total = (size_t)NTTM_MAX_PRIMES * 2 * per_dir;
     std::vector<double2> h(total);
     for (int i = 0; i < NTTM_MAX_PRIMES; i++) {
         const uint64_t p = NTTM_PRIMES[i];
@@ -181,7 +181,7 @@ static int build_tables(void)
         g.root3x32[i] = h_powmod(gen, ((p - 1) >> 32) / 3, p);
         for (int dir = 0; dir < 2; dir++) {
             const uint64_t w32 = dir ? h_inv(g.root32[i], p) : g.root32[i];
-            double2 *st = &h[((size_t)i * 2 + dir) * per_dir], *bh = st + nst, *bl = bh + nbt, *bq = bl + nbt;
+            double2 *st = &h[((size_t)i * 2 + dir) * per_dir], *bq = st + nst;
             double2 *b3 = bq + nbq;
             const uint64_t W3 = dir ? h_inv(g.root3x32[i], p) : g.root3x32[i];
             for (int k = 0; k < 4; k++) {                                /* b3[k 2^8 + x] = W3^(x 2^(8k)) */
@@ -198,12 +198,6 @@ static int build_tables(void)
                 uint64_t v = 1;
                 for (size_t j = 0; j < ((size_t)1 << s); j++) { st[((size_t)1 << s) + j] = h_shoup(v, p); v = h_mulmod(v, ws, p); }
             }
-            const uint64_t w16 = h_powmod(w32, (uint64_t)1 << NTTM_BT_LOG, p);             /* order 2^16 */
-            uint64_t vh = 1, vl = 1;
-            for (size_t j = 0; j < nbt; j++) {
-                bh[j] = h_shoup(vh, p); vh = h_mulmod(vh, w16, p);
-                bl[j] = h_shoup(vl, p); vl = h_mulmod(vl, w32, p);
-            }
             for (int k = 0; k < 4; k++) {                                /* bq[k 2^8 + x] = w32^(x 2^(8k)) */
                 const uint64_t wk = h_powmod(w32, (uint64_t)1 << (NTTM_BQ_LOG * k), p);
                 uint64_t v = 1;
@@ -218,7 +212,7 @@ static int build_tables(void)
     for (int i = 0; i < NTTM_MAX_PRIMES; i++)
         for (int dir = 0; dir < 2; dir++) {
             const double2 *st = g.tables + ((size_t)i * 2 + dir) * per_dir;
-            TwTables t; t.stage = st; t.bh = st + nst; t.bl = st + nst + nbt; t.bq = st + nst + 2 * nbt;
+            TwTables t; t.stage = st; t.bq = st + nst;
             t.b3 = t.bq + nbq;
             if (dir) g.twi[i] = t; else g.twf[i] = t;
         }

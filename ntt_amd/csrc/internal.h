@@ -9,8 +9,7 @@
 
 #define NTTM_TILE_LOG 13            /* points per transform tile: 2^13 FP64 values = 64 KiB of LDS */
 #define NTTM_STAGE_LOG 14           /* stage-major twiddle table: T[2^s + j] = w_{2^(s+1)}^j for s < 14 */
-#define NTTM_BT_LOG 16              /* boundary twiddles: w_{2^32}^E = H[E >> 16] * L[E & 0xffff] */
-#define NTTM_BQ_LOG 8               /* the same from four tables of 2^8 entries (LDS-resident in streaming passes) */
+#define NTTM_BQ_LOG 8               /* boundary twiddles w_{2^32}^E from four tables of 2^8 entries, one per byte of E */
 #define NTTM_THREADS 256
 
 struct PrimeK {                     /* per-prime constants, passed by value */
@@ -20,7 +19,6 @@ struct PrimeK {                     /* per-prime constants, passed by value */
 
 struct TwTables {                   /* device pointers, one set per prime and direction */
     const double2 *stage;           /* (w, wq) balanced, 2^NTTM_STAGE_LOG entries */
-    const double2 *bh, *bl;         /* boundary tables, 2^NTTM_BT_LOG entries each */
     const double2 *bq;              /* four-level boundary table: bq[k 2^8 + x] = w32^(x 2^(8k)), k < 4 (16 KiB) */
     const double2 *b3;              /* the same for the root W3 of order 3 2^32: b3[k 2^8 + x] = W3^(x 2^(8k)); entry
                                      * 4 2^8 is the FORWARD cube root of unity w3 = W3^(2^32) in both directions */

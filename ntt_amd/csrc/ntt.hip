@@ -257,17 +257,6 @@ struct SwzLds {
 
 enum { PM_FWD = 0, PM_FWD_LIMB = 1, PM_INV = 2, PM_INV_PW = 3 };
 
-/* w_N^(col k) for N = 2^lom with the exponent wrapped to 32 bits (only its residue mod 2^lom matters) */
-__device__ __forceinline__ double boundary_twiddle(double v, uint32_t col, uint32_t k, int lom, const TwTables &tw,
-                                                   double p)
-{
-    const uint32_t E = (col * k) << (32 - lom);
-    const double2 h = tw.bh[E >> NTTM_BT_LOG];
-    const double2 l = tw.bl[E & ((1u << NTTM_BT_LOG) - 1)];
-    v = mul_shoup(v, h.x, h.y, p);
-    return mul_shoup(v, l.x, l.y, p);
-}
-
 /* GM, GW, GH >= 0 fix the tile shape (m, logw, logh) at compile time: round splits, LDS strides and shifts fold to
  * constants; -1 takes them from the arguments */
 /* w^E for a 32-bit exponent fraction E from the four-level table (bq[k 2^8 + x] = w32^(x 2^(8k))), balanced
