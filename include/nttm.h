@@ -85,14 +85,19 @@ uint64_t nttm_root(int i, int logn);
 /* largest min(na, nb) a product on np primes reconstructs exactly: floor((p_0 ... p_{np-1} - 1) / (10^18 - 1)^2) */
 uint64_t nttm_max_terms(int np);
 
-/* c[0 .. na+nb) = a b; device pointers.  na or nb == 0 gives c = 0. */
+/* c[0 .. na+nb) = a b; device pointers.  na or nb == 0 gives c = 0.  A product that no single transform holds
+ * (more than 2^32 points, more terms than the prime set reconstructs, or more workspace than nttm_config caps) is
+ * formed as a grid of piece products added into c (the reference's grid, E/rns_dist.c:610-700). */
 int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream);
 /* c[0 .. na+nb+1) = a b + x, nx <= na + nb; the top limb is the final carry */
 int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
                  const uint64_t *x, size_t nx, void *stream);
 /* the cut product: c[0 .. min(na+nb, w)) = a b mod 10^(18 w) (w = SIZE_MAX: the whole product).  The operands are cut
- * to their low w limbs first.  The caller reads only limbs at or above lowcut (lowcut <= min(na+nb, w)); a product
- * that fits one transform is formed whole, so every limb written is exact.  No carry beyond limb w-1 is checked. */
+ * to their low w limbs first.  The caller reads only limbs at or above lowcut (lowcut <= min(na+nb, w)).  A product
+ * that fits one transform is formed whole, so every limb written is exact; a product formed as a grid of pieces
+ * (see nttm_mul) leaves out the pieces ending at or below lowcut, so its limbs from lowcut on may fall short of the
+ * exact product's by what those pieces carry into them (at most one unit of 10^(18 lowcut) per piece left out).
+ * No carry beyond limb w-1 is checked. */
 int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut, size_t w,
                  void *stream);
 /* N independent products; P is HOST memory holding device pointers.  Products are grouped by transform length and

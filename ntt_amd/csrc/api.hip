@@ -34,6 +34,7 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
 uint64_t nttm_crt_blocks(uint64_t nc);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
+int nttm_run_add_carry(uint64_t *c, uint64_t n, const uint64_t *carry, hipStream_t s);
 int nttm_run_r3(bool inv, double *x, uint64_t batch, int logk, const PrimeK &pk, const double2 *b3,
                 const uint64_t *limbs, uint64_t nlimbs, const LoadDesc *ldesc, int which, const double2 *scale,
                 hipStream_t s);
@@ -92,6 +93,7 @@ struct Ctx {
     double2 *tables;
     CrtConst cc;
     char *ws; size_t ws_bytes;               /* device workspace */
+    uint64_t *tmp; size_t tmp_limbs;         /* a piece product of a grid (above one transform) */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     hipEvent_t ev[4];
     double last_ms[3];
@@ -282,6 +284,7 @@ extern "C" void nttm_shutdown(void)
     (void)hipDeviceSynchronize();
     prof_release();
     if (g.ws) (void)hipFree(g.ws);
+    if (g.tmp) (void)hipFree(g.tmp);
     if (g.tables) (void)hipFree(g.tables);
     if (g.hpin) (void)hipHostFree(g.hpin);
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
@@ -445,6 +448,29 @@ static void choose_len(uint64_t ncoef, int *logn_out, int *r3_out)
 }
 
 /* c = a b (+ x), device pointers; synchronous (returns after the stream has drained) */
+/* the one-transform schedule of an na x nb product writing nc limbs: primes, length and workspace bytes; nonzero when
+ * no transform holds it (the prime set's bound, or more than 2^32 points) */
+struct OnePlan { int np, logn, r3; uint64_t n; size_t planes_bytes, crt_words, ws_bytes; };
+static int one_plan(size_t na, size_t nb, size_t nc, OnePlan *o)
+{
+    const int want = g.cfg.nprimes;
+    const size_t mn = na < nb ? na : nb;
+    o->np = want ? want : (uint64_t)mn <= max_terms_np(3) ? 3 : 4;
+    if ((uint64_t)mn > max_terms_np(o->np)) return 1;
+    choose_len(na + nb - 1, &o->logn, &o->r3);
+    if (o->logn > NTTM_LOGN_MAX) return 1;
+    o->n = (uint64_t)(o->r3 ? 3 : 1) << o->logn;
+    o->planes_bytes = (size_t)2 * o->np * o->n * 8;
+    o->crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
+    o->ws_bytes = align_up(o->planes_bytes, 256) + 256 + o->crt_words * 4 + 256;
+    return 0;
+}
+static bool fits_one(size_t na, size_t nb, size_t nc)
+{
+    OnePlan o;
+    return !one_plan(na, nb, nc, &o) && (!g.cfg.workspace_bytes || o.ws_bytes <= g.cfg.workspace_bytes);
+}
+
 /* wcut: write only the low wcut limbs of the result (the value mod B^wcut; no check of what lies above) */
 static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
                     hipStream_t s, size_t wcut = SIZE_MAX)
@@ -495,25 +521,95 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     return 0;
 }
 
+/* ---- products above one transform: the grid (E/rns_dist.c:610-700, our own form) ---------------------------------------
+ * A product that no single transform holds (more than 2^32 points, the prime set's term bound, or the configured
+ * workspace cap) is formed from ka x kb piece products a_i b_j (pieces of ceil(na/ka) and ceil(nb/kb) limbs).  The grid
+ * takes the fewest plane points in total, then the fewest pieces (i, j <= 64).  Each piece is added into c at limb
+ * o = oa + ob by one product with the window c[o .. o + la + lb) as its x (c = a_i b_j + window, written back), and its
+ * top limb (0 or 1) carried into c above the window on the device.  The cuts skip whole pieces: a piece at or above
+ * limb w only adds a multiple of B^w, and one ending at or below lowcut is left out (the caller reads the limbs above
+ * lowcut and tolerates what those pieces would have carried into them, E/rns_dist.c:678-679). */
+static int grid_shape(size_t na, size_t nb, int *ka, int *kb)
+{
+    double best = 0; *ka = *kb = 0;
+    for (int i = 1; i <= 64; i++)
+        for (int j = 1; j <= 64; j++) {
+            const size_t pa = (na + i - 1) / i, pb = (nb + j - 1) / j;
+            if (!fits_one(pa, pb, pa + pb + 1)) continue;
+            OnePlan o; one_plan(pa, pb, pa + pb + 1, &o);
+            const double cost = (double)i * j * (double)o.n;
+            if (!*ka || cost < best || (cost == best && i * j < *ka * *kb)) { best = cost; *ka = i; *kb = j; }
+        }
+    if (!*ka) return nttm_fail(NTTM_EBOUND, "no grid of at most 64 x 64 pieces forms a %zu x %zu-limb product", na, nb);
+    return 0;
+}
+static int tmp_need(size_t limbs)
+{
+    if (limbs <= g.tmp_limbs) return 0;
+    if (g.tmp) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.tmp)); g.tmp = nullptr; g.tmp_limbs = 0; }
+    if (hipMalloc(&g.tmp, limbs * 8) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "hipMalloc of %zu limbs failed", limbs); }
+    g.tmp_limbs = limbs;
+    return 0;
+}
+static int mul_grid(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
+                    size_t nx, hipStream_t s, size_t lowcut, size_t w)
+{
+    const size_t nc_full = na + nb + (x ? 1 : 0), nout = nc_full < w ? nc_full : w;
+    int ka, kb; RCK(grid_shape(na, nb, &ka, &kb));
+    const size_t pa = (na + ka - 1) / ka, pb = (nb + kb - 1) / kb;
+    RCK(tmp_need(pa + pb + 1));
+    HIPCK(hipMemsetAsync(c, 0, nout * 8, s));
+    if (x && nx) HIPCK(hipMemcpyAsync(c, x, (nx < nout ? nx : nout) * 8, hipMemcpyDeviceToDevice, s));
+    for (int j = 0; j < kb; j++)
+        for (int i = 0; i < ka; i++) {
+            const size_t oa = (size_t)i * pa, ob = (size_t)j * pb;
+            if (oa >= na || ob >= nb) continue;
+            const size_t la = na - oa < pa ? na - oa : pa, lb = nb - ob < pb ? nb - ob : pb, o = oa + ob, len = la + lb;
+            if (o >= nout || o + len <= lowcut) continue;             /* at or above w; below the low cut */
+            const size_t wp = nout - o;                               /* limbs of c from o on */
+            if (wp <= len) {                                          /* the piece reaches the top: mod B^wp */
+                RCK(mul_core(g.tmp, a + oa, la, b + ob, lb, c + o, wp, s, wp));
+                HIPCK(hipMemcpyAsync(c + o, g.tmp, wp * 8, hipMemcpyDeviceToDevice, s));
+            } else {
+                RCK(mul_core(g.tmp, a + oa, la, b + ob, lb, c + o, len, s));
+                HIPCK(hipMemcpyAsync(c + o, g.tmp, len * 8, hipMemcpyDeviceToDevice, s));
+                RCK(nttm_run_add_carry(c + o + len, wp - len, g.tmp + len, s));
+            }
+        }
+    HIPCK(hipStreamSynchronize(s));
+    return 0;
+}
+/* one transform when it holds the product, else the grid */
+static int mul_any(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
+                   hipStream_t s, size_t lowcut = 0, size_t w = SIZE_MAX)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    const size_t nc_full = na + nb + (x ? 1 : 0), nout = nc_full < w ? nc_full : w;
+    if (!na || !nb || !nout || fits_one(na, nb, nout)) return mul_core(c, a, na, b, nb, x, nx, s, w);
+    if (x && nx > na + nb) return nttm_fail(NTTM_EINVAL, "nx = %zu exceeds na + nb = %zu", nx, na + nb);
+    if (!c || !a || !b || (nx && !x)) return nttm_fail(NTTM_EINVAL, "null pointer");
+    return mul_grid(c, a, na, b, nb, x, nx, s, lowcut, w);
+}
+
 extern "C" int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream)
 {
-    return mul_core(c, a, na, b, nb, nullptr, 0, (hipStream_t)stream);
+    return mul_any(c, a, na, b, nb, nullptr, 0, (hipStream_t)stream);
 }
 extern "C" int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
                             size_t nx, void *stream)
 {
     if (!x) return nttm_fail(NTTM_EINVAL, "nttm_mul_add: null x");
-    return mul_core(c, a, na, b, nb, x, nx, (hipStream_t)stream);
+    return mul_any(c, a, na, b, nb, x, nx, (hipStream_t)stream);
 }
 /* the cut product (E/rns_dist.c:581-589,684-693; E/rns_mul.c rns_mul_local): the operands cut to their low w limbs,
  * the low min(na + nb, w) limbs of the product written.  A product that fits one transform is formed whole (the
- * reference skips no piece of it either), so lowcut only states which limbs the caller reads. */
+ * reference skips no piece of it either); a grid skips the pieces ending at or below lowcut. */
 extern "C" int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut,
                             size_t w, void *stream)
 {
     const size_t nout = na + nb < w ? na + nb : w;
     if (lowcut > nout) return nttm_fail(NTTM_EINVAL, "nttm_mul_cut: lowcut %zu above the %zu limbs written", lowcut, nout);
-    return mul_core(c, a, na < w ? na : w, b, nb < w ? nb : w, nullptr, 0, (hipStream_t)stream, w);
+    return mul_any(c, a, na < w ? na : w, b, nb < w ? nb : w, nullptr, 0, (hipStream_t)stream, lowcut, w);
 }
 extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
 {
@@ -529,7 +625,7 @@ extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     hipStream_t s = (hipStream_t)stream;
     for (size_t i = 0; i < N; i++) {
-        RCK(mul_core(P[i].c, P[i].a, P[i].na, P[i].b, P[i].nb, P[i].x, P[i].nx, s));
+        RCK(mul_any(P[i].c, P[i].a, P[i].na, P[i].b, P[i].nb, P[i].x, P[i].nx, s));
         const size_t nc = P[i].na + P[i].nb + (P[i].x ? 1 : 0);
         P[i].ncn = nc;
     }

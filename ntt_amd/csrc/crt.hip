@@ -388,6 +388,44 @@ __global__ __launch_bounds__(256) void k_crt_fix(const CrtDesc *desc, uint64_t n
     }
 }
 
+/* c[0 .. n) += *carry (0 or 1), the carry rippling through the leading limbs equal to B - 1 (one workgroup, 2048
+ * limbs per step; a carry beyond limb n - 1 is dropped: the caller's bound makes it zero) */
+__global__ __launch_bounds__(256) void k_add_carry(uint64_t *c, uint64_t n, const uint64_t *carry)
+{
+    __shared__ uint32_t s_first;
+    const int tid = threadIdx.x;
+    const uint64_t cin = *carry;
+    if (!cin) return;
+    for (uint64_t base = 0; base < n; base += 256 * 8) {
+        if (tid == 0) s_first = 0xffffffffu;
+        __syncthreads();
+        uint64_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t off = (uint32_t)(i * 256 + tid);
+            const uint64_t k = base + off;
+            v[i] = k < n ? c[k] : 0;
+            if (k < n && v[i] != NTTM_B - 1) atomicMin(&s_first, off);
+        }
+        __syncthreads();
+        const uint32_t f = s_first;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t off = (uint32_t)(i * 256 + tid);
+            const uint64_t k = base + off;
+            if (k < n && off <= f) c[k] = off < f ? 0 : v[i] + cin;
+        }
+        if (f != 0xffffffffu) return;
+        __syncthreads();
+    }
+}
+int nttm_run_add_carry(uint64_t *c, uint64_t n, const uint64_t *carry, hipStream_t s)
+{
+    if (!n) return 0;
+    hipLaunchKernelGGL(k_add_carry, dim3(1), dim3(256), 0, s, c, n, carry);
+    return nttm_hip_check(hipGetLastError(), "k_add_carry launch");
+}
+
 __global__ void k_norm(const CrtDesc *d, uint64_t nprod, uint64_t *ncn)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

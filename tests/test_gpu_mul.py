@@ -106,6 +106,41 @@ def test_mul_cut(nttm):
     assert np.array_equal(c, O.mul_school(a[:1000], b[:1000])[:1000])
 
 
+def test_grid_above_one_transform(nttm):
+    """products that no transform within the workspace cap holds are formed as a grid of piece products
+    (E/rns_dist.c:610-700): exact against the oracle, with x, with the closed form's carry ripples, and cut"""
+    import torch
+    cap = 4 << 20                                                # 4 MiB: 2^17-point products need a grid
+    nttm.shutdown(); nttm.init(0, workspace_bytes=cap)
+    try:
+        P3 = [nttm.prime(i) for i in range(3)]
+        a, s = O.gen_limbs(60000, 61)
+        b, _ = O.gen_limbs(50000, s)
+        full = O.mul_ntt(a, b, P3)
+        say("grid product")
+        c = host(nttm.mul(dev(a), dev(b)))
+        assert np.array_equal(c, full)
+        x, _ = O.gen_limbs(110000, 62)
+        say("grid product + x")
+        c = host(nttm.mul_add(dev(a), dev(b), dev(x)))
+        assert np.array_equal(c, O.mul_ntt(a, b, P3, x))
+        ones = np.full(65000, B - 1, dtype=np.uint64)                # (B^n - 1)^2: carries through every piece
+        say("grid all ones")
+        c = host(nttm.mul(dev(ones), dev(ones)))
+        n = 65000
+        assert int(c[0]) == 1 and not c[1:n].any() and int(c[n]) == B - 2 and (c[n + 1:] == B - 1).all()
+        say("grid low cut")
+        c = host(nttm.mul_cut(dev(a), dev(b), 0, 70000))
+        assert np.array_equal(c, full[:70000])
+        say("grid high cut")
+        lowcut = 60000
+        c = host(nttm.mul_cut(dev(a), dev(b), lowcut))
+        hi_got, hi_want = to_int(c[lowcut:]), to_int(full[lowcut:])
+        assert 0 <= hi_want - hi_got <= 64 * 64                  # short by the skipped pieces' carries at most
+    finally:
+        torch.cuda.synchronize(); nttm.shutdown(); nttm.init(0)
+
+
 def test_zero_operands(nttm):
     import torch
     a, _ = O.gen_limbs(10, 1)
