@@ -103,6 +103,20 @@ int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, s
 /* N independent products; P is HOST memory holding device pointers.  Products are grouped by transform length and
  * run batched; ncn is filled on return (the call synchronizes the stream). */
 int nttm_mul_batch(nttm_prod *P, size_t N, void *stream);
+/* a node of a binary-splitting tree: P and Q as device limb arrays */
+typedef struct nttm_bs_node {
+    const uint64_t *p; size_t pn;
+    const uint64_t *q; size_t qn;
+} nttm_bs_node;
+/* the product tree of binary splitting (E/binsplit.c:1583-1700): the n leaves are combined pairwise, level by level,
+ * into (P_L Q_R + P_R, Q_L Q_R) -- each level one nttm_mul_batch call with the fused + x and Q_R shared by the pair's
+ * two products; an odd last node moves up unchanged -- until one node remains.  Its P and Q are written to p and q
+ * (capacities p_cap and q_cap limbs, NTTM_EINVAL when too small) with their normalized lengths in *pn and *qn.  The
+ * library holds the inner levels in its own device memory.  Every pair needs |P_R| <= |P_L| + |Q_R| limbs (the product's
+ * x bound), which holds when every leaf has P < Q, as in a convergent series: then every node does.  Synchronizes the
+ * stream. */
+int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q, size_t q_cap,
+                 size_t *qn, void *stream);
 /* the same N products, one product call each (the unbatched baseline of the batch tier) */
 int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
 

@@ -31,6 +31,11 @@ class Prod(C.Structure):
                 ("c", C.c_void_p), ("x", C.c_void_p), ("nx", C.c_size_t), ("ncn", C.c_size_t)]
 
 
+class BsNode(C.Structure):
+    """nttm_bs_node: P and Q of a binary-splitting node (device limbs)."""
+    _fields_ = [("p", C.c_void_p), ("pn", C.c_size_t), ("q", C.c_void_p), ("qn", C.c_size_t)]
+
+
 class KStat(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
 
@@ -64,6 +69,9 @@ def lib():
         L.nttm_mul_add.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp]; L.nttm_mul_add.restype = i32
         L.nttm_mul_batch.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_batch.restype = i32
         L.nttm_mul_cut.argtypes = [vp, vp, sz, vp, sz, sz, sz, vp]; L.nttm_mul_cut.restype = i32
+        L.nttm_bs_tree.argtypes = [C.POINTER(BsNode), sz, vp, sz, C.POINTER(C.c_size_t), vp, sz,
+                                   C.POINTER(C.c_size_t), vp]
+        L.nttm_bs_tree.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
@@ -179,6 +187,22 @@ def mul_cut(a, b, lowcut=0, w=WHOLE, out=None, stream=None):
     _check(lib().nttm_mul_cut(_ptr(c), _ptr(a) if na else None, na, _ptr(b) if nb else None, nb, lowcut, w,
                               _stream(stream)))
     return c[:n]
+
+
+def bs_tree(leaves, stream=None):
+    """binary-splitting product tree over leaves [(P_i, Q_i)] of int64 device tensors: the root (P, Q), normalized."""
+    import torch
+    arr = (BsNode * len(leaves))()
+    for i, (p, q) in enumerate(leaves):
+        arr[i] = BsNode(p.data_ptr() if p.numel() else None, p.numel(), q.data_ptr() if q.numel() else None, q.numel())
+    dev = leaves[0][0].device if leaves[0][0].numel() else leaves[0][1].device
+    pcap = sum(p.numel() for p, _ in leaves) + sum(q.numel() for _, q in leaves) + len(leaves)
+    qcap = sum(q.numel() for _, q in leaves)
+    P = torch.empty(max(pcap, 1), dtype=torch.int64, device=dev)
+    Q = torch.empty(max(qcap, 1), dtype=torch.int64, device=dev)
+    pn, qn = C.c_size_t(), C.c_size_t()
+    _check(lib().nttm_bs_tree(arr, len(leaves), _ptr(P), pcap, C.byref(pn), _ptr(Q), qcap, C.byref(qn), _stream(stream)))
+    return P[:pn.value], Q[:qn.value]
 
 
 def make_prods(items):

@@ -94,6 +94,7 @@ struct Ctx {
     CrtConst cc;
     char *ws; size_t ws_bytes;               /* device workspace */
     uint64_t *tmp; size_t tmp_limbs;         /* a piece product of a grid (above one transform) */
+    uint64_t *bsp[2]; size_t bsp_limbs[2];   /* the inner levels of a binary-splitting tree (even / odd levels) */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     hipEvent_t ev[4];
     double last_ms[3];
@@ -285,6 +286,7 @@ extern "C" void nttm_shutdown(void)
     prof_release();
     if (g.ws) (void)hipFree(g.ws);
     if (g.tmp) (void)hipFree(g.tmp);
+    for (int k = 0; k < 2; k++) if (g.bsp[k]) (void)hipFree(g.bsp[k]);
     if (g.tables) (void)hipFree(g.tables);
     if (g.hpin) (void)hipHostFree(g.hpin);
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
@@ -756,6 +758,71 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         while (n) { uint64_t v; HIPCK(hipMemcpy(&v, q.x + n - 1, 8, hipMemcpyDeviceToHost)); if (v) break; n--; }
         q.ncn = n;
     }
+    return 0;
+}
+
+/* ---- the binary-splitting product tree ----------------------------------------------------------------------------------- */
+static int bsp_need(int k, size_t limbs)
+{
+    if (limbs <= g.bsp_limbs[k]) return 0;
+    if (g.bsp[k]) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.bsp[k])); g.bsp[k] = nullptr; g.bsp_limbs[k] = 0; }
+    if (hipMalloc(&g.bsp[k], limbs * 8) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "hipMalloc of %zu limbs failed", limbs); }
+    g.bsp_limbs[k] = limbs;
+    return 0;
+}
+extern "C" int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q,
+                            size_t q_cap, size_t *qn, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (!leaves || !n || !pn || !qn || (p_cap && !p) || (q_cap && !q)) return nttm_fail(NTTM_EINVAL, "nttm_bs_tree arguments");
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t i = 0; i < n; i++)
+        if ((leaves[i].pn && !leaves[i].p) || (leaves[i].qn && !leaves[i].q)) return nttm_fail(NTTM_EINVAL, "leaf %zu: null limbs", i);
+    std::vector<nttm_bs_node> cur(leaves, leaves + n), nxt;
+    std::vector<nttm_prod> prods;
+    for (int level = 0; cur.size() > 1; level++) {
+        const size_t npairs = cur.size() / 2, odd = cur.size() & 1;
+        size_t total = 0;                                        /* the parents' limbs (the odd node copied) */
+        for (size_t i = 0; i < npairs; i++) total += cur[2 * i].pn + cur[2 * i + 1].qn + 1 + cur[2 * i].qn + cur[2 * i + 1].qn;
+        if (odd) total += cur.back().pn + cur.back().qn;
+        RCK(bsp_need(level & 1, total ? total : 1));
+        uint64_t *pool = g.bsp[level & 1];
+        nxt.assign(npairs + odd, nttm_bs_node());
+        prods.assign(2 * npairs, nttm_prod());
+        size_t off = 0;
+        for (size_t i = 0; i < npairs; i++) {                    /* P = P_L Q_R + P_R, Q = Q_L Q_R: Q_R shared */
+            const nttm_bs_node &L = cur[2 * i], &R = cur[2 * i + 1];
+            if (R.pn > L.pn + R.qn)
+                return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: level %d pair %zu: |P_R| = %zu exceeds |P_L| + |Q_R| = %zu",
+                                 level, i, R.pn, L.pn + R.qn);
+            uint64_t *pc = pool + off; off += L.pn + R.qn + 1;
+            uint64_t *qc = pool + off; off += L.qn + R.qn;
+            nttm_prod &a = prods[2 * i], &b = prods[2 * i + 1];
+            a.a = L.p; a.na = L.pn; a.b = R.q; a.nb = R.qn; a.c = pc; a.x = R.pn ? R.p : nullptr; a.nx = R.pn;
+            b.a = L.q; b.na = L.qn; b.b = R.q; b.nb = R.qn; b.c = qc; b.x = nullptr; b.nx = 0;
+            if (!a.x) { a.x = pc; a.nx = 0; }                     /* P_R = 0: an empty x keeps the carry limb */
+            nxt[i].p = pc; nxt[i].q = qc;
+        }
+        if (odd) {                                               /* the last node moves up (into this level's pool) */
+            const nttm_bs_node &L = cur.back();
+            uint64_t *pc = pool + off; off += L.pn;
+            uint64_t *qc = pool + off; off += L.qn;
+            if (L.pn) HIPCK(hipMemcpyAsync(pc, L.p, L.pn * 8, hipMemcpyDeviceToDevice, s));
+            if (L.qn) HIPCK(hipMemcpyAsync(qc, L.q, L.qn * 8, hipMemcpyDeviceToDevice, s));
+            nxt[npairs].p = pc; nxt[npairs].pn = L.pn; nxt[npairs].q = qc; nxt[npairs].qn = L.qn;
+        }
+        RCK(nttm_mul_batch(prods.data(), prods.size(), stream));
+        for (size_t i = 0; i < npairs; i++) { nxt[i].pn = prods[2 * i].ncn; nxt[i].qn = prods[2 * i + 1].ncn; }
+        cur.swap(nxt);
+    }
+    const nttm_bs_node &root = cur[0];
+    if (root.pn > p_cap || root.qn > q_cap)
+        return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: the root needs %zu and %zu limbs (capacities %zu, %zu)", root.pn,
+                         root.qn, p_cap, q_cap);
+    if (root.pn) HIPCK(hipMemcpyAsync(p, root.p, root.pn * 8, hipMemcpyDeviceToDevice, s));
+    if (root.qn) HIPCK(hipMemcpyAsync(q, root.q, root.qn * 8, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
+    *pn = root.pn; *qn = root.qn;
     return 0;
 }
 

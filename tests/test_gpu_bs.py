@@ -1,0 +1,68 @@
+"""GPU parity of the binary-splitting product tree (nttm_bs_tree, the caller of the batched product in the reference,
+E/binsplit.c:1583-1700) against exact Python integers.  The node combination (P_L Q_R + P_R, Q_L Q_R) is associative,
+so any tree shape gives the same root; the check folds the leaves in a balanced recursion.  For e's series the root
+has closed forms: Q = N!, P = sum_{k=1..N} N!/k!."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import dev, host, say
+
+pytestmark = pytest.mark.gpu
+B = O.B
+
+
+def to_int(limbs):
+    v = 0
+    for x in reversed([int(t) for t in limbs]):
+        v = v * B + x
+    return v
+
+
+def from_int(v):
+    out = []
+    while v:
+        v, r = divmod(v, B)
+        out.append(r)
+    return np.array(out, dtype=np.uint64)
+
+
+def combine(nodes):
+    if len(nodes) == 1:
+        return nodes[0]
+    m = len(nodes) // 2
+    (pl, ql), (pr, qr) = combine(nodes[:m]), combine(nodes[m:])
+    return pl * qr + pr, ql * qr
+
+
+@pytest.mark.parametrize("n", [1, 2, 5000, (1 << 15) + 7])
+def test_bs_tree_e_series(nttm, n):
+    """leaves P_k = 1, Q_k = k + 1: the root is (sum_{k=1..n} n!/k!, n!)"""
+    one = dev(np.array([1], dtype=np.uint64))
+    leaves = [(one, dev(np.array([k + 1], dtype=np.uint64))) for k in range(n)]
+    say("bs tree e", n)
+    P, Q = nttm.bs_tree(leaves)
+    q = to_int(host(Q))
+    assert q == math.factorial(n)
+    p_want, q_want = combine([(1, k + 1) for k in range(n)])
+    assert q_want == q and to_int(host(P)) == p_want
+
+
+def test_bs_tree_random_nodes(nttm):
+    """multi-limb leaves of random lengths with P < Q (as in a convergent series; some P = 0), an odd count at several
+    levels"""
+    rng = np.random.default_rng(7)
+    leaves, ints, s = [], [], 11
+    for i in range(777):
+        lq = int(rng.integers(1, 40))
+        lp = int(rng.integers(0, lq))                            # fewer limbs than Q: P < Q
+        p, s = O.gen_limbs(lp, s) if lp else (np.zeros(0, dtype=np.uint64), s)
+        q, s = O.gen_limbs(lq, s)
+        leaves.append((dev(p), dev(q)))
+        ints.append((to_int(p), to_int(q)))
+    say("bs tree random 777")
+    P, Q = nttm.bs_tree(leaves)
+    p_want, q_want = combine(ints)
+    assert np.array_equal(host(Q), from_int(q_want)) and np.array_equal(host(P), from_int(p_want))
