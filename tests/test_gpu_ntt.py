@@ -1,6 +1,6 @@
 """GPU parity of the transform primitives, the load, the pointwise product and the CRT against the oracle, through the
-C ABI (bit-exact: integer residues).  Mirrors the reference's t_ntt sections 1, 2, 4b and t_crt / t_crtk
-(E/tests/t_ntt.c:182-249,328-366; E/tests/t_crt.c; E/tests/t_crtk.c)."""
+C ABI (bit-exact: integer residues).  Mirrors the reference's t_ntt sections 1, 2, 4, 4b and t_crt / t_crtk
+(E/tests/t_ntt.c:182-249,302-366; E/tests/t_crt.c; E/tests/t_crtk.c)."""
 import numpy as np
 import pytest
 
@@ -171,3 +171,44 @@ def test_crt_overflow_is_reported(nttm):
     with pytest.raises(ntt_amd.NttmError) as e:
         nttm.crt([dev(q) for q in planes], 10, 10)             # the value needs 12 limbs
     assert e.value.code == -6
+
+
+def test_pass_split_independence(nttm):
+    """the forward transform and the product do not depend on the pass split, tile, points per thread, the fixed-shape
+    or streaming kernels, or the length rule (E/tests/t_ntt.c:302-327, section 4)"""
+    import os
+    import torch
+    logn, prime = 24, 1
+    p, w = nttm.prime(prime), nttm.root(prime, logn)
+    x = rand_res(1 << logn, p, 2024)
+    a, s = O.gen_limbs(3 << 19, 5)
+    b, _ = O.gen_limbs(5 << 18, s)
+    configs = [dict(), dict(tile_log=12), dict(pts=8), dict(tile_log=12, pts=8), dict(radix3=1), dict(radix3=2),
+               dict(env="NTTM_GENERIC_PASSES"), dict(env="NTTM_STREAM")]
+    ref_X = ref_c = None
+    try:
+        for cfg in configs:
+            cfg = dict(cfg)
+            env = cfg.pop("env", None)
+            torch.cuda.synchronize(); nttm.shutdown()
+            if env:
+                os.environ[env] = "1" if env == "NTTM_GENERIC_PASSES" else "0"
+            try:
+                nttm.init(0, **cfg)
+                say("split independence", cfg, env)
+                t = dev(x)
+                nttm.ntt_fwd(prime, t, logn)
+                X = host(t)
+                nttm.ntt_inv(prime, t, logn)
+                assert np.array_equal(host(t), x), (cfg, env)
+                c = host(nttm.mul(dev(a), dev(b)))
+            finally:
+                if env:
+                    del os.environ[env]
+                torch.cuda.synchronize(); nttm.shutdown()
+            if ref_X is None:
+                ref_X, ref_c = X, c
+                assert O.tref_res(a, b, c)
+            assert np.array_equal(X, ref_X) and np.array_equal(c, ref_c), (cfg, env)
+    finally:
+        nttm.init(0)
