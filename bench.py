@@ -128,7 +128,7 @@ def load_traffic(name):
         return None
 
 
-def bench_leaves(nttm, reps):
+def bench_leaves(nttm, reps, each=True):
     import torch
     ops = leaf_workload()
     items = []
@@ -143,6 +143,9 @@ def bench_leaves(nttm, reps):
         nttm.mul_batch(prods)
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
+    if not each:
+        return {"products": len(ops), "batched_products_per_s": round(len(ops) / t_batch, 1),
+                "batched_ms": round(1e3 * t_batch, 3)}
     ref = [it[3].clone() for it in items[:64]]
     t0 = time.perf_counter()
     nttm.mul_each(prods)
@@ -265,6 +268,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-leaves", action="store_true")
     ap.add_argument("--leaf-reps", type=int, default=3)
+    ap.add_argument("--leaves-only", action="store_true", help="only the batched-leaf workloads, one JSON object")
+    ap.add_argument("--batched-only", action="store_true", help="with --leaves-only: the 4096-leaf batch alone")
     ap.add_argument("--tile-log", type=int, default=0, help="strided-pass tile, log2 points (12 or 13; 0 = default)")
     ap.add_argument("--pts", type=int, default=0, help="transform points per thread (8 or 16; 0 = default)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -277,6 +282,14 @@ def main():
 
     import torch
     import ntt_amd as nttm
+
+    if args.leaves_only:                                         # the batched-leaf workloads alone (for profiling)
+        nttm.init(0, tile_log=args.tile_log, pts=args.pts)
+        leaves = bench_leaves(nttm, args.leaf_reps, each=not args.batched_only)
+        if not args.batched_only:
+            leaves["binsplit_level"] = bench_binsplit(nttm, args.leaf_reps)
+        print(json.dumps({"leaves": leaves}))
+        return
 
     n = 1 << args.log_limbs
     print("[bench] generating 2 x 2^%d limbs" % args.log_limbs, file=sys.stderr, flush=True)
