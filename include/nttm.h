@@ -117,6 +117,11 @@ typedef struct nttm_bs_node {
  * stream. */
 int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q, size_t q_cap,
                  size_t *qn, void *stream);
+/* e's fraction to m limbs: out[0 .. m) = floor((e - 2) 10^(18 m)), least significant limb first, HOST memory (the
+ * limbs read from the top are e's decimal digits after "2.", 18 per limb).  The reference's pipeline on one GPU
+ * (E/ALGORITHM.md:12-20,59-94): binary splitting of sum 1/k! with nttm_bs_tree, a Newton reciprocal of Q and a
+ * corrected quotient, every product on the device.  Synchronous. */
+int nttm_e_fraction(uint64_t *out, size_t m, void *stream);
 /* the same N products, one product call each (the unbatched baseline of the batch tier) */
 int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
 

@@ -72,6 +72,7 @@ def lib():
         L.nttm_bs_tree.argtypes = [C.POINTER(BsNode), sz, vp, sz, C.POINTER(C.c_size_t), vp, sz,
                                    C.POINTER(C.c_size_t), vp]
         L.nttm_bs_tree.restype = i32
+        L.nttm_e_fraction.argtypes = [C.POINTER(C.c_uint64), sz, vp]; L.nttm_e_fraction.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
@@ -203,6 +204,19 @@ def bs_tree(leaves, stream=None):
     pn, qn = C.c_size_t(), C.c_size_t()
     _check(lib().nttm_bs_tree(arr, len(leaves), _ptr(P), pcap, C.byref(pn), _ptr(Q), qcap, C.byref(qn), _stream(stream)))
     return P[:pn.value], Q[:qn.value]
+
+
+def e_fraction(m, stream=None):
+    """floor((e - 2) 10^(18 m)) as m limbs (numpy uint64, least significant first)."""
+    import numpy as np
+    out = np.zeros(m, dtype=np.uint64)
+    _check(lib().nttm_e_fraction(out.ctypes.data_as(C.POINTER(C.c_uint64)), m, _stream(stream)))
+    return out
+
+
+def e_digits(m, stream=None):
+    """e's first 18 m decimal digits after the point, as a string."""
+    return "".join("%018d" % int(v) for v in reversed(e_fraction(m, stream)))
 
 
 def make_prods(items):
