@@ -57,6 +57,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libnttm.so is not built (run __graft_entry__.build() or python -m ntt_amd.build)")
+        try:                     # torch's HIP runtime first: the library then binds to that one copy of it
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, sz, u64, i32 = C.c_void_p, C.c_size_t, C.c_uint64, C.c_int
         L.nttm_init.argtypes = [i32, C.POINTER(Config)]; L.nttm_init.restype = i32
