@@ -77,6 +77,7 @@ def lib():
                                    C.POINTER(C.c_size_t), vp]
         L.nttm_bs_tree.restype = i32
         L.nttm_e_fraction.argtypes = [C.POINTER(C.c_uint64), sz, vp]; L.nttm_e_fraction.restype = i32
+        L.nttm_e_last_phase_ms.argtypes = [C.POINTER(C.c_double)] * 4; L.nttm_e_last_phase_ms.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
@@ -218,9 +219,28 @@ def e_fraction(m, stream=None):
     return out
 
 
+def e_last_phase_ms():
+    """wall ms of the last e_fraction's phases: leaf folding, product tree, reciprocal, quotient"""
+    v = [C.c_double() for _ in range(4)]
+    _check(lib().nttm_e_last_phase_ms(*[C.byref(x) for x in v]))
+    return dict(zip(("seed", "tree", "recip", "quot"), (x.value for x in v)))
+
+
+def limbs_to_ascii(limbs):
+    """base-10^18 limbs (least significant first) -> bytes, 18 ASCII digits per limb from the top limb"""
+    import numpy as np
+    d = np.ascontiguousarray(np.asarray(limbs, dtype=np.uint64)[::-1])
+    out = np.empty((len(d), 18), dtype=np.uint8)
+    for part, col in (((d // 10 ** 9).astype(np.uint32), 0), ((d % 10 ** 9).astype(np.uint32), 9)):
+        for i in range(8, -1, -1):
+            out[:, col + i] = (part % 10).astype(np.uint8) + 48
+            part //= 10
+    return out.tobytes()
+
+
 def e_digits(m, stream=None):
     """e's first 18 m decimal digits after the point, as a string."""
-    return "".join("%018d" % int(v) for v in reversed(e_fraction(m, stream)))
+    return limbs_to_ascii(e_fraction(m, stream)).decode()
 
 
 def make_prods(items):

@@ -51,8 +51,6 @@ int nttm_hip_check(hipError_t e, const char *what)
     if (e == hipSuccess) return 0;
     return nttm_fail(NTTM_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
-#define HIPCK(call) do { int rc_ = nttm_hip_check((call), #call); if (rc_) return rc_; } while (0)
-#define RCK(call) do { int rc_ = (call); if (rc_) return rc_; } while (0)
 extern "C" const char *nttm_last_error(void) { return g_err; }
 
 /* ---- host modular arithmetic (exact, __int128) -------------------------------------------------------------------- */
@@ -693,12 +691,13 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
          * length) and the double-fused tiles lie inside one transform */
         std::vector<uint32_t> slot(cnt);
         std::vector<LoadDesc> ldb;
+        bool can;
         {
             Plan pq; plan_make2(&pq, logn, r3 ? 3 * cnt : cnt, product_tile_log(logn), NTTM_FUSED_LOG);
-            const bool can = logn >= NTTM_FUSED_LOG && (pq.npass > 1 || r3);
+            can = logn >= NTTM_FUSED_LOG && (pq.npass > 1 || r3);
             std::map<std::pair<const uint64_t *, uint64_t>, uint32_t> seen;
-            for (uint64_t j = 0; j < cnt; j++) {
-                auto key = std::make_pair(ld[j].b, can ? ld[j].nb : j);       /* no sharing: every key distinct */
+            for (uint64_t j = 0; j < cnt && can; j++) {
+                auto key = std::make_pair(ld[j].b, ld[j].nb);
                 auto it = seen.find(key);
                 if (it == seen.end()) {
                     it = seen.emplace(key, (uint32_t)ldb.size()).first;
@@ -707,7 +706,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
                 slot[j] = it->second;
             }
         }
-        const uint64_t cnt_b = ldb.size(), tb = r3 ? 3 * cnt : cnt;
+        const uint64_t cnt_b = can ? ldb.size() : cnt, tb = r3 ? 3 * cnt : cnt;    /* no sharing: one B per product */
         const bool share = cnt_b < cnt;
         std::vector<uint32_t> bmap(share ? tb : 0);
         for (uint64_t t = 0; share && t < tb; t++) bmap[t] = r3 ? 3 * slot[t / 3] + (uint32_t)(t % 3) : slot[t];
@@ -727,7 +726,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         HIPCK(hipStreamSynchronize(s));                      /* the pinned staging is reused per group */
         memcpy(hp, cd.data(), cnt * sizeof(CrtDesc));
         memcpy(hp + h_ld, ld.data(), cnt * sizeof(LoadDesc));
-        memcpy(hp + h_ldb, ldb.data(), cnt_b * sizeof(LoadDesc));
+        if (share) memcpy(hp + h_ldb, ldb.data(), cnt_b * sizeof(LoadDesc));
         if (share) memcpy(hp + h_bmap, bmap.data(), bmap.size() * 4);
         HIPCK(hipMemcpyAsync(g.ws + off_cd, hp, cnt * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + h_ld, cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
