@@ -1,0 +1,131 @@
+"""The e pipeline's host-side reciprocal and quotient (ntt_amd/csrc/ebig.h), built for the host with a schoolbook
+product (tests/native/ebig_host.cpp), against exact Python integers: the long-division base case, the Newton reciprocal's
+3-unit error bound at every precision, the exact corrected quotient, and the cost shape (products on truncated Q)."""
+import ctypes as C
+import json
+import math
+import os
+import random
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "native", "ebig_host.cpp")
+HDR = os.path.join(HERE, "..", "ntt_amd", "csrc", "ebig.h")
+SO = os.path.join(HERE, "native", "ebig_host.so")
+B = 10 ** 18
+U64P = C.POINTER(C.c_uint64)
+
+
+@pytest.fixture(scope="module")
+def E():
+    if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(SRC), os.path.getmtime(HDR)):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", SO, SRC], check=True)
+    L = C.CDLL(SO)
+    sz = C.c_size_t
+    L.eb_t_recip.argtypes = [U64P, sz, sz, U64P, sz]; L.eb_t_recip.restype = C.c_long
+    L.eb_t_quotient.argtypes = [U64P, sz, U64P, sz, sz, U64P, sz]; L.eb_t_quotient.restype = C.c_long
+    L.eb_t_div_pow.argtypes = [sz, U64P, sz, U64P, sz]; L.eb_t_div_pow.restype = C.c_long
+    L.eb_t_products.restype = C.c_long
+    L.eb_t_max_operand.restype = sz
+    return L
+
+
+def limbs(v):
+    out = []
+    while v:
+        v, r = divmod(v, B)
+        out.append(r)
+    return out
+
+
+def arr(v):
+    l = limbs(v)
+    return (C.c_uint64 * max(len(l), 1))(*l), len(l)
+
+
+def value(buf, n):
+    return sum(int(buf[i]) * B ** i for i in range(n))
+
+
+def recip(E, q, m):
+    qa, n = arr(q)
+    cap = m + 4
+    out = (C.c_uint64 * cap)()
+    k = E.eb_t_recip(qa, n, m, out, cap)
+    assert k >= 0
+    return value(out, k)
+
+
+def quotient(E, a, q, m):
+    aa, na = arr(a)
+    qa, n = arr(q)
+    cap = m + 2
+    out = (C.c_uint64 * cap)()
+    k = E.eb_t_quotient(aa, na, qa, n, m, out, cap)
+    assert k >= 0
+    return value(out, k)
+
+
+def q_kinds(rnd, n):
+    """Q of n limbs: random, all B - 1, a power of B, top limb 1 or B - 1 over random low limbs"""
+    r = rnd.randrange(B ** (n - 1), B ** n)
+    return [r, B ** n - 1, B ** (n - 1), B ** (n - 1) + rnd.randrange(B ** (n - 1)),
+            (B - 1) * B ** (n - 1) + rnd.randrange(B ** (n - 1))]
+
+
+def test_div_pow_base_case(E):
+    rnd = random.Random(3)
+    for _ in range(60):
+        nd = rnd.randint(1, 6)
+        d = rnd.randrange(B ** (nd - 1), B ** nd)
+        e = rnd.randint(nd - 1, nd + 9)
+        da, n = arr(d)
+        out = (C.c_uint64 * (e + 2))()
+        k = E.eb_t_div_pow(e, da, n, out, e + 2)
+        assert value(out, k) == B ** e // d
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (1, 9), (2, 8), (3, 9), (5, 17), (12, 30), (40, 70), (90, 33), (7, 200)])
+def test_recip_within_three_units(E, n, m):
+    rnd = random.Random(n * 1000 + m)
+    for q in q_kinds(rnd, n):
+        x = recip(E, q, m)
+        assert abs(x - B ** (n + m) // q) <= 3, (n, m)
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (2, 3), (4, 9), (10, 10), (25, 60), (60, 25), (3, 120)])
+def test_quotient_exact(E, n, m):
+    rnd = random.Random(7 * n + m)
+    for q in q_kinds(rnd, n):
+        for a in (0, 1, q - 1, rnd.randrange(q), q // 2):
+            assert quotient(E, a, q, m) == a * B ** m // q, (n, m)
+
+
+def test_reciprocal_products_on_truncated_q(E):
+    """Q of 600 limbs, 40 limbs of reciprocal: every Newton product stays near 40 limbs, and their number grows with
+    log m"""
+    rnd = random.Random(11)
+    q = rnd.randrange(B ** 599, B ** 600)
+    E.eb_t_reset()
+    x = recip(E, q, 40)
+    assert abs(x - B ** 640 // q) <= 3
+    assert E.eb_t_max_operand() <= 40 + 4
+    assert E.eb_t_products() <= 2 * math.ceil(math.log2(40))
+
+
+def test_e_fraction_from_series(E):
+    """e's fraction through the host reciprocal and quotient from P, Q of the series; the reference's first window"""
+    m = 8
+    need = 18 * (m + 2) * math.log(10)
+    n = 2
+    while math.lgamma(n + 1) < need:
+        n += 1
+    q = math.factorial(n)
+    p = sum(q // math.factorial(k) for k in range(1, n + 1))
+    y = quotient(E, p - q, q, m)
+    digits = "%0*d" % (18 * m, y)
+    win = json.load(open(os.path.join(HERE, "golden", "e_windows.json")))["windows"]["50"]
+    assert digits[49:99] == win
+    assert digits.startswith("718281828459045235360287471352662497757")

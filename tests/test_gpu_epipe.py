@@ -1,6 +1,8 @@
-"""e's digits through the product library (nttm_e_fraction: binary splitting, Newton reciprocal, corrected quotient,
-every product on the device), pinned by exact Python integers at small sizes and by the reference's own 50-digit
-windows of e (E/verify.c:197-201, tests/golden/e_windows.json) at a million digits."""
+"""e's digits through the product library (nttm_e_fraction: leaves folded on the device, binary splitting, Newton
+reciprocal, corrected quotient, every product on the device), pinned by exact Python integers at small sizes, by the
+reference's own 50-digit windows of e (E/verify.c:197-201) at 10^6 and 10^8 digits, and at 10^9 digits by the
+reference's sha1 of the whole digit output (TESTING.md:31) and its window at 10^9 (tests/golden/e_windows.json)."""
+import hashlib
 import json
 import math
 import os
@@ -12,7 +14,8 @@ from gpu_util import say
 pytestmark = pytest.mark.gpu
 B = 10 ** 18
 HERE = os.path.dirname(os.path.abspath(__file__))
-WIN = json.load(open(os.path.join(HERE, "golden", "e_windows.json")))["windows"]
+GOLD = json.load(open(os.path.join(HERE, "golden", "e_windows.json")))
+WIN = GOLD["windows"]
 
 
 def e_fraction_python(m):
@@ -42,3 +45,26 @@ def test_e_million_digits_windows(nttm):
     for off in ("50", "1000000"):
         o = int(off)
         assert d[o - 1:o + 49] == WIN[off], off
+
+
+def test_e_1e8_digits_windows(nttm):
+    m = 5555556                                                  # 100,000,008 digits
+    say("e 1e8 digits")
+    d = nttm.limbs_to_ascii(nttm.e_fraction(m))
+    for off in ("50", "1000000", "100000000"):
+        o = int(off)
+        assert d[o - 1:o + 49].decode() == WIN[off], off
+
+
+def test_e_1e9_digits_reference_hash(nttm):
+    """the reference's production check at 10^9 digits: sha1 of "2." + digits + newline, and the last window"""
+    m = 55555556                                                 # 1,000,000,008 digits
+    say("e 1e9 digits")
+    d = nttm.limbs_to_ascii(nttm.e_fraction(m))
+    say("e 1e9 phases", nttm.e_last_phase_ms())
+    assert d[999999950:10 ** 9].decode() == WIN["999999951"]
+    h = hashlib.sha1()
+    h.update(b"2.")
+    h.update(memoryview(d)[:10 ** 9])
+    h.update(b"\n")
+    assert h.hexdigest()[:16] == GOLD["sha1_16"]["1000000000"]
