@@ -6,8 +6,9 @@ Default workload: one 2^27 x 2^27-limb product (base-10^18 uint64 limbs drawn by
 product c = a b through the C ABI (load, 9 transforms with the fused pointwise product, CRT with the device carry scan).
 Also measured in the same run: the batched binary-splitting leaves (4096 products, log-uniform lengths in
 [64, 65536] limbs) in one batched call against one call per product; one binary-splitting level (256 nodes, each
-P = P1 Q2 + P2 and Q = Q1 Q2 sharing Q2) batched with Q2 shared and unshared; and, on the host, GMP mpz_mul on a
-bounded sample of the same operands as the CPU baseline.
+P = P1 Q2 + P2 and Q = Q1 Q2 sharing Q2) batched with Q2 shared and unshared; e to 10^9 digits through the whole
+pipeline (checked against the reference's sha1 of its digit output); and, on the host, GMP mpz_mul on a bounded sample
+of the same operands as the CPU baseline.
 
 Prints one JSON line (rank 0).  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -196,6 +197,33 @@ PROF_NAMES = (("k_fused2", "k_fused2"), ("k_pass_stream<0", "k_pass<fwd>"), ("k_
               ("k_crt_fix", "k_crt_fix"))
 
 
+def bench_e(nttm, digits):
+    """e to `digits` fractional digits through nttm_e_fraction (one run, wall clock), checked against the reference's
+    sha1 of its ASCII output and its 50-digit windows (tests/golden/e_windows.json)"""
+    import hashlib
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "e_windows.json")))
+    m = (digits + 17) // 18
+    t0 = time.perf_counter()
+    f = nttm.e_fraction(m)
+    sec = time.perf_counter() - t0
+    phases = nttm.e_last_phase_ms()
+    d = nttm.limbs_to_ascii(f)
+    del f
+    out = {"digits": digits, "limbs": m, "seconds": round(sec, 3), "digits_per_s": round(digits / sec, 1),
+           "phases_ms": {k: round(v, 1) for k, v in phases.items()},
+           "windows_ok": {o: d[int(o) - 1:int(o) + 49].decode() == w
+                          for o, w in gold["windows"].items() if int(o) + 49 <= digits}}
+    want = gold["sha1_16"].get(str(digits))
+    if want:
+        h = hashlib.sha1(b"2.")
+        h.update(memoryview(d)[:digits])
+        h.update(b"\n")
+        out["sha1_16"] = h.hexdigest()[:16]
+        out["sha1_matches_reference"] = out["sha1_16"] == want
+    out["reference"] = "10^9 digits in 7.55-9.18 s on 4 MI300A APUs (results/MEMORY_IO.md:154,163,202)"
+    return out
+
+
 def trace_summary(db):
     """per-kernel calls / total / average / min / max duration (us) of a rocprofv3 --kernel-trace results database"""
     import sqlite3
@@ -268,6 +296,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-leaves", action="store_true")
     ap.add_argument("--leaf-reps", type=int, default=3)
+    ap.add_argument("--e-digits", type=int, default=10 ** 9, help="digits of e for the pipeline line (0: skip)")
     ap.add_argument("--leaves-only", action="store_true", help="only the batched-leaf workloads, one JSON object")
     ap.add_argument("--batched-only", action="store_true", help="with --leaves-only: the 4096-leaf batch alone")
     ap.add_argument("--tile-log", type=int, default=0, help="strided-pass tile, log2 points (12 or 13; 0 = default)")
@@ -347,6 +376,12 @@ def main():
         print("[bench] binary-splitting level", file=sys.stderr, flush=True)
         leaves["binsplit_level"] = bench_binsplit(nttm, args.leaf_reps)
 
+    epipe = None
+    if args.e_digits:
+        print("[bench] e to %d digits" % args.e_digits, file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        epipe = bench_e(nttm, args.e_digits)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (GMP)", file=sys.stderr, flush=True)
@@ -369,7 +404,7 @@ def main():
                                     "crt": round(phase[2], 3)},
                        "kernels_per_step": kernels, "deterministic": deterministic,
                        "vs_baseline_ref": "time ratio to 0.1263 s, 1 MI300A (results/D13b_strategy_e0.txt:26)",
-                       "leaves": leaves},
+                       "leaves": leaves, "e_pipeline": epipe},
             "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     nttm.shutdown()

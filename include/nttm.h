@@ -121,7 +121,7 @@ int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap
  * limbs read from the top are e's decimal digits after "2.", 18 per limb).  The reference's pipeline on one GPU
  * (E/ALGORITHM.md:12-20,59-94): binary splitting of sum 1/k! (leaves of up to 64 terms folded on the device, then
  * nttm_bs_tree), a Newton reciprocal of Q with doubling precision and a corrected quotient, every product on the
- * device.  N terms with N! > 10^(18 (m + 2)) must stay below 10^9 (m up to about 4.9 10^8 limbs).  Synchronous. */
+ * device.  N terms with N! > 10^(18 (m + 2)) must stay below 10^9 - 2 (m up to about 4.7 10^8 limbs, 8.5 10^9 digits).  Synchronous. */
 int nttm_e_fraction(uint64_t *out, size_t m, void *stream);
 /* wall time of the phases of the last nttm_e_fraction (ms): leaf folding, the product tree (with the copies of P and
  * Q to the host), the reciprocal, the quotient (the reference's run prints its phase split, E/ecalc.c:237) */
