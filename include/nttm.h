@@ -43,7 +43,8 @@ enum {
     NTTM_EHIP = -3,        /* a HIP runtime call failed */
     NTTM_EBOUND = -4,      /* the product exceeds what the prime set reconstructs, or the length exceeds 2^32 */
     NTTM_ESTATE = -5,      /* not initialised */
-    NTTM_ECHECK = -6       /* a device-side post-condition failed (non-zero carry beyond the top limb) */
+    NTTM_ECHECK = -6,      /* a device-side post-condition failed (non-zero carry beyond the top limb) */
+    NTTM_EIO = -7          /* a file could not be written */
 };
 
 #define NTTM_MAX_PRIMES 4
@@ -126,6 +127,22 @@ int nttm_e_fraction(uint64_t *out, size_t m, void *stream);
 /* wall time of the phases of the last nttm_e_fraction (ms): leaf folding, the product tree (with the copies of P and
  * Q to the host), the reciprocal, the quotient (the reference's run prints its phase split, E/ecalc.c:237) */
 int nttm_e_last_phase_ms(double *seed_ms, double *tree_ms, double *recip_ms, double *quot_ms);
+
+/* ---- packed digit output (the reference's ECPACK18 version-2 file, E/packed_fmt.h:1-20,50-55) ---- */
+/* bytes of n limbs in the 60-bit form: 15 (n / 2) + 8 (n & 1)  (E/packed_fmt.h:56 ecp60_bytes) */
+uint64_t nttm_pack60_bytes(size_t n);
+/* the limbs [lo, hi) of x (device) in file order, most significant first, as the 60-bit little-endian bit stream
+ * (E/packed_fmt.h:58-78 ecp60_put_pair / ecp60_pack); out: device, 4-byte aligned, nttm_pack60_bytes(hi - lo) bytes.
+ * Asynchronous on the stream. */
+int nttm_pack60(uint8_t *out, const uint64_t *x, size_t lo, size_t hi, void *stream);
+/* res[j] = (sum_k x[k] 10^(18 k)) mod q[j] for nq <= 8 odd moduli in (10^18, 2^63); x on the device, res on the host
+ * (E/verify.c:30 vf_limbs_mod, the residues in the packed header).  Synchronous. */
+int nttm_limbs_mod(uint64_t *res, const uint64_t *x, size_t n, const uint64_t *q, int nq, void *stream);
+/* writes the single-part packed file of e's digits (E/mn_out.c:299-313 and E/packed_fmt.h:32-48 for the header):
+ * X = 2 10^(18 m) + frac (frac: nttm_e_fraction's m limbs, host memory), nl = m + 1 limbs, pad 17, d = 18 m digits,
+ * d_out of them in the ASCII form ("2." + digits 1..d_out + "\n"), the residues of X mod the eight primes above 2^62
+ * (E/verify.c:18).  Synchronous. */
+int nttm_e_write_packed(const char *path, const uint64_t *frac, size_t m, uint64_t d_out, void *stream);
 /* the same N products, one product call each (the unbatched baseline of the batch tier) */
 int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
 

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "nttm.h")
 
 B = 10 ** 18
 Y_FULL, Y_BCAST, Y_PAIR = 0, 1, 2
-ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EBOUND", -5: "ESTATE", -6: "ECHECK"}
+ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "EBOUND", -5: "ESTATE", -6: "ECHECK", -7: "EIO"}
 
 
 class NttmError(RuntimeError):
@@ -78,6 +78,12 @@ def lib():
         L.nttm_bs_tree.restype = i32
         L.nttm_e_fraction.argtypes = [C.POINTER(C.c_uint64), sz, vp]; L.nttm_e_fraction.restype = i32
         L.nttm_e_last_phase_ms.argtypes = [C.POINTER(C.c_double)] * 4; L.nttm_e_last_phase_ms.restype = i32
+        L.nttm_pack60_bytes.argtypes = [sz]; L.nttm_pack60_bytes.restype = u64
+        L.nttm_pack60.argtypes = [vp, vp, sz, sz, vp]; L.nttm_pack60.restype = i32
+        L.nttm_limbs_mod.argtypes = [C.POINTER(C.c_uint64), vp, sz, C.POINTER(C.c_uint64), i32, vp]
+        L.nttm_limbs_mod.restype = i32
+        L.nttm_e_write_packed.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), sz, u64, vp]
+        L.nttm_e_write_packed.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
@@ -236,6 +242,39 @@ def limbs_to_ascii(limbs):
             out[:, col + i] = (part % 10).astype(np.uint8) + 48
             part //= 10
     return out.tobytes()
+
+
+T1_PRIMES = (4611686018427388039, 4611686018427388073, 4611686018427388081, 4611686018427388091,
+             4611686018427388093, 4611686018427388097, 4611686018427388157, 4611686018427388181)
+
+
+def pack60(x, lo=0, hi=None, stream=None):
+    """limbs [lo, hi) of the int64 device tensor x, most significant first, in the 60-bit packed form: a uint8 device
+    tensor of pack60_bytes(hi - lo) bytes"""
+    import torch
+    hi = x.numel() if hi is None else hi
+    nb = int(lib().nttm_pack60_bytes(hi - lo))
+    out = torch.empty((nb + 3) // 4 * 4 or 4, dtype=torch.uint8, device=x.device)
+    _check(lib().nttm_pack60(_ptr(out), _ptr(x), lo, hi, _stream(stream)))
+    return out[:nb]
+
+
+def limbs_mod(x, qs=T1_PRIMES, stream=None):
+    """[sum_k x[k] 10^(18 k) mod q for q in qs] for an int64 device tensor x of limbs"""
+    import numpy as np
+    q = np.ascontiguousarray(qs, dtype=np.uint64)
+    res = np.zeros(len(q), dtype=np.uint64)
+    _check(lib().nttm_limbs_mod(res.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(x) if x.numel() else None, x.numel(),
+                                q.ctypes.data_as(C.POINTER(C.c_uint64)), len(q), _stream(stream)))
+    return [int(v) for v in res]
+
+
+def e_write_packed(path, frac, d_out, stream=None):
+    """the packed digit file of e from e_fraction's limbs (numpy uint64), d_out digits in its text form"""
+    import numpy as np
+    f = np.ascontiguousarray(frac, dtype=np.uint64)
+    _check(lib().nttm_e_write_packed(os.fsencode(path), f.ctypes.data_as(C.POINTER(C.c_uint64)), len(f), d_out,
+                                     _stream(stream)))
 
 
 def e_digits(m, stream=None):

@@ -76,6 +76,9 @@ def lib():
         L.orc_gmp_tref_edges.restype = C.c_int
         L.orc_gmp_check.argtypes = [_u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t]
         L.orc_gmp_check.restype = C.c_int
+        L.orc_pack60_bytes.argtypes = [C.c_size_t]; L.orc_pack60_bytes.restype = C.c_uint64
+        L.orc_pack60.argtypes = [_u64p, C.c_size_t, C.c_void_p]; L.orc_pack60.restype = None
+        L.orc_unpack60.argtypes = [C.c_void_p, C.c_size_t, _u64p]; L.orc_unpack60.restype = None
         _lib = L
     return _lib
 
@@ -206,6 +209,60 @@ def limbs_mod(a, q):
 def max_terms(primes):
     P = np.ascontiguousarray(primes, dtype=np.uint64)
     return int(lib().orc_max_terms(_p(P), len(P)))
+
+
+# ---- the packed digit file (E/packed_fmt.h) --------------------------------------------------------------------------
+# the T1 primes of the header residues: the first eight primes above 2^62 (E/verify.c:15-20)
+T1_PRIMES = (4611686018427388039, 4611686018427388073, 4611686018427388081, 4611686018427388091,
+             4611686018427388093, 4611686018427388097, 4611686018427388157, 4611686018427388181)
+# the header (E/packed_fmt.h:32-48): magic, 9 x u32, 4 bytes of alignment, 10 x u64, q[8], dres[8]
+_HDR_U32 = ("version", "hdr_bytes", "endian", "limb_bytes", "limb_digits", "order", "part", "nparts", "reserved0")
+_HDR_U64 = ("d", "d_out", "nl", "pad", "lo", "hi", "k0", "k1", "complete", "nq")
+
+
+def pack60(limbs_file_order):
+    """the 60-bit stream of limbs given most significant first (E/packed_fmt.h:50-78), as bytes"""
+    l = np.ascontiguousarray(limbs_file_order, dtype=np.uint64)
+    out = np.zeros(max(int(lib().orc_pack60_bytes(len(l))), 1), dtype=np.uint8)
+    lib().orc_pack60(_p(l), len(l), out.ctypes.data)
+    return out[:int(lib().orc_pack60_bytes(len(l)))].tobytes()
+
+
+def unpack60(buf, n):
+    """n limbs (file order) from a 60-bit stream"""
+    b = np.frombuffer(buf, dtype=np.uint8)
+    if len(b) < int(lib().orc_pack60_bytes(n)):
+        raise ValueError("stream shorter than %d limbs" % n)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    lib().orc_unpack60(b.ctypes.data, n, _p(out))
+    return out[:n]
+
+
+def read_packed(path):
+    """(header dict, limbs in file order) of a single-part version-2 packed file"""
+    import struct
+    with open(path, "rb") as f:
+        raw = f.read()
+    h = {"magic": raw[:8]}
+    h.update(zip(_HDR_U32, struct.unpack_from("<9I", raw, 8)))
+    h.update(zip(_HDR_U64, struct.unpack_from("<10Q", raw, 48)))
+    h["q"] = list(struct.unpack_from("<8Q", raw, 128))
+    h["dres"] = list(struct.unpack_from("<8Q", raw, 192))
+    h["text"] = raw[1024:h["hdr_bytes"]].split(b"\0", 1)[0].decode(errors="replace")
+    n = h["hi"] - h["lo"]
+    return h, unpack60(raw[h["hdr_bytes"]:], n)
+
+
+def packed_ascii(h, limbs_file_order):
+    """the text form of a whole packed number: "2." + digits 1..d_out + "\\n" (E/packed_fmt.h:6-14)"""
+    d = np.asarray(limbs_file_order, dtype=np.uint64)
+    out = np.empty((len(d), 18), dtype=np.uint8)
+    for part, col in (((d // 10 ** 9).astype(np.uint32), 0), ((d % 10 ** 9).astype(np.uint32), 9)):
+        for i in range(8, -1, -1):
+            out[:, col + i] = (part % 10).astype(np.uint8) + 48
+            part //= 10
+    s = out.tobytes()[h["pad"]:]                                  # digit k at index k
+    return s[:1] + b"." + s[1:h["d_out"] + 1] + b"\n"
 
 
 def have_gmp():

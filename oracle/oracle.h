@@ -89,6 +89,13 @@ uint64_t orc_limbs_mod(const uint64_t *a, size_t n, uint64_t q);
  * returns 1 if c passes.  Parts (3)/(4) need GMP: orc_gmp_tref_edges. */
 int orc_tref_res(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc);
 
+/* ---- packed digit output (E/packed_fmt.h:50-88): l in file order (most significant limb first) ---- */
+uint64_t orc_pack60_bytes(size_t n);
+/* the 60-bit little-endian bit stream of n limbs, written bit by bit */
+void orc_pack60(const uint64_t *l, size_t n, uint8_t *out);
+/* n limbs read back from the stream, two per 15 bytes */
+void orc_unpack60(const uint8_t *in, size_t n, uint64_t *l);
+
 /* ---- GMP bridge (orc_gmp.c) ---- */
 int orc_have_gmp(void);
 /* c = a b exactly via mpz_mul (conversions by product tree); c has room for na+nb limbs.  Returns 0 or -1 (no GMP) */

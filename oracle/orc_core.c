@@ -417,3 +417,34 @@ int orc_tref_res(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, con
     }
     return 1;
 }
+
+/* ---- the 60-bit packed digit stream (E/packed_fmt.h:50-88) ---------------------------------------------------------
+ * orc_pack60 writes the definition bit by bit: file limb j occupies bits [60 j, 60 j + 60) of a little-endian stream,
+ * padded with zero bits to 15 (n / 2) + 8 (n & 1) bytes.  orc_unpack60 reads it back pairwise: 15 bytes hold two limbs
+ * (E/packed_fmt.h:64-70), a last odd limb 8 bytes. */
+uint64_t orc_pack60_bytes(size_t n) { return 15 * (uint64_t)(n / 2) + 8 * (uint64_t)(n & 1); }
+void orc_pack60(const uint64_t *l, size_t n, uint8_t *out)
+{
+    memset(out, 0, orc_pack60_bytes(n));
+    for (size_t j = 0; j < n; j++)
+        for (int b = 0; b < 60; b++)
+            if ((l[j] >> b) & 1) { const size_t bit = 60 * j + (size_t)b; out[bit >> 3] |= (uint8_t)(1u << (bit & 7)); }
+}
+void orc_unpack60(const uint8_t *in, size_t n, uint64_t *l)
+{
+    const uint64_t m60 = ((uint64_t)1 << 60) - 1;
+    for (size_t i = 0; 2 * i + 1 < n; i++) {
+        const uint8_t *p = in + 15 * i;
+        uint64_t w0 = 0, w1 = 0;
+        for (int k = 0; k < 8; k++) w0 |= (uint64_t)p[k] << (8 * k);
+        for (int k = 0; k < 7; k++) w1 |= (uint64_t)p[8 + k] << (8 * k);
+        l[2 * i] = w0 & m60;
+        l[2 * i + 1] = (w0 >> 60) | (w1 << 4);
+    }
+    if (n & 1) {
+        const uint8_t *p = in + 15 * (n / 2);
+        uint64_t a = 0;
+        for (int k = 0; k < 8; k++) a |= (uint64_t)p[k] << (8 * k);
+        l[n - 1] = a;
+    }
+}

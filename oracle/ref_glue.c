@@ -6,6 +6,8 @@
 #include "tref.h"
 #include "modarith.h"
 #include "crt.h"
+#include "packed_fmt.h"
+#include "verify.h"
 
 void ref_gen_limbs(uint64_t *a, size_t n, int kind, uint64_t *seed)
 {
@@ -35,3 +37,7 @@ void ref_tables(uint64_t *gen, uint64_t *w33, uint64_t *w3x33, uint64_t *mu115)
 }
 uint64_t ref_root(int i, int logn) { return ec_root(i, logn); }
 uint64_t ref_root3(int i, int logk) { return ec_root3(i, logk); }
+/* the packed digit stream (E/packed_fmt.h) and the T1 primes of its header residues (E/verify.c:18) */
+void ref_ecp60_pack(const uint64_t *l, size_t n, uint8_t *out) { ecp60_pack(l, n, out); }
+uint64_t ref_ecp60_bytes(uint64_t n) { return ecp60_bytes(n); }
+void ref_t1_q(uint64_t *q) { for (int i = 0; i < T1_NQ; i++) q[i] = t1_q[i]; }

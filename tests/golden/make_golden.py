@@ -110,6 +110,27 @@ def main():
     a = rng.integers(0, O.B, 100, dtype=np.uint64)
     out["limbs_mod"] = {"a": [int(v) for v in a], "q": t1q, "r": [int(R.ref_limbs_mod_u64(p(a), 100, q)) for q in t1q]}
 
+    # the packed digit stream (E/packed_fmt.h:56-78) and the header residues' primes and residues (E/verify.c:18,30)
+    R.ref_ecp60_pack.argtypes = [u64p, C.c_size_t, C.c_void_p]; R.ref_ecp60_pack.restype = None
+    R.ref_ecp60_bytes.argtypes = [C.c_uint64]; R.ref_ecp60_bytes.restype = C.c_uint64
+    R.ref_t1_q.argtypes = [u64p]; R.ref_t1_q.restype = None
+    R.vf_limbs_mod.argtypes = [u64p, C.c_size_t, C.c_uint64]; R.vf_limbs_mod.restype = C.c_uint64
+    rng2 = np.random.default_rng(0xEC60)
+    packs = []
+    for n in (1, 2, 3, 5, 8, 33):
+        l = rng2.integers(0, O.B, n, dtype=np.uint64)
+        l[0] = O.B - 1
+        if n > 2:
+            l[1] = 0
+        nb = int(R.ref_ecp60_bytes(n))
+        buf = np.zeros(nb + 16, dtype=np.uint8)
+        R.ref_ecp60_pack(p(l), n, buf.ctypes.data)
+        packs.append({"limbs": [int(v) for v in l], "bytes": nb, "hex": buf[:nb].tobytes().hex()})
+    out["pack60"] = packs
+    T1 = np.zeros(8, dtype=np.uint64); R.ref_t1_q(p(T1))
+    a = rng2.integers(0, O.B, 600, dtype=np.uint64)
+    out["t1"] = {"q": [int(v) for v in T1], "a": [int(v) for v in a], "r": [int(R.vf_limbs_mod(p(a), 600, int(q))) for q in T1]}
+
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", OUT)

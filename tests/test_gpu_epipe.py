@@ -48,7 +48,7 @@ def test_e_million_digits_windows(nttm):
 
 
 def test_e_1e8_digits_windows(nttm):
-    m = 5555556                                                  # 100,000,008 digits
+    m = 5555560                                                  # 100,000,080 digits: the 10^8 window fits
     say("e 1e8 digits")
     d = nttm.limbs_to_ascii(nttm.e_fraction(m))
     for off in ("50", "1000000", "100000000"):
@@ -56,11 +56,13 @@ def test_e_1e8_digits_windows(nttm):
         assert d[o - 1:o + 49].decode() == WIN[off], off
 
 
-def test_e_1e9_digits_reference_hash(nttm):
-    """the reference's production check at 10^9 digits: sha1 of "2." + digits + newline, and the last window"""
+def test_e_1e9_digits_reference_hash(nttm, orc, tmp_path):
+    """the reference's production check at 10^9 digits: sha1 of "2." + digits + newline, and the last window; then
+    the same digits through the packed file (nttm_e_write_packed) read back by the oracle's reader"""
     m = 55555556                                                 # 1,000,000,008 digits
     say("e 1e9 digits")
-    d = nttm.limbs_to_ascii(nttm.e_fraction(m))
+    frac = nttm.e_fraction(m)
+    d = nttm.limbs_to_ascii(frac)
     say("e 1e9 phases", nttm.e_last_phase_ms())
     assert d[999999950:10 ** 9].decode() == WIN["999999951"]
     h = hashlib.sha1()
@@ -68,3 +70,13 @@ def test_e_1e9_digits_reference_hash(nttm):
     h.update(memoryview(d)[:10 ** 9])
     h.update(b"\n")
     assert h.hexdigest()[:16] == GOLD["sha1_16"]["1000000000"]
+    del d
+    say("e 1e9 packed file")
+    path = tmp_path / "e1e9.packed"
+    nttm.e_write_packed(str(path), frac, 10 ** 9)
+    del frac
+    hdr, limbs = orc.read_packed(str(path))
+    os.remove(path)
+    txt = orc.packed_ascii(hdr, limbs)
+    assert len(txt) == 10 ** 9 + 3
+    assert hashlib.sha1(txt).hexdigest()[:16] == GOLD["sha1_16"]["1000000000"]
