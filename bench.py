@@ -144,19 +144,38 @@ def bench_leaves(nttm, reps, each=True):
         nttm.mul_batch(prods)
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
+    ref = [it[3].clone() for it in items]
+    # the same batch as an autotuned plan: one graph launch per batch
+    t0 = time.perf_counter()
+    plan = nttm.BatchPlan(prods, autotune=True)
+    torch.cuda.synchronize()
+    t_plan_make = time.perf_counter() - t0
+    plan.run()                                               # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / reps
+    graph_same = all(torch.equal(r, it[3]) for r, it in zip(ref, items))
+    sched = [l for l in plan.info().splitlines() if l.startswith("group")]
+    plan.close()
+    out = {"products": len(ops), "batched_products_per_s": round(len(ops) / t_batch, 1),
+           "batched_ms": round(1e3 * t_batch, 3),
+           "graph_products_per_s": round(len(ops) / t_graph, 1), "graph_ms": round(1e3 * t_graph, 3),
+           "graph_plan_autotune_s": round(t_plan_make, 3), "graph_equals_batch": bool(graph_same),
+           "graph_schedule": sched}
     if not each:
-        return {"products": len(ops), "batched_products_per_s": round(len(ops) / t_batch, 1),
-                "batched_ms": round(1e3 * t_batch, 3)}
-    ref = [it[3].clone() for it in items[:64]]
+        return out
     t0 = time.perf_counter()
     nttm.mul_each(prods)
     torch.cuda.synchronize()
     t_each = time.perf_counter() - t0
-    same = all(torch.equal(r, it[3]) for r, it in zip(ref, items[:64]))
-    return {"products": len(ops), "batched_products_per_s": round(len(ops) / t_batch, 1),
-            "per_launch_products_per_s": round(len(ops) / t_each, 1), "speedup": round(t_each / t_batch, 2),
-            "batched_ms": round(1e3 * t_batch, 3), "per_launch_ms": round(1e3 * t_each, 3),
-            "batch_equals_each_first64": bool(same)}
+    same = all(torch.equal(r, it[3]) for r, it in zip(ref[:64], items[:64]))
+    out.update({"per_launch_products_per_s": round(len(ops) / t_each, 1), "speedup": round(t_each / t_batch, 2),
+                "graph_speedup": round(t_each / t_graph, 2), "per_launch_ms": round(1e3 * t_each, 3),
+                "batch_equals_each_first64": bool(same)})
+    return out
 
 
 def bench_binsplit(nttm, reps, nodes=256, lo=12000, hi=16000, seed=0x5EED0007):

@@ -143,6 +143,24 @@ int nttm_limbs_mod(uint64_t *res, const uint64_t *x, size_t n, const uint64_t *q
  * d_out of them in the ASCII form ("2." + digits 1..d_out + "\n"), the residues of X mod the eight primes above 2^62
  * (E/verify.c:18).  Synchronous. */
 int nttm_e_write_packed(const char *path, const uint64_t *frac, size_t m, uint64_t d_out, void *stream);
+/* a batch fixed once and run many times (the binary-splitting leaves' repeated rns_mul_batch calls, E/rns_mul.c:792):
+ * the products are grouped by transform length; with autotune != 0 each group's schedule is chosen by timing the
+ * candidates on these operands -- per length class, 3 2^k points for the products that fit them or 2^(k+2); per group,
+ * a strided-pass tile of 2^12 or 2^13 points and 8 or 16 points per thread -- else the library's defaults are taken.
+ * Every descriptor is uploaded once and all launches are captured into one hipGraph.  P is HOST memory holding device
+ * pointers; the operands and outputs it names must stay allocated until the plan is destroyed (their contents may
+ * change between runs).  Creating an autotuned plan runs the products several times.  Destroy plans before
+ * nttm_shutdown.  No counterpart in the reference (it plans per call). */
+typedef struct nttm_batch_plan nttm_batch_plan;
+int nttm_batch_plan_create(const nttm_prod *P, size_t N, int autotune, nttm_batch_plan **plan, void *stream);
+/* all N products, one graph launch on the stream (asynchronous) */
+int nttm_batch_plan_run(nttm_batch_plan *plan, void *stream);
+/* after a run: the normalized lengths (ncn[i] for product i; HOST memory, N entries); synchronizes the stream and
+ * checks the CRT's carry flags */
+int nttm_batch_plan_ncn(nttm_batch_plan *plan, size_t *ncn, void *stream);
+/* the chosen schedule as text, one line per group */
+int nttm_batch_plan_info(const nttm_batch_plan *plan, char *buf, size_t cap);
+void nttm_batch_plan_destroy(nttm_batch_plan *plan);
 /* the same N products, one product call each (the unbatched baseline of the batch tier) */
 int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
 

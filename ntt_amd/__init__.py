@@ -85,6 +85,12 @@ def lib():
         L.nttm_e_write_packed.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), sz, u64, vp]
         L.nttm_e_write_packed.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
+        L.nttm_batch_plan_create.argtypes = [C.POINTER(Prod), sz, i32, C.POINTER(vp), vp]
+        L.nttm_batch_plan_create.restype = i32
+        L.nttm_batch_plan_run.argtypes = [vp, vp]; L.nttm_batch_plan_run.restype = i32
+        L.nttm_batch_plan_ncn.argtypes = [vp, C.POINTER(C.c_size_t), vp]; L.nttm_batch_plan_ncn.restype = i32
+        L.nttm_batch_plan_info.argtypes = [vp, C.c_char_p, sz]; L.nttm_batch_plan_info.restype = i32
+        L.nttm_batch_plan_destroy.argtypes = [vp]; L.nttm_batch_plan_destroy.restype = None
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
         L.nttm_pw.argtypes = [i32, vp, vp, sz, vp]; L.nttm_pw.restype = i32
@@ -300,6 +306,42 @@ def mul_batch(prods, stream=None):
 def mul_each(prods, stream=None):
     _check(lib().nttm_mul_each(prods, len(prods), _stream(stream)))
     return prods
+
+
+class BatchPlan:
+    """a batch of products (a ctypes Prod array, see make_prods) planned once -- schedule autotuned, descriptors
+    uploaded, launches captured as one hipGraph -- and run many times"""
+
+    def __init__(self, prods, autotune=True, stream=None):
+        self.prods = prods                                       # keeps the descriptors' tensors' owner alive
+        self.n = len(prods)
+        h = C.c_void_p()
+        _check(lib().nttm_batch_plan_create(prods, self.n, 1 if autotune else 0, C.byref(h), _stream(stream)))
+        self.h = h
+
+    def run(self, stream=None):
+        _check(lib().nttm_batch_plan_run(self.h, _stream(stream)))
+
+    def ncn(self, stream=None):
+        out = (C.c_size_t * max(self.n, 1))()
+        _check(lib().nttm_batch_plan_ncn(self.h, out, _stream(stream)))
+        return [int(out[i]) for i in range(self.n)]
+
+    def info(self):
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().nttm_batch_plan_info(self.h, buf, len(buf)))
+        return buf.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().nttm_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ntt_fwd(prime_idx, x, logn, batch=1, stream=None):

@@ -8,6 +8,7 @@
 #include <vector>
 #include <map>
 #include <algorithm>
+#include <chrono>
 #include "internal.h"
 #include "plan.h"
 
@@ -15,6 +16,7 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
 int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s);
 int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hipStream_t s);
 void nttm_set_pts(int pts);
+int nttm_get_pts(void);
 void nttm_set_fixed_shapes(int on);
 void nttm_set_stream(int on, int ncu);
 #define NTTM_FUSED_LOG 12          /* tile of the fused forward + pointwise + inverse pass (k_fused) */
@@ -34,6 +36,7 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
 uint64_t nttm_crt_blocks(uint64_t nc);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
+int nttm_run_fill_x(const void *desc_dev, uint64_t nprod, hipStream_t s);
 int nttm_run_add_carry(uint64_t *c, uint64_t n, const uint64_t *carry, hipStream_t s);
 int nttm_run_r3(bool inv, double *x, uint64_t batch, int logk, const PrimeK &pk, const double2 *b3,
                 const uint64_t *limbs, uint64_t nlimbs, const LoadDesc *ldesc, int which, const double2 *scale,
@@ -642,6 +645,98 @@ extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
     return 0;
 }
 
+/* one length group of a batch: its products' descriptors and their layout in device memory (offsets from the group's
+ * descriptor base), for a given schedule (length, strided-pass tile, points per thread) */
+struct GroupBuild {
+    int r3 = 0, logn = 0, tile = 0, pts = 0;
+    uint64_t n = 0, cnt = 0, cnt_b = 0, nblk = 0;
+    bool share = false;
+    std::vector<CrtDesc> cd;
+    std::vector<LoadDesc> ld, ldb;
+    std::vector<uint32_t> bmap;
+    size_t planes_bytes = 0;                                     /* np (cnt + cnt_b) n doubles */
+    size_t off_cd = 0, off_ld = 0, off_ldb = 0, off_bmap = 0, off_ncn = 0, off_crt = 0, desc_bytes = 0;
+    double crt_bytes = 0;
+};
+static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<size_t> &G, int np, int r3, int logn,
+                        int tile, int pts)
+{
+    gb.r3 = r3; gb.logn = logn; gb.tile = tile; gb.pts = pts;
+    const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn, cnt = G.size();          /* points per product plane */
+    gb.n = n; gb.cnt = cnt; gb.nblk = 0; gb.crt_bytes = 0;
+    gb.cd.assign(cnt, CrtDesc()); gb.ld.assign(cnt, LoadDesc()); gb.ldb.clear(); gb.bmap.clear();
+    for (uint64_t j = 0; j < cnt; j++) {
+        const nttm_prod &q = P[G[j]];
+        LoadDesc &l = gb.ld[j];
+        CrtDesc &c = gb.cd[j];
+        l.a = q.a; l.na = q.na; l.b = q.b; l.nb = q.nb;
+        c.pl = nullptr; c.plane = cnt * n; c.ncoef = q.na + q.nb - 1; c.x = q.x; c.nx = q.x ? q.nx : 0;
+        c.c = q.c; c.nc = q.na + q.nb + (q.x ? 1 : 0); c.first_block = gb.nblk; c.trunc = 0;
+        gb.nblk += nttm_crt_blocks(c.nc);
+        gb.crt_bytes += 8.0 * np * c.ncoef + 8.0 * c.nc + 8.0 * c.nx;
+    }
+    /* products that share their B operand (the same b and nb: the binary-splitting tree's P1 Q2 + P2 and Q1 Q2,
+     * E/rns_mul.c:476-495) transform it once, where B has planes (a transform of more than one pass, or a radix-3
+     * length) and the double-fused tiles lie inside one transform */
+    std::vector<uint32_t> slot(cnt);
+    Plan pq; plan_make2(&pq, logn, r3 ? 3 * cnt : cnt, product_tile_log(logn, tile), NTTM_FUSED_LOG);
+    const bool can = logn >= NTTM_FUSED_LOG && (pq.npass > 1 || r3);
+    std::map<std::pair<const uint64_t *, uint64_t>, uint32_t> seen;
+    for (uint64_t j = 0; j < cnt && can; j++) {
+        auto key = std::make_pair(gb.ld[j].b, gb.ld[j].nb);
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            it = seen.emplace(key, (uint32_t)gb.ldb.size()).first;
+            gb.ldb.push_back(gb.ld[j]);
+        }
+        slot[j] = it->second;
+    }
+    gb.cnt_b = can ? gb.ldb.size() : cnt;                       /* no sharing: one B per product */
+    gb.share = gb.cnt_b < cnt;
+    if (!gb.share) gb.ldb.clear();
+    const uint64_t tb = r3 ? 3 * cnt : cnt;
+    if (gb.share) {
+        gb.bmap.resize(tb);
+        for (uint64_t t = 0; t < tb; t++) gb.bmap[t] = r3 ? 3 * slot[t / 3] + (uint32_t)(t % 3) : slot[t];
+    }
+    gb.planes_bytes = (size_t)np * (cnt + gb.cnt_b) * n * 8;
+    gb.off_cd = 0;
+    gb.off_ld = align_up(gb.off_cd + cnt * sizeof(CrtDesc), 256);
+    gb.off_ldb = align_up(gb.off_ld + cnt * sizeof(LoadDesc), 256);
+    gb.off_bmap = align_up(gb.off_ldb + gb.ldb.size() * sizeof(LoadDesc), 256);
+    gb.off_ncn = align_up(gb.off_bmap + gb.bmap.size() * 4, 256);
+    gb.off_crt = align_up(gb.off_ncn + cnt * 8, 256);
+    gb.desc_bytes = align_up(gb.off_crt + nttm_crt_scratch_words(gb.nblk) * 4, 256);
+}
+/* the group's descriptors as one host image (planes at A), to copy to the group's descriptor base */
+static void group_image(GroupBuild &gb, double *A, char *img)
+{
+    for (uint64_t j = 0; j < gb.cnt; j++) gb.cd[j].pl = A + j * gb.n;
+    memset(img, 0, gb.desc_bytes);
+    memcpy(img + gb.off_cd, gb.cd.data(), gb.cnt * sizeof(CrtDesc));
+    memcpy(img + gb.off_ld, gb.ld.data(), gb.cnt * sizeof(LoadDesc));
+    if (gb.share) {
+        memcpy(img + gb.off_ldb, gb.ldb.data(), gb.ldb.size() * sizeof(LoadDesc));
+        memcpy(img + gb.off_bmap, gb.bmap.data(), gb.bmap.size() * 4);
+    }
+}
+/* the group's launches: transforms, CRT with the carry scan, normalized lengths; under the group's tile and points
+ * per thread (the library's settings are restored afterwards) */
+static int group_launch(const GroupBuild &gb, int np, double *A, char *dbase, hipStream_t s)
+{
+    const int tile0 = g.cfg.tile_log, pts0 = nttm_get_pts();
+    g.cfg.tile_log = gb.tile; nttm_set_pts(gb.pts);
+    double *Bp = A + (size_t)np * gb.cnt * gb.n;
+    int rc = product_transforms(np, gb.logn, gb.r3, gb.cnt, A, Bp, nullptr, 0, nullptr, 0, (const LoadDesc *)(dbase + gb.off_ld),
+                                s, nullptr, gb.cnt_b, gb.share ? (const LoadDesc *)(dbase + gb.off_ldb) : nullptr,
+                                gb.share ? (const uint32_t *)(dbase + gb.off_bmap) : nullptr);
+    if (!rc) rc = nttm_crt_launch_batch(dbase + gb.off_cd, gb.cnt, gb.nblk, np, g.cc, (uint32_t *)(dbase + gb.off_crt), s,
+                                        gb.crt_bytes);
+    if (!rc) rc = nttm_run_norm(dbase + gb.off_cd, gb.cnt, (uint64_t *)(dbase + gb.off_ncn), s);
+    g.cfg.tile_log = tile0; nttm_set_pts(pts0);
+    return rc;
+}
+
 extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
 {
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
@@ -675,78 +770,22 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         const std::vector<size_t> &G = groups[gi];
         if (G.empty()) continue;
         const int r3 = gi > NTTM_LOGN_MAX, logn = (int)(gi % (NTTM_LOGN_MAX + 1));
-        const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn, cnt = G.size();      /* points per product plane */
-        uint64_t nblk = 0;
-        std::vector<CrtDesc> cd(cnt);
-        std::vector<LoadDesc> ld(cnt);
-        for (uint64_t j = 0; j < cnt; j++) {
-            const nttm_prod &q = P[G[j]];
-            ld[j].a = q.a; ld[j].na = q.na; ld[j].b = q.b; ld[j].nb = q.nb;
-            cd[j].plane = cnt * n; cd[j].ncoef = q.na + q.nb - 1; cd[j].x = q.x; cd[j].nx = q.x ? q.nx : 0;
-            cd[j].c = q.c; cd[j].nc = q.na + q.nb + (q.x ? 1 : 0); cd[j].first_block = nblk;
-            nblk += nttm_crt_blocks(cd[j].nc);
-        }
-        /* products that share their B operand (the same b and nb: the binary-splitting tree's P1 Q2 + P2 and Q1 Q2,
-         * E/rns_mul.c:476-495) transform it once, where B has planes (a transform of more than one pass, or a radix-3
-         * length) and the double-fused tiles lie inside one transform */
-        std::vector<uint32_t> slot(cnt);
-        std::vector<LoadDesc> ldb;
-        bool can;
-        {
-            Plan pq; plan_make2(&pq, logn, r3 ? 3 * cnt : cnt, product_tile_log(logn), NTTM_FUSED_LOG);
-            can = logn >= NTTM_FUSED_LOG && (pq.npass > 1 || r3);
-            std::map<std::pair<const uint64_t *, uint64_t>, uint32_t> seen;
-            for (uint64_t j = 0; j < cnt && can; j++) {
-                auto key = std::make_pair(ld[j].b, ld[j].nb);
-                auto it = seen.find(key);
-                if (it == seen.end()) {
-                    it = seen.emplace(key, (uint32_t)ldb.size()).first;
-                    ldb.push_back(ld[j]);
-                }
-                slot[j] = it->second;
-            }
-        }
-        const uint64_t cnt_b = can ? ldb.size() : cnt, tb = r3 ? 3 * cnt : cnt;    /* no sharing: one B per product */
-        const bool share = cnt_b < cnt;
-        std::vector<uint32_t> bmap(share ? tb : 0);
-        for (uint64_t t = 0; share && t < tb; t++) bmap[t] = r3 ? 3 * slot[t / 3] + (uint32_t)(t % 3) : slot[t];
-        const size_t planes_bytes = (size_t)np * (cnt + cnt_b) * n * 8;
-        const size_t off_cd = align_up(planes_bytes, 256);
-        const size_t off_ld = align_up(off_cd + cnt * sizeof(CrtDesc), 256);
-        const size_t off_ldb = align_up(off_ld + cnt * sizeof(LoadDesc), 256);
-        const size_t off_bmap = align_up(off_ldb + cnt_b * sizeof(LoadDesc), 256);
-        const size_t off_ncn = align_up(off_bmap + bmap.size() * 4, 256);
-        const size_t off_crt = align_up(off_ncn + cnt * 8, 256);
-        RCK(ws_need(off_crt + nttm_crt_scratch_words(nblk) * 4 + 256));
-        const size_t h_ld = cnt * sizeof(CrtDesc), h_ldb = h_ld + cnt * sizeof(LoadDesc), h_bmap = h_ldb + cnt_b * sizeof(LoadDesc);
-        RCK(hpin_need(h_bmap + bmap.size() * 4 + 4096));
-        double *A = (double *)g.ws, *Bp = A + (size_t)np * cnt * n;
-        for (uint64_t j = 0; j < cnt; j++) cd[j].pl = A + j * n;
+        GroupBuild gb;
+        group_build(gb, P, G, np, r3, logn, g.cfg.tile_log, nttm_get_pts());
+        const size_t off_d = align_up(gb.planes_bytes, 256);
+        RCK(ws_need(off_d + gb.desc_bytes + 256));
+        RCK(hpin_need(gb.desc_bytes + 4096));
         char *hp = (char *)g.hpin;
         HIPCK(hipStreamSynchronize(s));                      /* the pinned staging is reused per group */
-        memcpy(hp, cd.data(), cnt * sizeof(CrtDesc));
-        memcpy(hp + h_ld, ld.data(), cnt * sizeof(LoadDesc));
-        if (share) memcpy(hp + h_ldb, ldb.data(), cnt_b * sizeof(LoadDesc));
-        if (share) memcpy(hp + h_bmap, bmap.data(), bmap.size() * 4);
-        HIPCK(hipMemcpyAsync(g.ws + off_cd, hp, cnt * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
-        HIPCK(hipMemcpyAsync(g.ws + off_ld, hp + h_ld, cnt * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
-        if (share) {
-            HIPCK(hipMemcpyAsync(g.ws + off_ldb, hp + h_ldb, cnt_b * sizeof(LoadDesc), hipMemcpyHostToDevice, s));
-            HIPCK(hipMemcpyAsync(g.ws + off_bmap, hp + h_bmap, bmap.size() * 4, hipMemcpyHostToDevice, s));
-        }
-        RCK(product_transforms(np, logn, r3, cnt, A, Bp, nullptr, 0, nullptr, 0, (const LoadDesc *)(g.ws + off_ld), s,
-                               nullptr, cnt_b, share ? (const LoadDesc *)(g.ws + off_ldb) : nullptr,
-                               share ? (const uint32_t *)(g.ws + off_bmap) : nullptr));
-        double crt_bytes = 0;
-        for (uint64_t j = 0; j < cnt; j++) crt_bytes += 8.0 * np * cd[j].ncoef + 8.0 * cd[j].nc + 8.0 * cd[j].nx;
-        RCK(nttm_crt_launch_batch(g.ws + off_cd, cnt, nblk, np, g.cc, (uint32_t *)(g.ws + off_crt), s, crt_bytes));
-        RCK(nttm_run_norm(g.ws + off_cd, cnt, (uint64_t *)(g.ws + off_ncn), s));
-        HIPCK(hipMemcpyAsync(hp, g.ws + off_ncn, cnt * 8, hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(hp + cnt * 8, g.ws + off_crt, 8, hipMemcpyDeviceToHost, s));
+        group_image(gb, (double *)g.ws, hp);
+        HIPCK(hipMemcpyAsync(g.ws + off_d, hp, gb.off_ncn, hipMemcpyHostToDevice, s));   /* descriptors up to ncn */
+        RCK(group_launch(gb, np, (double *)g.ws, g.ws + off_d, s));
+        HIPCK(hipMemcpyAsync(hp, g.ws + off_d + gb.off_ncn, gb.cnt * 8, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(hp + gb.cnt * 8, g.ws + off_d + gb.off_crt, 8, hipMemcpyDeviceToHost, s));
         HIPCK(hipStreamSynchronize(s));
-        const uint32_t flags = ((const uint32_t *)(hp + cnt * 8))[1];
+        const uint32_t flags = ((const uint32_t *)(hp + gb.cnt * 8))[1];
         if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
-        for (uint64_t j = 0; j < cnt; j++) P[G[j]].ncn = ((const uint64_t *)hp)[j];
+        for (uint64_t j = 0; j < gb.cnt; j++) P[G[j]].ncn = ((const uint64_t *)hp)[j];
     }
     /* zero-operand products: normalized length of x */
     for (size_t i = 0; i < N; i++) {
@@ -1021,6 +1060,260 @@ extern "C" int nttm_profile_read(nttm_kstat *out, int cap)
         out[k].launches++; out[k].total_ms += ms; out[k].alg_bytes += r.bytes;
     }
     return n;
+}
+
+/* ---- batch plans: a batch's whole schedule as one hipGraph, each group's schedule chosen by measurement -------------
+ * The binary-splitting leaves of the reference go through rns_mul_batch again and again with the same shapes
+ * (E/rns_mul.c:792-1028).  A plan fixes one batch: it groups the products, chooses per length class whether the
+ * products that fit 3 2^k points take that length or 2^(k+2), and per group the strided-pass tile (2^12 or 2^13
+ * points) and the points per thread (8 or 16) by timing every candidate on the batch's own operands; uploads every
+ * descriptor once; and captures all groups' launches (transforms, CRT, carry scan, lengths) as one graph that a run
+ * launches with one call. */
+struct nttm_batch_plan {
+    size_t N = 0;
+    int np = 3;
+    std::vector<GroupBuild> gb;
+    std::vector<std::vector<size_t>> gidx;
+    std::vector<double> gms;                                     /* each group's measured time (ms; 0 untuned) */
+    std::vector<size_t> goff;                                    /* descriptor bases in blob */
+    std::vector<size_t> zidx;                                    /* products with an empty operand */
+    size_t off_z = 0, off_zncn = 0;
+    char *blob = nullptr;
+    size_t blob_bytes = 0;
+    uint64_t *hstage = nullptr;                                  /* pinned: lengths and flags read back */
+    size_t hstage_words = 0;
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double tune_ms = 0;
+    int candidates = 0;
+};
+
+static void plan_free(nttm_batch_plan *pl)
+{
+    if (!pl) return;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->cs) (void)hipStreamDestroy(pl->cs);
+    if (pl->blob) (void)hipFree(pl->blob);
+    if (pl->hstage) (void)hipHostFree(pl->hstage);
+    delete pl;
+}
+
+/* a device buffer that grows (the autotuner's trial runs) */
+struct TuneBuf {
+    char *p = nullptr; size_t bytes = 0;
+    ~TuneBuf() { release(); }
+    void release() { if (p) { (void)hipDeviceSynchronize(); (void)hipFree(p); } p = nullptr; bytes = 0; }
+    int need(size_t b)
+    {
+        if (b <= bytes) return 0;
+        if (p) { (void)hipDeviceSynchronize(); (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (hipMalloc(&p, b) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "plan: %zu bytes", b); }
+        bytes = b;
+        return 0;
+    }
+};
+/* the fastest (tile, points per thread) for one group of products, timed on the stream; *ms its time */
+static int tune_group(const nttm_prod *P, const std::vector<size_t> &G, int np, int r3, int logn, bool tune, TuneBuf &tb,
+                      std::vector<char> &img, hipStream_t s, GroupBuild &best, double *ms, int *ntried)
+{
+    static const int cand[4][2] = {{12, 16}, {13, 16}, {12, 8}, {13, 8}};
+    *ms = 0;
+    if (!tune) { group_build(best, P, G, np, r3, logn, g.cfg.tile_log, nttm_get_pts()); return 0; }
+    double best_ms = 1e30;
+    hipEvent_t e0 = g.ev[0], e1 = g.ev[1];
+    for (int c = 0; c < 4; c++) {
+        GroupBuild gb;
+        group_build(gb, P, G, np, r3, logn, cand[c][0], cand[c][1]);
+        const size_t off_d = align_up(gb.planes_bytes, 256);
+        RCK(tb.need(off_d + gb.desc_bytes + 256));
+        img.resize(gb.desc_bytes);
+        group_image(gb, (double *)tb.p, img.data());
+        HIPCK(hipMemcpyAsync(tb.p + off_d, img.data(), gb.desc_bytes, hipMemcpyHostToDevice, s));
+        RCK(group_launch(gb, np, (double *)tb.p, tb.p + off_d, s));       /* warm */
+        double t = 1e30;
+        for (int r = 0; r < 2; r++) {
+            HIPCK(hipEventRecord(e0, s));
+            RCK(group_launch(gb, np, (double *)tb.p, tb.p + off_d, s));
+            HIPCK(hipEventRecord(e1, s));
+            HIPCK(hipEventSynchronize(e1));
+            float f = 0; HIPCK(hipEventElapsedTime(&f, e0, e1));
+            t = std::min(t, (double)f);
+        }
+        (*ntried)++;
+        if (t < best_ms) { best_ms = t; best = std::move(gb); }
+    }
+    *ms = best_ms;
+    return 0;
+}
+
+extern "C" int nttm_batch_plan_create(const nttm_prod *P, size_t N, int autotune, nttm_batch_plan **out, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (!out || (N && !P)) return nttm_fail(NTTM_EINVAL, "nttm_batch_plan_create arguments");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    nttm_batch_plan *pl = new nttm_batch_plan();
+    pl->N = N;
+    auto fail = [&](int rc) { plan_free(pl); return rc; };
+    /* validate and class the products by their 2^k length */
+    size_t mn_max = 0;
+    std::vector<std::vector<size_t>> cls(NTTM_LOGN_MAX + 1);
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        if (q.x && q.nx > q.na + q.nb) return fail(nttm_fail(NTTM_EINVAL, "product %zu: nx exceeds na + nb", i));
+        if (!q.c || (q.na && !q.a) || (q.nb && !q.b) || (q.nx && !q.x)) return fail(nttm_fail(NTTM_EINVAL, "product %zu: null pointer", i));
+        if (!q.na || !q.nb) { pl->zidx.push_back(i); continue; }
+        mn_max = std::max(mn_max, (size_t)std::min(q.na, q.nb));
+        int logn, r3;
+        choose_len(q.na + q.nb - 1, NTTM_R3_NEVER, g.cfg.tile_log, &logn, &r3);
+        if (logn > NTTM_LOGN_MAX) return fail(nttm_fail(NTTM_EBOUND, "product %zu: transform above 2^%d", i, NTTM_LOGN_MAX));
+        cls[logn].push_back(i);
+    }
+    int rc = choose_np(mn_max ? mn_max : 1, &pl->np);
+    if (rc) return fail(rc);
+    const int np = pl->np;
+    /* the schedule: per class k the radix-3 subset (products of at most 3 2^(k-2) coefficients) either takes 3 2^(k-2)
+     * points or stays at 2^k; per group the tile and points per thread */
+    TuneBuf tb;
+    std::vector<char> img;
+    const bool tune = autotune != 0;
+    auto t_start = std::chrono::steady_clock::now();
+    for (int k = 0; k <= NTTM_LOGN_MAX; k++) {
+        const std::vector<size_t> &C = cls[k];
+        if (C.empty()) continue;
+        std::vector<size_t> S3, S2;
+        for (size_t i : C) {
+            int logn, r3;
+            choose_len(P[i].na + P[i].nb - 1, tune ? NTTM_R3_FIT : g.cfg.radix3, g.cfg.tile_log, &logn, &r3);
+            (r3 ? S3 : S2).push_back(i);
+        }
+        GroupBuild a, b3, b2;
+        double ta = 0, t3 = 0, t2 = 0;
+        if (tune && !S3.empty()) {                               /* both variants timed */
+            if ((rc = tune_group(P, C, np, 0, k, true, tb, img, s, a, &ta, &pl->candidates))) return fail(rc);
+            if ((rc = tune_group(P, S3, np, 1, k - 2, true, tb, img, s, b3, &t3, &pl->candidates))) return fail(rc);
+            if (!S2.empty() && (rc = tune_group(P, S2, np, 0, k, true, tb, img, s, b2, &t2, &pl->candidates))) return fail(rc);
+            if (ta <= t3 + t2) { pl->gb.push_back(std::move(a)); pl->gidx.push_back(C); pl->gms.push_back(ta); continue; }
+        } else if (S3.empty()) {
+            if ((rc = tune_group(P, C, np, 0, k, tune, tb, img, s, a, &ta, &pl->candidates))) return fail(rc);
+            pl->gb.push_back(std::move(a)); pl->gidx.push_back(C); pl->gms.push_back(ta);
+            continue;
+        } else {                                                 /* untuned: the library's rule */
+            if ((rc = tune_group(P, S3, np, 1, k - 2, false, tb, img, s, b3, &t3, &pl->candidates))) return fail(rc);
+            if (!S2.empty() && (rc = tune_group(P, S2, np, 0, k, false, tb, img, s, b2, &t2, &pl->candidates))) return fail(rc);
+        }
+        pl->gb.push_back(std::move(b3)); pl->gidx.push_back(S3); pl->gms.push_back(t3);
+        if (!S2.empty()) { pl->gb.push_back(std::move(b2)); pl->gidx.push_back(S2); pl->gms.push_back(t2); }
+    }
+    pl->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    /* one device blob: the planes (the largest group's; groups run one after another), then every descriptor */
+    size_t planes = 0;
+    for (const GroupBuild &x : pl->gb) planes = std::max(planes, x.planes_bytes);
+    size_t off = align_up(std::max(planes, (size_t)256), 256);
+    for (const GroupBuild &x : pl->gb) { pl->goff.push_back(off); off = align_up(off + x.desc_bytes, 256); }
+    pl->off_z = off; off = align_up(off + pl->zidx.size() * sizeof(CrtDesc), 256);
+    pl->off_zncn = off; off = align_up(off + pl->zidx.size() * 8, 256);
+    pl->blob_bytes = off;
+    tb.release();
+    if (hipMalloc(&pl->blob, pl->blob_bytes) != hipSuccess) {
+        (void)hipGetLastError(); pl->blob = nullptr;
+        return fail(nttm_fail(NTTM_ENOMEM, "plan: %zu bytes", pl->blob_bytes));
+    }
+    const size_t first = pl->goff.empty() ? pl->off_z : pl->goff[0];         /* the descriptors, after the planes */
+    std::vector<char> himg(pl->blob_bytes - first, 0);
+    double *A = (double *)pl->blob;
+    for (size_t i = 0; i < pl->gb.size(); i++) group_image(pl->gb[i], A, himg.data() + (pl->goff[i] - first));
+    for (size_t z = 0; z < pl->zidx.size(); z++) {
+        const nttm_prod &q = P[pl->zidx[z]];
+        CrtDesc c{}; c.c = q.c; c.nc = q.na + q.nb + (q.x ? 1 : 0); c.x = q.x; c.nx = q.x ? q.nx : 0;
+        memcpy(himg.data() + (pl->off_z - first) + z * sizeof(CrtDesc), &c, sizeof c);
+    }
+    if ((rc = nttm_hip_check(hipMemcpy(pl->blob + first, himg.data(), himg.size(), hipMemcpyHostToDevice),
+                             "plan descriptors"))) return fail(rc);
+    size_t words = pl->zidx.size() + 2 * pl->gb.size();
+    for (const GroupBuild &x : pl->gb) words += x.cnt;
+    pl->hstage_words = words + 1;
+    if (hipHostMalloc((void **)&pl->hstage, pl->hstage_words * 8, 0) != hipSuccess) {
+        (void)hipGetLastError(); pl->hstage = nullptr; return fail(nttm_fail(NTTM_ENOMEM, "plan: pinned staging"));
+    }
+    /* capture every group's launches on a stream of the plan's own (default priority) */
+    if ((rc = nttm_hip_check(hipStreamCreate(&pl->cs), "plan stream"))) return fail(rc);
+    if ((rc = nttm_hip_check(hipStreamSynchronize(s), "plan tuning"))) return fail(rc);
+    const int prof0 = g_prof_on;
+    g_prof_on = 0;                                               /* no event records inside the graph */
+    if ((rc = nttm_hip_check(hipStreamBeginCapture(pl->cs, hipStreamCaptureModeRelaxed), "begin capture"))) { g_prof_on = prof0; return fail(rc); }
+    for (size_t i = 0; i < pl->gb.size() && !rc; i++) rc = group_launch(pl->gb[i], np, A, pl->blob + pl->goff[i], pl->cs);
+    if (!rc) rc = nttm_run_fill_x(pl->blob + pl->off_z, pl->zidx.size(), pl->cs);
+    if (!rc) rc = nttm_run_norm(pl->blob + pl->off_z, pl->zidx.size(), (uint64_t *)(pl->blob + pl->off_zncn), pl->cs);
+    hipGraph_t gr = nullptr;
+    const int rc_end = nttm_hip_check(hipStreamEndCapture(pl->cs, &gr), "end capture");
+    g_prof_on = prof0;
+    pl->graph = gr;
+    if (rc || rc_end) return fail(rc ? rc : rc_end);
+    if ((rc = nttm_hip_check(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0), "graph instantiate"))) return fail(rc);
+    *out = pl;
+    return 0;
+}
+
+extern "C" int nttm_batch_plan_run(nttm_batch_plan *pl, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (!pl) return nttm_fail(NTTM_EINVAL, "null plan");
+    if (!pl->exec) return 0;
+    return nttm_hip_check(hipGraphLaunch(pl->exec, (hipStream_t)stream), "graph launch");
+}
+
+extern "C" int nttm_batch_plan_ncn(nttm_batch_plan *pl, size_t *ncn, void *stream)
+{
+    if (!pl || !ncn) return nttm_fail(NTTM_EINVAL, "nttm_batch_plan_ncn arguments");
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t *h = pl->hstage;
+    size_t w = 0;
+    std::vector<size_t> at(pl->gb.size());
+    for (size_t i = 0; i < pl->gb.size(); i++) {
+        const GroupBuild &x = pl->gb[i];
+        at[i] = w;
+        HIPCK(hipMemcpyAsync(h + w, pl->blob + pl->goff[i] + x.off_ncn, x.cnt * 8, hipMemcpyDeviceToHost, s));
+        w += x.cnt;
+        HIPCK(hipMemcpyAsync(h + w, pl->blob + pl->goff[i] + x.off_crt, 8, hipMemcpyDeviceToHost, s));
+        w += 2;
+    }
+    if (!pl->zidx.empty()) HIPCK(hipMemcpyAsync(h + w, pl->blob + pl->off_zncn, pl->zidx.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (size_t i = 0; i < pl->gb.size(); i++) {
+        const GroupBuild &x = pl->gb[i];
+        const uint32_t flags = ((const uint32_t *)(h + at[i] + x.cnt))[1];
+        if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
+        for (uint64_t j = 0; j < x.cnt; j++) ncn[pl->gidx[i][j]] = h[at[i] + j];
+    }
+    for (size_t z = 0; z < pl->zidx.size(); z++) ncn[pl->zidx[z]] = h[w + z];
+    return 0;
+}
+
+/* the chosen schedule as text: one line per group, then the tuning summary */
+extern "C" int nttm_batch_plan_info(const nttm_batch_plan *pl, char *buf, size_t cap)
+{
+    if (!pl || !buf || !cap) return nttm_fail(NTTM_EINVAL, "nttm_batch_plan_info arguments");
+    size_t k = 0;
+    buf[0] = 0;
+    for (size_t i = 0; i < pl->gb.size() && k < cap; i++) {
+        const GroupBuild &x = pl->gb[i];
+        k += snprintf(buf + k, cap - k, "group points=%llu r3=%d logn=%d products=%llu shared_b=%llu tile=%d pts=%d ms=%.4f\n",
+                      (unsigned long long)x.n, x.r3, x.logn, (unsigned long long)x.cnt,
+                      (unsigned long long)(x.share ? x.cnt - x.cnt_b : 0), x.tile, x.pts, pl->gms[i]);
+    }
+    if (k < cap) k += snprintf(buf + k, cap - k, "plan products=%zu empty=%zu primes=%d candidates=%d tune_ms=%.1f blob_bytes=%zu\n",
+                               pl->N, pl->zidx.size(), pl->np, pl->candidates, pl->tune_ms, pl->blob_bytes);
+    return k < cap ? 0 : nttm_fail(NTTM_EINVAL, "nttm_batch_plan_info: buffer too small");
+}
+
+extern "C" void nttm_batch_plan_destroy(nttm_batch_plan *pl)
+{
+    if (pl && pl->cs) (void)hipStreamSynchronize(pl->cs);
+    if (g.init) (void)hipDeviceSynchronize();
+    plan_free(pl);
 }
 
 /* ---- host-side introspection for tests: the pass plan and the tile -> element map ---------------------------------- */

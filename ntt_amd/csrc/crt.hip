@@ -469,6 +469,20 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
     return nttm_hip_check(hipGetLastError(), "k_crt_fix launch");
 }
 
+/* products with an empty operand: c[0 .. nc) = x zero-extended (a kernel, not a memset, so a captured graph holds it) */
+__global__ __launch_bounds__(256) void k_fill_x(const CrtDesc *d, uint64_t nprod)
+{
+    const CrtDesc q = d[blockIdx.x];
+    for (uint64_t k = threadIdx.x; k < q.nc; k += 256) q.c[k] = k < q.nx ? q.x[k] : 0;
+}
+int nttm_run_fill_x(const void *desc_dev, uint64_t nprod, hipStream_t s)
+{
+    if (!nprod) return 0;
+    if (nprod > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "fill: too many products");
+    hipLaunchKernelGGL(k_fill_x, dim3((unsigned)nprod), dim3(256), 0, s, (const CrtDesc *)desc_dev, nprod);
+    return nttm_hip_check(hipGetLastError(), "k_fill_x launch");
+}
+
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s)
 {
     if (!nprod) return 0;

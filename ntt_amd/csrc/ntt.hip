@@ -921,6 +921,7 @@ static int pass_geom_ok(const PassGeom &g, int logn, uint64_t batch)
 
 static int g_pts = PASS_PTS;                                   /* points per thread of full tiles: 16 or 8 */
 void nttm_set_pts(int pts) { g_pts = pts == 8 ? 8 : PASS_PTS; }
+int nttm_get_pts(void) { return g_pts; }
 static int g_fixed_shapes = 1;                                  /* use the fixed-shape pass kernels where they apply */
 void nttm_set_fixed_shapes(int on) { g_fixed_shapes = on != 0; }
 static int g_stream = 1, g_ncu = 256;                           /* streaming strided passes; CUs of the device */
