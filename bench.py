@@ -259,6 +259,42 @@ def trace_summary(db):
               % (n, r[1], r[2], r[3], r[4], r[5], 100 * r[2] / tot, r[6], r[7], r[8]))
 
 
+def replay_summary(db, replays=3):
+    """the trailing repeated sequence of library kernels (k_*) in a kernel trace -- the last `replays` identical runs of
+    one schedule, e.g. graph replays -- summarized per run: kernel time per family, and the span from the first start
+    to the last end (the difference is the idle time between launches)"""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, start, end from kernels order by start").fetchall()
+    ks = [r for r in rows if r[0].startswith("k_") or r[0].startswith("void k_")]
+    names = [r[0] for r in ks]
+    period = None
+    for R in range(1, len(ks) // replays + 1):
+        tail = names[len(names) - replays * R:]
+        if all(tail[i * R:(i + 1) * R] == tail[:R] for i in range(replays)):
+            period = R
+            break
+    if period is None:
+        print("no repeated tail found")
+        return
+    runs = [ks[len(ks) - (replays - i) * period:len(ks) - (replays - i - 1) * period] for i in range(replays)]
+    fam = {}
+    for run in runs:
+        for n, s, e in run:
+            f = fam.setdefault(n, [0, 0.0])
+            f[0] += 1
+            f[1] += (e - s) / 1e3
+    busy = sum(v[1] for v in fam.values()) / replays
+    span = sum((run[-1][2] - run[0][1]) / 1e3 for run in runs) / replays
+    print("%d kernels per run; per run: kernel time %.1f us, span %.1f us (%.1f%% idle between launches)"
+          % (period, busy, span, 100 * (1 - busy / span)))
+    print("| kernel | launches per run | us per run | % |")
+    print("|---|---|---|---|")
+    for n, (c, t) in sorted(fam.items(), key=lambda kv: -kv[1][1]):
+        nm = n if len(n) < 60 else n[:57] + "..."
+        print("| %s | %d | %.1f | %.1f |" % (nm, c // replays, t / replays, 100 * t / replays / busy))
+
+
 def traffic_summary(fetch_db, write_db):
     """HBM bytes per dispatch of every kernel from a FETCH_SIZE and a WRITE_SIZE rocprofv3 --pmc run of
     `bench.py --copy-probe`, each scaled by what its counter reads for the probe's 1 GiB device copy (the first
@@ -301,6 +337,9 @@ def traffic_summary(fetch_db, write_db):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--trace-summary":
         trace_summary(sys.argv[2])
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--replay-summary":
+        replay_summary(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 3)
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
         traffic_summary(sys.argv[2], sys.argv[3])
