@@ -215,6 +215,11 @@ int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_log, int *o
 /* the transform length the products take for ncoef = na + nb - 1 coefficients under a radix3 rule (NTTM_R3_*) and
  * tile_log (0: default): 2^logn (r3 = 0) or 3 2^logn (r3 = 1) points */
 int nttm_debug_product_len(size_t ncoef, int radix3, int tile_log, int *logn, int *r3);
+/* the e pipeline's device addition / subtraction (k_addsub with the CRT's carry scan): r[0 .. nout) = a 10^(18 sa)
+ * +- b 10^(18 sb) mod 10^(18 nout), device pointers; *ge = the carry out of limb nout - 1 (for a subtraction: 1 when
+ * the difference is >= 0).  Synchronous. */
+int nttm_debug_addsub(uint64_t *r, size_t nout, const uint64_t *a, size_t na, size_t sa, const uint64_t *b, size_t nb,
+                      size_t sb, int sub, int *ge, void *stream);
 /* the global element indices of tile `tile` of a pass of that geometry; returns the tile's element count */
 int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out);
 

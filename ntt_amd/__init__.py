@@ -107,6 +107,8 @@ def lib():
         L.nttm_debug_plan.argtypes = [i32, u64, i32, i32, C.POINTER(C.c_int), i32]; L.nttm_debug_plan.restype = i32
         L.nttm_debug_product_len.argtypes = [sz, i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.nttm_debug_product_len.restype = i32
+        L.nttm_debug_addsub.argtypes = [vp, sz, vp, sz, sz, vp, sz, sz, i32, C.POINTER(C.c_int), vp]
+        L.nttm_debug_addsub.restype = i32
         L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
         L.nttm_debug_tile_indices.restype = i32
         _lib = L
@@ -236,6 +238,17 @@ def e_last_phase_ms():
     v = [C.c_double() for _ in range(4)]
     _check(lib().nttm_e_last_phase_ms(*[C.byref(x) for x in v]))
     return dict(zip(("seed", "tree", "recip", "quot"), (x.value for x in v)))
+
+
+def debug_addsub(a, b, nout, sa=0, sb=0, sub=False, stream=None):
+    """(r, carry out): r = a B^sa +- b B^sb mod B^nout through the device addition (int64 device tensors)"""
+    import torch
+    r = torch.empty(max(nout, 1), dtype=torch.int64, device=a.device)
+    ge = C.c_int()
+    _check(lib().nttm_debug_addsub(_ptr(r), nout, _ptr(a) if a.numel() else None, a.numel(), sa,
+                                   _ptr(b) if b.numel() else None, b.numel(), sb, 1 if sub else 0, C.byref(ge),
+                                   _stream(stream)))
+    return r[:nout], bool(ge.value)
 
 
 def limbs_to_ascii(limbs):

@@ -469,6 +469,128 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
     return nttm_hip_check(hipGetLastError(), "k_crt_fix launch");
 }
 
+/* ---- addition and subtraction of limb arrays, with the carry resolved like the CRT's -------------------------------
+ * r[0 .. nout) = a B^sa + b B^sb, or a B^sa - b B^sb computed as a B^sa + (B^nout - 1 - b B^sb) + 1 (b's limbs
+ * complemented, 1 added at limb 0): its carry out of limb nout - 1 is 1 exactly when the difference is >= 0.
+ * b may instead be B^bpow (bpow >= 0).  Each 2048-limb block writes its limbs for a carry in of 0 and its one-bit
+ * carry map, with the same status words as k_crt; k_crt_scan then gives every block its carry in (and, from the last
+ * block's error bits, the final carry out) and k_addsub_fix adds the carries.  Limbs past nout count as B - 1 so that
+ * the last block's map is the carry out of limb nout - 1. */
+struct AddArgs {
+    const uint64_t *a; uint64_t na, sa;
+    const uint64_t *b; uint64_t nb, sb;
+    int64_t apow, bpow;
+    uint64_t *r; uint64_t nout, nblocks;
+    int sub;
+    uint32_t *status;
+};
+__device__ __forceinline__ uint64_t add_limb(const uint64_t *p, uint64_t n, uint64_t s, uint64_t k)
+{
+    return (k >= s && k - s < n) ? p[k - s] : 0;
+}
+__global__ __launch_bounds__(CRT_T) void k_addsub(AddArgs a)
+{
+    __shared__ uint32_t s_wave[CRT_T / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t blk = blockIdx.x, k0 = blk * CRT_CH;
+    uint64_t wv[CRT_PER], GX[CRT_PER], GY[CRT_PER];
+#pragma unroll
+    for (int i = 0; i < CRT_PER; i++) {
+        const uint64_t k = k0 + wave * 64 * CRT_PER + i * 64 + lane;
+        uint64_t w = NTTM_B - 1;
+        if (k < a.nout) {
+            const uint64_t x = a.apow >= 0 ? (k == (uint64_t)a.apow ? 1u : 0u) : add_limb(a.a, a.na, a.sa, k);
+            const uint64_t y = a.bpow >= 0 ? (k == (uint64_t)a.bpow ? 1u : 0u) : add_limb(a.b, a.nb, a.sb, k);
+            w = x + (a.sub ? NTTM_B - 1 - y : y) + (a.sub && k == 0 ? 1u : 0u);          /* <= 2B - 1 */
+        }
+        wv[i] = w;
+        const uint64_t g = __ballot(w >= NTTM_B), pr = __ballot(w == NTTM_B - 1);
+        GX[i] = g; GY[i] = g | pr;
+    }
+    uint32_t F = CRT_ID_MAP;
+#pragma unroll
+    for (int i = 0; i < CRT_PER; i++) F = map_compose(add_cout(GX[i], GY[i], 0) | (add_cout(GX[i], GY[i], 1) << 1), F);
+    if (lane == 0) s_wave[wave] = F;
+    __syncthreads();
+    uint32_t P = CRT_ID_MAP, A = CRT_ID_MAP;
+    for (int w = 0; w < CRT_T / 64; w++) {
+        if (w < wave) P = map_compose(s_wave[w], P);
+        A = map_compose(s_wave[w], A);
+    }
+    uint32_t c = map_apply(P, 0);
+#pragma unroll
+    for (int i = 0; i < CRT_PER; i++) {
+        const uint64_t S = GX[i] + GY[i] + c;
+        const uint32_t cout = add_cout(GX[i], GY[i], c);
+        const uint64_t CI = S ^ GX[i] ^ GY[i];
+        const uint32_t oin = (uint32_t)(CI >> lane) & 1u;
+        const uint32_t oout = lane == 63 ? cout : (uint32_t)(CI >> (lane + 1)) & 1u;
+        const uint64_t k = k0 + wave * 64 * CRT_PER + i * 64 + lane;
+        if (k < a.nout) a.r[k] = wv[i] + oin - (oout ? NTTM_B : 0);
+        c = cout;
+    }
+    if (tid == 0) {
+        uint32_t st = A | (blk == 0 ? CRT_ST_FIRST : 0u);
+        if (blk + 1 == a.nblocks) st |= ((A & 1u) ? CRT_ST_E0 : 0u) | ((A & 2u) ? CRT_ST_E1 : 0u);
+        a.status[blk] = st;
+    }
+}
+/* the blocks whose carry in is 1: their limbs plus one (a ripple through the leading limbs equal to B - 1) */
+__global__ __launch_bounds__(64) void k_addsub_fix(uint64_t *r, uint64_t nout, const uint32_t *status, uint64_t nblocks)
+{
+    const int lane = threadIdx.x;
+    const uint64_t blk = blockIdx.x;
+    if (blk >= nblocks || !(status[blk] & CRT_ST_CIN)) return;
+    const uint64_t k0 = blk * CRT_CH, end = k0 + CRT_CH < nout ? k0 + CRT_CH : nout;
+    for (uint64_t base = k0; base < end; base += 64) {
+        const uint64_t k = base + lane;
+        const uint64_t v = k < end ? r[k] : 0;
+        const uint64_t ne = __ballot(k < end && v != NTTM_B - 1);
+        const int f = ne ? __ffsll((long long)ne) - 1 : 64;                    /* first limb below B - 1 */
+        if (k < end && lane <= f) r[k] = lane < f ? 0 : v + 1;
+        if (ne) return;
+    }
+}
+int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, uint64_t sa, int64_t apow, const uint64_t *b,
+                    uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s)
+{
+    if (!nout) return 0;
+    const uint64_t nblocks = (nout + CRT_CH - 1) / CRT_CH;
+    if (nblocks > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "addsub: too many blocks");
+    if ((uintptr_t)scratch & 15) return nttm_fail(NTTM_EINVAL, "addsub: scratch not 16-byte aligned");
+    AddArgs A;
+    A.a = a; A.na = a ? na : 0; A.sa = sa; A.apow = apow; A.b = b; A.nb = b ? nb : 0; A.sb = sb; A.bpow = bpow;
+    A.r = r; A.nout = nout; A.nblocks = nblocks; A.sub = sub; A.status = scratch + CRT_CTL_WORDS;
+    hipLaunchKernelGGL(k_addsub, dim3((unsigned)nblocks), dim3(CRT_T), 0, s, A);
+    int rc = nttm_hip_check(hipGetLastError(), "k_addsub launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_crt_scan, dim3(1), dim3(CRT_SCAN_T), 0, s, A.status, nblocks, scratch);
+    if ((rc = nttm_hip_check(hipGetLastError(), "k_crt_scan launch"))) return rc;
+    hipLaunchKernelGGL(k_addsub_fix, dim3((unsigned)nblocks), dim3(64), 0, s, r, nout, (const uint32_t *)A.status, nblocks);
+    return nttm_hip_check(hipGetLastError(), "k_addsub_fix launch");
+}
+
+/* the normalized length of x[0 .. n): one more than the index of the highest nonzero limb */
+__global__ __launch_bounds__(256) void k_top_nonzero(const uint64_t *x, uint64_t n, unsigned long long *out)
+{
+    unsigned long long best = 0;
+    for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256)
+        if (x[k]) best = k + 1;
+    for (int d = 32; d; d >>= 1) {
+        const unsigned long long o = __shfl_down(best, d, 64);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0 && best) atomicMax(out, best);
+}
+int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s)
+{
+    int rc = nttm_hip_check(hipMemsetAsync(out_dev, 0, 8, s), "clear length");
+    if (rc || !n) return rc;
+    const uint64_t blocks = (n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048;
+    hipLaunchKernelGGL(k_top_nonzero, dim3((unsigned)blocks), dim3(256), 0, s, x, n, (unsigned long long *)out_dev);
+    return nttm_hip_check(hipGetLastError(), "k_top_nonzero launch");
+}
+
 /* products with an empty operand: c[0 .. nc) = x zero-extended (a kernel, not a memset, so a captured graph holds it) */
 __global__ __launch_bounds__(256) void k_fill_x(const CrtDesc *d, uint64_t nprod)
 {

@@ -1,8 +1,11 @@
-/* ebig.h - host-side limb vectors and the Newton reciprocal / corrected quotient of the e pipeline.
+/* ebig.h - the Newton reciprocal and the corrected quotient of the e pipeline, written over a number backend.
  *
  * Numbers are base-10^18 limb vectors (least significant first, normalized: no leading zero limb; 0 is empty).  The
- * linear work (additions, subtractions, shifts by whole limbs, comparisons) is done here; every product goes through a
- * caller-supplied multiplier `int mul(const Big &a, const Big &b, Big &c)` (the device product in the library).
+ * algorithms below use only the backend's operations (the device backend in epipe.hip, HostOps here for the host):
+ *   size(a), top(a, t) (the top t limbs), shr(a, k) (drop the low k limbs), limb(a, i), low(a, k, r) (a mod B^k),
+ *   mul(a, b, c), pow_sub(K, a, r) (B^K - a, for a < B^K), addsub(r, a, sa, b, sb, sub, &ge) (a B^sa +- b B^sb; for a
+ *   subtraction ge tells whether it was >= 0, and r is only meaningful then), add_one / sub_one, div_pow(e, d, r)
+ *   (floor(B^e / d) for a few limbs).
  *
  * Reciprocal (the reference's Newton correction form, E/ALGORITHM.md:197-246, E/newton_db.c:218): x_l approximates
  * B^(n+l) / Q (Q of n limbs) to within 3 units.  From x_h to x_l with l <= 2h - 2, using only the top t = l + 2 limbs
@@ -10,9 +13,9 @@
  * The relative error after the step is eps_h^2 + B^(-l-1) (the truncation of Q) + one unit of rounding; with
  * eps_h <= 3 B^-h and l <= 2h - 2 the squared term is below 9/B units, so the error stays within 3 units at every
  * level.  Each step costs two products of about 1.5 l limbs; the levels halve down to a base of at most 8 limbs, which
- * is exact long division on the host.
+ * is exact long division.
  * Quotient: Y = floor(A x_m / B^n) lies within 4 of floor(A B^m / Q) for A < Q; one product Y Q and a few
- * subtractions make it exact. */
+ * additions and subtractions of Q to the remainder R = A B^m - Y Q make it exact. */
 #ifndef NTTM_EBIG_H
 #define NTTM_EBIG_H
 #include <stdint.h>
@@ -101,54 +104,101 @@ static inline Big big_div_pow(size_t e, const Big &D)
     return q;
 }
 
-/* x ~ B^(n+m) / Q within 3 units (Q normalized, n = |Q| >= 1); returns 0 or the multiplier's error, -1 on a broken
+/* the host backend: limb vectors, products by the caller's multiplier int mul(const Big &, const Big &, Big &) */
+template <class Mul> struct HostOps {
+    typedef Big Num;
+    Mul &m;
+    explicit HostOps(Mul &mm) : m(mm) {}
+    size_t size(const Big &a) const { return a.size(); }
+    Big top(const Big &a, size_t t) const { return big_top(a, t); }
+    Big shr(const Big &a, size_t k) const { return big_shr(a, k); }
+    uint64_t limb(const Big &a, size_t i) const { return i < a.size() ? a[i] : 0; }
+    int low(const Big &a, size_t k, Big &r) const { r.assign(a.begin(), a.begin() + std::min(k, a.size())); big_norm(r); return 0; }
+    int mul(const Big &a, const Big &b, Big &c) { return m(a, b, c); }
+    int pow_sub(size_t K, const Big &a, Big &r) const { r = big_sub(big_pow(K), a); return 0; }
+    int addsub(Big &r, const Big &a, size_t sa, const Big &b, size_t sb, bool sub, bool *ge) const
+    {
+        const Big A = big_shl(a, sa), Bv = big_shl(b, sb);
+        if (!sub) { r = big_add(A, Bv); if (ge) *ge = true; return 0; }
+        const bool g = big_cmp(A, Bv) >= 0;
+        if (ge) *ge = g;
+        r = g ? big_sub(A, Bv) : Big();
+        return 0;
+    }
+    int add_one(Big &a) const { a = big_add(a, big_small(1)); return 0; }
+    int sub_one(Big &a) const { if (a.empty()) return -1; a = big_sub(a, big_small(1)); return 0; }
+    int div_pow(size_t e, const Big &d, Big &r) const { r = big_div_pow(e, d); return 0; }
+};
+
+/* x ~ B^(n+m) / Q within 3 units (Q normalized, n = |Q| >= 1); returns 0, the backend's error, or -1 on a broken
  * invariant */
-template <class Mul> int eb_recip(Mul &mul, const Big &Q, size_t m, Big &x)
+template <class O> int eb_recip(O &o, const typename O::Num &Q, size_t m, typename O::Num &x)
 {
-    const size_t n = Q.size();
+    typedef typename O::Num N;
+    const size_t n = o.size(Q);
     if (!n) return -1;
     std::vector<size_t> lv;
     for (size_t l = m;;) { lv.push_back(l); if (l <= 8) break; l = (l + 3) / 2; }    /* h = ceil((l + 2) / 2) */
     size_t h = lv.back();
-    {
-        const size_t t0 = std::min(n, h + 2);
-        x = big_div_pow(t0 + h, big_top(Q, t0));
-    }
-    Big T, E, D;
+    const size_t t0 = std::min(n, h + 2);
+    int rc = o.div_pow(t0 + h, o.top(Q, t0), x);
+    if (rc) return rc;
     for (size_t i = lv.size() - 1; i-- > 0;) {
         const size_t l = lv[i], t = std::min(n, l + 2), K = t + h;
-        int rc = mul(big_top(Q, t), x, T);
-        if (rc) return rc;
+        N T, E, D, xn;
+        if ((rc = o.mul(o.top(Q, t), x, T))) return rc;
         bool neg;
-        if (T.size() > K) {                                        /* T >= B^K: E = T - B^K, T < 2 B^K */
-            if (T.size() != K + 1 || T[K] != 1) return -1;
-            T.pop_back(); big_norm(T); E.swap(T); neg = true;
+        if (o.size(T) > K) {                                        /* T >= B^K: E = T - B^K, T < 2 B^K */
+            if (o.size(T) != K + 1 || o.limb(T, K) != 1) return -1;
+            if ((rc = o.low(T, K, E))) return rc;
+            neg = true;
         } else {
-            E = big_sub(big_pow(K), T); neg = false;
+            if ((rc = o.pow_sub(K, T, E))) return rc;
+            neg = false;
         }
-        if ((rc = mul(x, E, D))) return rc;
-        const Big st = big_shr(D, t + 2 * h - l), y = big_shl(x, l - h);
-        if (neg && big_cmp(y, st) < 0) return -1;
-        x = neg ? big_sub(y, st) : big_add(y, st);
+        T = N();
+        if ((rc = o.mul(x, E, D))) return rc;
+        E = N();
+        bool ge = true;
+        if ((rc = o.addsub(xn, x, l - h, o.shr(D, t + 2 * h - l), 0, neg, &ge))) return rc;   /* x B^(l-h) +- step */
+        if (!ge) return -1;
+        x = xn;
         h = l;
     }
     return 0;
 }
 
 /* Y = floor(A B^m / Q) for 0 <= A < Q, from x ~ B^(n+m) / Q within 3 units */
-template <class Mul> int eb_quotient(Mul &mul, const Big &A, const Big &Q, size_t m, const Big &x, Big &Y)
+template <class O> int eb_quotient(O &o, const typename O::Num &A, const typename O::Num &Q, size_t m,
+                                   const typename O::Num &x, typename O::Num &Y)
 {
-    const size_t n = Q.size();
-    Big T;
-    int rc = mul(A, x, T);
+    typedef typename O::Num N;
+    const size_t n = o.size(Q);
+    N T, R, R2;
+    int rc = o.mul(A, x, T);
     if (rc) return rc;
-    Y = big_shr(T, n);
-    if ((rc = mul(Y, Q, T))) return rc;
-    const Big ABm = big_shl(A, m);
+    Y = o.shr(T, n);
+    if ((rc = o.mul(Y, Q, T))) return rc;
+    bool ge, neg = false;
+    if ((rc = o.addsub(R, A, m, T, 0, true, &ge))) return rc;      /* R = A B^m - Y Q */
+    if (!ge) {                                                       /* negative: R = Y Q - A B^m */
+        if ((rc = o.addsub(R, T, 0, A, m, true, &ge))) return rc;
+        neg = true;
+    }
+    T = N();
     int k = 0;
-    for (; k < 16 && big_cmp(T, ABm) > 0; k++) { Y = big_sub(Y, big_small(1)); T = big_sub(T, Q); }
-    Big R = big_sub(ABm, T);
-    for (; k < 16 && big_cmp(R, Q) >= 0; k++) { Y = big_add(Y, big_small(1)); R = big_sub(R, Q); }
+    for (; k < 16 && neg; k++) {                                     /* R < 0: Y -= 1, R += Q */
+        if ((rc = o.sub_one(Y))) return rc;
+        if ((rc = o.addsub(R2, Q, 0, R, 0, true, &ge))) return rc;  /* Q - |R| */
+        if (ge) { R = R2; neg = false; }
+        else { if ((rc = o.addsub(R2, R, 0, Q, 0, true, &ge))) return rc; R = R2; }
+    }
+    for (; k < 16; k++) {                                            /* R >= Q: Y += 1, R -= Q */
+        if ((rc = o.addsub(R2, R, 0, Q, 0, true, &ge))) return rc;
+        if (!ge) break;
+        R = R2;
+        if ((rc = o.add_one(Y))) return rc;
+    }
     return k < 16 ? 0 : -1;
 }
 #endif

@@ -1,5 +1,5 @@
 /* epipe.hip - digits of e on one GPU through the product library: the caller side of the hot path (binary splitting
- * with batched products, then a Newton reciprocal and a corrected quotient, every product on the device).
+ * with batched products, then a Newton reciprocal and a corrected quotient, every number in device memory).
  *
  * The reference's pipeline (E/ALGORITHM.md:12-20,59-94) sums 1/k! by binary splitting into P/Q, takes a Newton
  * reciprocal of Q with cut products (E/newton_db.c) and divides.  Here, in base B = 10^18 (the quotient's limbs are the
@@ -12,10 +12,13 @@
  *   4. Y = floor(A x / B^n), corrected by the remainder A B^m - Y Q to floor(A B^m / Q).
  * The series tail after N terms is below 2/(N+1)! < B^-(m+2), so Y is e's fraction to m limbs unless the m+1-th and
  * m+2-th limbs of the exact fraction are all B - 1 (which the caller's check windows would catch).
- * Linear host work (additions, subtractions, shifts) runs on the host; every product runs on the device (nttm_mul). */
+ * DevOps is ebig.h's backend on the device: products by nttm_mul, additions and subtractions by k_addsub with the
+ * CRT's block carry maps (crt.hip), normalized lengths read back as one word; only the few-limb base of the Newton
+ * iteration and the final fraction cross to the host. */
 #include <math.h>
 #include <string.h>
 #include <chrono>
+#include <memory>
 #include <vector>
 #include "internal.h"
 #include "ebig.h"
@@ -23,6 +26,12 @@
 #define E_SEED_WD 64              /* base-10^9 digits one thread holds per number */
 #define E_SEED_GMAX 64            /* terms per leaf at most */
 #define E_D9 1000000000ULL
+
+int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, uint64_t sa, int64_t apow, const uint64_t *b,
+                    uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s);
+int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
+uint64_t nttm_crt_blocks(uint64_t nc);
+uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 
 /* leaf i folds the terms [i G, min((i + 1) G, nterms)): P = sum_j prod_{k > j} (k + 1), Q = prod (k + 1); written as
  * wl limbs each at pq + 2 wl i (P) and pq + 2 wl i + wl (Q), normalized lengths at lens[2 i], lens[2 i + 1].  The host
@@ -69,44 +78,164 @@ __global__ __launch_bounds__(256) void k_e_seed(uint64_t nterms, uint32_t G, uin
     lens[2 * leaf + 1] = qn;
 }
 
-/* a b on the device, the operands copied from and the product back to the host */
-struct DevMul {
-    uint64_t *da = nullptr, *db = nullptr, *dc = nullptr;
-    size_t cap = 0;
-    hipStream_t s;
-    ~DevMul() { release(); }
-    void release()
+/* ---- the device backend of ebig.h ------------------------------------------------------------------------------------ */
+struct DBuf {
+    uint64_t *p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+};
+/* a normalized number: n limbs at buf + off (views share the buffer) */
+struct DNum {
+    std::shared_ptr<DBuf> buf;
+    size_t off = 0, n = 0;
+    const uint64_t *ptr() const { return buf ? buf->p + off : nullptr; }
+    uint64_t *wptr() const { return buf ? buf->p + off : nullptr; }
+};
+
+struct DevOps {
+    typedef DNum Num;
+    hipStream_t s = nullptr;
+    uint32_t *scratch = nullptr; size_t scratch_words = 0;        /* carry status words of k_addsub */
+    uint64_t *dword = nullptr;                                    /* one device word (lengths) */
+    uint64_t *hword = nullptr;                                    /* pinned host words for read-back */
+    int err = 0;
+    ~DevOps()
     {
-        if (da) (void)hipFree(da);
-        if (db) (void)hipFree(db);
-        if (dc) (void)hipFree(dc);
-        da = db = dc = nullptr; cap = 0;
+        if (scratch) (void)hipFree(scratch);
+        if (dword) (void)hipFree(dword);
+        if (hword) (void)hipHostFree(hword);
     }
-    int grow(size_t limbs)
+    int init(hipStream_t st)
     {
-        if (limbs <= cap) return 0;
-        release();
-        const size_t c = limbs + limbs / 4 + 64;
-        if (hipMalloc(&da, c * 8) != hipSuccess || hipMalloc(&db, c * 8) != hipSuccess || hipMalloc(&dc, 2 * c * 8) != hipSuccess) {
-            (void)hipGetLastError(); release();
-            return nttm_fail(NTTM_ENOMEM, "e pipeline: device buffers of %zu limbs", c);
+        s = st;
+        if (hipMalloc(&dword, 64) != hipSuccess || hipHostMalloc((void **)&hword, 64, 0) != hipSuccess) {
+            (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "e pipeline: device words");
         }
-        cap = c;
         return 0;
     }
-    int operator()(const Big &a, const Big &b, Big &c)
+    int alloc(size_t n, DNum &r)
     {
-        if (a.empty() || b.empty()) { c.clear(); return 0; }
-        int rc = grow(std::max(a.size(), b.size()));
-        if (rc) return rc;
-        if ((rc = nttm_hip_check(hipMemcpyAsync(da, a.data(), a.size() * 8, hipMemcpyHostToDevice, s), "copy"))) return rc;
-        if ((rc = nttm_hip_check(hipMemcpyAsync(db, b.data(), b.size() * 8, hipMemcpyHostToDevice, s), "copy"))) return rc;
-        if ((rc = nttm_mul(dc, da, a.size(), db, b.size(), s))) return rc;
-        c.resize(a.size() + b.size());
-        if ((rc = nttm_hip_check(hipMemcpyAsync(c.data(), dc, c.size() * 8, hipMemcpyDeviceToHost, s), "copy"))) return rc;
-        if ((rc = nttm_hip_check(hipStreamSynchronize(s), "sync"))) return rc;
-        big_norm(c);
+        r = DNum();
+        std::shared_ptr<DBuf> b = std::make_shared<DBuf>();
+        if (hipMalloc(&b->p, (n ? n : 1) * 8) != hipSuccess) {
+            (void)hipGetLastError(); b->p = nullptr;
+            return nttm_fail(NTTM_ENOMEM, "e pipeline: %zu limbs", n);
+        }
+        r.buf = b; r.n = n;
         return 0;
+    }
+    int read_word(const void *src, size_t bytes, uint64_t *v)
+    {
+        *v = 0;
+        RCK(nttm_hip_check(hipMemcpyAsync(hword, src, bytes, hipMemcpyDeviceToHost, s), "read back"));
+        RCK(nttm_hip_check(hipStreamSynchronize(s), "read back"));
+        memcpy(v, hword, bytes);
+        return 0;
+    }
+    /* drop leading zero limbs: the top limb first, a device scan when it is zero */
+    int norm(DNum &a)
+    {
+        if (!a.n) { a = DNum(); return 0; }
+        uint64_t v;
+        RCK(read_word(a.ptr() + a.n - 1, 8, &v));
+        if (v) return 0;
+        RCK(nttm_run_top_nonzero(a.ptr(), a.n - 1, dword, s));
+        RCK(read_word(dword, 8, &v));
+        a.n = v;
+        if (!a.n) a = DNum();
+        return 0;
+    }
+    size_t size(const DNum &a) const { return a.n; }
+    DNum top(const DNum &a, size_t t) const { if (a.n <= t) return a; DNum r = a; r.off += a.n - t; r.n = t; return r; }
+    DNum shr(const DNum &a, size_t k) const { if (a.n <= k) return DNum(); DNum r = a; r.off += k; r.n -= k; return r; }
+    uint64_t limb(const DNum &a, size_t i)
+    {
+        uint64_t v = ~(uint64_t)0;
+        if (i >= a.n) return 0;
+        if ((err = read_word(a.ptr() + i, 8, &v))) return ~(uint64_t)0;
+        return v;
+    }
+    int low(const DNum &a, size_t k, DNum &r) { r = a; r.n = std::min(k, a.n); return norm(r); }
+    int mul(const DNum &a, const DNum &b, DNum &c)
+    {
+        if (!a.n || !b.n) { c = DNum(); return 0; }
+        DNum t;
+        RCK(alloc(a.n + b.n, t));
+        RCK(nttm_mul(t.wptr(), a.ptr(), a.n, b.ptr(), b.n, s));
+        uint64_t v;
+        RCK(read_word(t.ptr() + t.n - 1, 8, &v));
+        if (!v) t.n--;                                             /* normalized operands: at most one zero limb on top */
+        c = t;
+        return 0;
+    }
+    /* r = a B^sa +- (b B^sb, or B^bpow); for a subtraction *ge = (the difference >= 0), r empty otherwise */
+    int addsub_raw(DNum &r, const DNum *a, size_t sa, int64_t apow, const DNum *b, size_t sb, int64_t bpow, bool sub, bool *ge)
+    {
+        const size_t ea = apow >= 0 ? (size_t)apow + 1 : (a && a->n ? a->n + sa : 0);
+        const size_t eb = bpow >= 0 ? (size_t)bpow + 1 : (b && b->n ? b->n + sb : 0);
+        const size_t nout = std::max(ea, eb) + (sub ? 0 : 1);
+        if (ge) *ge = true;
+        if (!nout) { r = DNum(); return 0; }
+        const size_t words = nttm_crt_scratch_words(nttm_crt_blocks(nout));
+        if (words > scratch_words) {
+            if (scratch) { (void)hipStreamSynchronize(s); (void)hipFree(scratch); scratch = nullptr; scratch_words = 0; }
+            if (hipMalloc(&scratch, words * 4) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "e pipeline: carry words"); }
+            scratch_words = words;
+        }
+        DNum t;
+        RCK(alloc(nout, t));
+        RCK(nttm_run_addsub(t.wptr(), nout, a ? a->ptr() : nullptr, a ? a->n : 0, sa, apow, b ? b->ptr() : nullptr,
+                            b ? b->n : 0, sb, bpow, sub ? 1 : 0, scratch, s));
+        if (sub) {
+            uint64_t v;
+            RCK(read_word(scratch, 8, &v));                        /* ctl[1]: the final carry out */
+            const bool g = (v >> 32) != 0;
+            if (ge) *ge = g;
+            if (!g) { r = DNum(); return 0; }
+        }
+        RCK(norm(t));
+        r = t;
+        return 0;
+    }
+    int addsub(DNum &r, const DNum &a, size_t sa, const DNum &b, size_t sb, bool sub, bool *ge)
+    {
+        return addsub_raw(r, &a, sa, -1, &b, sb, -1, sub, ge);
+    }
+    int pow_sub(size_t K, const DNum &a, DNum &r)
+    {
+        bool ge;
+        RCK(addsub_raw(r, nullptr, 0, (int64_t)K, &a, 0, -1, true, &ge));
+        return ge ? 0 : nttm_fail(NTTM_ECHECK, "e pipeline: B^%zu below its subtrahend", K);
+    }
+    int add_one(DNum &a) { DNum r; RCK(addsub_raw(r, &a, 0, -1, nullptr, 0, 0, false, nullptr)); a = r; return 0; }
+    int sub_one(DNum &a)
+    {
+        bool ge; DNum r;
+        RCK(addsub_raw(r, &a, 0, -1, nullptr, 0, 0, true, &ge));
+        if (!ge) return nttm_fail(NTTM_ECHECK, "e pipeline: 0 - 1");
+        a = r;
+        return 0;
+    }
+    int to_host(const DNum &a, Big &h)
+    {
+        h.assign(a.n, 0);
+        if (a.n) {
+            RCK(nttm_hip_check(hipMemcpyAsync(h.data(), a.ptr(), a.n * 8, hipMemcpyDeviceToHost, s), "copy to host"));
+            RCK(nttm_hip_check(hipStreamSynchronize(s), "copy to host"));
+        }
+        return 0;
+    }
+    int from_host(const Big &h, DNum &a)
+    {
+        if (h.empty()) { a = DNum(); return 0; }
+        RCK(alloc(h.size(), a));
+        RCK(nttm_hip_check(hipMemcpyAsync(a.wptr(), h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "copy to device"));
+        return nttm_hip_check(hipStreamSynchronize(s), "copy to device");
+    }
+    int div_pow(size_t e, const DNum &d, DNum &r)                  /* a few limbs: on the host */
+    {
+        Big hd;
+        RCK(to_host(d, hd));
+        return from_host(big_div_pow(e, hd), r);
     }
 };
 
@@ -126,91 +255,96 @@ static int seed_shape(size_t N, uint32_t *G, uint32_t *wl)
     return nttm_fail(NTTM_EBOUND, "e pipeline: no leaf shape for %zu terms", N);
 }
 
-/* P, Q of the N-term series on the host */
-static int e_series(size_t N, Big &P, Big &Q, hipStream_t s)
+/* P, Q of the N-term series, in device memory */
+static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
 {
+    hipStream_t s = o.s;
     uint32_t G, wl;
     RCK(seed_shape(N, &G, &wl));
     const uint64_t nleaf = (N + G - 1) / G;
-    uint64_t *pq = nullptr, *dp = nullptr, *dq = nullptr;
+    DNum pq;
     uint32_t *lens = nullptr;
-    int rc = 0;
     auto t0 = eclk::now();
-    if (hipMalloc(&pq, nleaf * 2 * wl * 8) != hipSuccess || hipMalloc(&lens, nleaf * 2 * 4) != hipSuccess) {
-        (void)hipGetLastError(); rc = nttm_fail(NTTM_ENOMEM, "e pipeline: %llu leaves", (unsigned long long)nleaf);
+    RCK(o.alloc(nleaf * 2 * wl, pq));
+    if (hipMalloc(&lens, nleaf * 2 * 4) != hipSuccess) {
+        (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "e pipeline: %llu leaves", (unsigned long long)nleaf);
     }
-    std::vector<uint32_t> hl;
-    if (!rc) {
-        hipLaunchKernelGGL(k_e_seed, dim3((unsigned)((nleaf + 255) / 256)), dim3(256), 0, s, (uint64_t)N, G, nleaf, wl, pq, lens);
-        rc = nttm_hip_check(hipGetLastError(), "k_e_seed launch");
-    }
-    if (!rc) {
-        hl.resize(2 * nleaf);
-        rc = nttm_hip_check(hipMemcpyAsync(hl.data(), lens, nleaf * 2 * 4, hipMemcpyDeviceToHost, s), "copy leaf lengths");
-        if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "k_e_seed");
-    }
+    std::vector<uint32_t> hl(2 * nleaf);
+    hipLaunchKernelGGL(k_e_seed, dim3((unsigned)((nleaf + 255) / 256)), dim3(256), 0, s, (uint64_t)N, G, nleaf, wl, pq.wptr(), lens);
+    int rc = nttm_hip_check(hipGetLastError(), "k_e_seed launch");
+    if (!rc) rc = nttm_hip_check(hipMemcpyAsync(hl.data(), lens, nleaf * 2 * 4, hipMemcpyDeviceToHost, s), "copy leaf lengths");
+    if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "k_e_seed");
+    (void)hipFree(lens);
+    if (rc) return rc;
     g_e_ms[0] = ms_since(t0);
     t0 = eclk::now();
     size_t cap = 2;
-    std::vector<nttm_bs_node> leaves;
-    if (!rc) {
-        leaves.resize(nleaf);
-        for (uint64_t i = 0; i < nleaf; i++) {
-            leaves[i].p = pq + 2 * wl * i; leaves[i].pn = hl[2 * i];
-            leaves[i].q = pq + 2 * wl * i + wl; leaves[i].qn = hl[2 * i + 1];
-            cap += hl[2 * i + 1];                                  /* |Q| <= sum of the leaves' |Q|; P < 2 Q */
-        }
-        if (hipMalloc(&dp, cap * 8) != hipSuccess || hipMalloc(&dq, cap * 8) != hipSuccess) {
-            (void)hipGetLastError(); rc = nttm_fail(NTTM_ENOMEM, "e pipeline: P and Q of %zu limbs", cap);
-        }
+    std::vector<nttm_bs_node> leaves(nleaf);
+    for (uint64_t i = 0; i < nleaf; i++) {
+        leaves[i].p = pq.ptr() + 2 * wl * i; leaves[i].pn = hl[2 * i];
+        leaves[i].q = pq.ptr() + 2 * wl * i + wl; leaves[i].qn = hl[2 * i + 1];
+        cap += hl[2 * i + 1];                                      /* |Q| <= sum of the leaves' |Q|; P < 2 Q */
     }
-    if (!rc) {
-        size_t pn = 0, qn = 0;
-        rc = nttm_bs_tree(leaves.data(), nleaf, dp, cap, &pn, dq, cap, &qn, s);
-        if (!rc) {
-            P.resize(pn); Q.resize(qn);
-            rc = nttm_hip_check(hipMemcpy(P.data(), dp, pn * 8, hipMemcpyDeviceToHost), "copy P");
-            if (!rc) rc = nttm_hip_check(hipMemcpy(Q.data(), dq, qn * 8, hipMemcpyDeviceToHost), "copy Q");
-        }
-    }
-    if (pq) (void)hipFree(pq);
-    if (lens) (void)hipFree(lens);
-    if (dp) (void)hipFree(dp);
-    if (dq) (void)hipFree(dq);
+    RCK(o.alloc(cap, P));
+    RCK(o.alloc(cap, Q));
+    size_t pn = 0, qn = 0;
+    RCK(nttm_bs_tree(leaves.data(), nleaf, P.wptr(), cap, &pn, Q.wptr(), cap, &qn, s));
+    P.n = pn; Q.n = qn;                                            /* normalized by the tree */
     g_e_ms[1] = ms_since(t0);
-    big_norm(P); big_norm(Q);
-    return rc;
+    return 0;
 }
 
 extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
 {
     if (!out || !m) return nttm_fail(NTTM_EINVAL, "nttm_e_fraction arguments");
-    hipStream_t s = (hipStream_t)stream;
     for (double &v : g_e_ms) v = 0;
     /* N: log(N!) > 18 (m + 2) log 10 */
     const double need = 18.0 * (double)(m + 2) * log(10.0);
     size_t N = 2;
     while (lgamma((double)N + 1.0) < need) N += N / 2 + 1;
     { size_t lo = N / 2, hi = N; while (lo + 1 < hi) { size_t md = (lo + hi) / 2; if (lgamma((double)md + 1.0) < need) lo = md; else hi = md; } N = hi; }
-    Big P, Q;
-    RCK(e_series(N, P, Q, s));
-    if (big_cmp(P, Q) <= 0) return nttm_fail(NTTM_ECHECK, "e pipeline: P <= Q");
-    const Big A = big_sub(P, Q);                                  /* e - 2 ~ A / Q, A < Q */
-    P.clear(); P.shrink_to_fit();
-    DevMul dm; dm.s = s;
-    Big x, Y;
+    DevOps o;
+    RCK(o.init((hipStream_t)stream));
+    DNum P, Q, A, x, Y;
+    RCK(e_series(o, N, P, Q));
+    bool ge;
+    RCK(o.addsub(A, P, 0, Q, 0, true, &ge));                      /* e - 2 ~ A / Q, 0 <= A < Q */
+    if (!ge || !A.n) return nttm_fail(NTTM_ECHECK, "e pipeline: P <= Q");
+    P = DNum();
     auto t0 = eclk::now();
-    int rc = eb_recip(dm, Q, m, x);                               /* x ~ B^(n+m) / Q */
-    if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: reciprocal invariant broken");
+    int rc = eb_recip(o, Q, m, x);                                /* x ~ B^(n+m) / Q */
+    if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: reciprocal invariant broken%s", o.err ? " (read-back failed)" : "");
     g_e_ms[2] = ms_since(t0);
     t0 = eclk::now();
-    rc = eb_quotient(dm, A, Q, m, x, Y);                          /* floor(A B^m / Q) */
+    rc = eb_quotient(o, A, Q, m, x, Y);                           /* floor(A B^m / Q) */
     if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: quotient correction did not converge");
-    g_e_ms[3] = ms_since(t0);
-    if (Y.size() > m) return nttm_fail(NTTM_ECHECK, "e pipeline: fraction of %zu limbs above %zu", Y.size(), m);
+    if (Y.n > m) return nttm_fail(NTTM_ECHECK, "e pipeline: fraction of %zu limbs above %zu", Y.n, m);
     memset(out, 0, m * 8);
-    memcpy(out, Y.data(), Y.size() * 8);
+    if (Y.n) {
+        RCK(nttm_hip_check(hipMemcpyAsync(out, Y.ptr(), Y.n * 8, hipMemcpyDeviceToHost, o.s), "copy fraction"));
+        RCK(nttm_hip_check(hipStreamSynchronize(o.s), "copy fraction"));
+    }
+    g_e_ms[3] = ms_since(t0);
     return 0;
+}
+
+/* the device addition / subtraction, for tests: r[0 .. nout) = a B^sa +- b B^sb (device pointers), *ge = the final
+ * carry out (for a subtraction: the difference is >= 0) */
+extern "C" int nttm_debug_addsub(uint64_t *r, size_t nout, const uint64_t *a, size_t na, size_t sa, const uint64_t *b,
+                                 size_t nb, size_t sb, int sub, int *ge, void *stream)
+{
+    if (!r || !ge || (na && !a) || (nb && !b)) return nttm_fail(NTTM_EINVAL, "nttm_debug_addsub arguments");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *scratch = nullptr;
+    const size_t words = nttm_crt_scratch_words(nttm_crt_blocks(nout ? nout : 1));
+    if (hipMalloc(&scratch, words * 4) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "addsub scratch"); }
+    int rc = nttm_run_addsub(r, nout, a, na, sa, -1, b, nb, sb, -1, sub, scratch, s);
+    uint32_t ctl[2] = {0, 0};
+    if (!rc && nout) rc = nttm_hip_check(hipMemcpyAsync(ctl, scratch, 8, hipMemcpyDeviceToHost, s), "read flag");
+    if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "addsub");
+    (void)hipFree(scratch);
+    *ge = ctl[1] != 0;
+    return rc;
 }
 
 extern "C" int nttm_e_last_phase_ms(double *seed_ms, double *tree_ms, double *recip_ms, double *quot_ms)

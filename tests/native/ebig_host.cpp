@@ -49,17 +49,19 @@ void eb_t_reset(void) { g_products = 0; g_max_operand = 0; }
 long eb_t_recip(const uint64_t *q, size_t n, size_t m, uint64_t *x, size_t cap)
 {
     SchoolMul mul;
+    HostOps<SchoolMul> o(mul);
     Big X;
-    if (eb_recip(mul, load(q, n), m, X)) return -1;
+    if (eb_recip(o, load(q, n), m, X)) return -1;
     return (long)store(X, x, cap);
 }
 /* floor(A B^m / Q) through eb_recip and eb_quotient; returns the limbs written, or -1 */
 long eb_t_quotient(const uint64_t *a, size_t na, const uint64_t *q, size_t n, size_t m, uint64_t *y, size_t cap)
 {
     SchoolMul mul;
+    HostOps<SchoolMul> o(mul);
     const Big Q = load(q, n);
     Big X, Y;
-    if (eb_recip(mul, Q, m, X) || eb_quotient(mul, load(a, na), Q, m, X, Y)) return -1;
+    if (eb_recip(o, Q, m, X) || eb_quotient(o, load(a, na), Q, m, X, Y)) return -1;
     return (long)store(Y, y, cap);
 }
 /* floor(B^e / D) by the host long division */
