@@ -355,6 +355,7 @@ def main():
     ap.add_argument("--no-leaves", action="store_true")
     ap.add_argument("--leaf-reps", type=int, default=3)
     ap.add_argument("--e-digits", type=int, default=10 ** 9, help="digits of e for the pipeline line (0: skip)")
+    ap.add_argument("--e-only", action="store_true", help="only the e pipeline, one JSON object")
     ap.add_argument("--leaves-only", action="store_true", help="only the batched-leaf workloads, one JSON object")
     ap.add_argument("--batched-only", action="store_true", help="with --leaves-only: the 4096-leaf batch alone")
     ap.add_argument("--tile-log", type=int, default=0, help="strided-pass tile, log2 points (12 or 13; 0 = default)")
@@ -370,6 +371,10 @@ def main():
     import torch
     import ntt_amd as nttm
 
+    if args.e_only:                                              # the e pipeline alone (for profiling)
+        nttm.init(0, tile_log=args.tile_log, pts=args.pts)
+        print(json.dumps({"e_pipeline": bench_e(nttm, args.e_digits)}))
+        return
     if args.leaves_only:                                         # the batched-leaf workloads alone (for profiling)
         nttm.init(0, tile_log=args.tile_log, pts=args.pts)
         leaves = bench_leaves(nttm, args.leaf_reps, each=not args.batched_only)
