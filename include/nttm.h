@@ -167,6 +167,12 @@ int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
 /* ---- transform primitives on canonical residues in [0, p) (uint64_t), batch transforms of 2^logn contiguous ---- */
 /* natural order in -> bit-reversed out */
 int nttm_ntt_fwd(int prime, uint64_t *x, int logn, size_t batch, void *stream);
+/* the same transform (identical output) with its first four levels as one split-precision radix-16 stage on INT8
+ * matrix cores: residues split into 7-bit digits, the 16-point DFT as 15 shifted digit products of exact INT32 sums
+ * (v_mfma_i32_16x16x64_i8), recombined mod p (the north star's split-precision stage; no counterpart in the
+ * reference).  logn >= 8.  It is slower than the FP64 stage it replaces, so no product uses it by default.
+ * Synchronous. */
+int nttm_ntt_fwd_mfma(int prime, uint64_t *x, int logn, size_t batch, void *stream);
 /* bit-reversed in -> natural out, times n^-1 */
 int nttm_ntt_inv(int prime, uint64_t *x, int logn, size_t batch, void *stream);
 /* x[i] = x[i] y[i] mod p */

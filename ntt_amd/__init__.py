@@ -92,6 +92,7 @@ def lib():
         L.nttm_batch_plan_info.argtypes = [vp, C.c_char_p, sz]; L.nttm_batch_plan_info.restype = i32
         L.nttm_batch_plan_destroy.argtypes = [vp]; L.nttm_batch_plan_destroy.restype = None
         L.nttm_ntt_fwd.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd.restype = i32
+        L.nttm_ntt_fwd_mfma.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_fwd_mfma.restype = i32
         L.nttm_ntt_inv.argtypes = [i32, vp, i32, sz, vp]; L.nttm_ntt_inv.restype = i32
         L.nttm_pw.argtypes = [i32, vp, vp, sz, vp]; L.nttm_pw.restype = i32
         L.nttm_ntt_inv_pw.argtypes = [i32, vp, vp, i32, i32, sz, vp]; L.nttm_ntt_inv_pw.restype = i32
@@ -359,6 +360,11 @@ class BatchPlan:
 
 def ntt_fwd(prime_idx, x, logn, batch=1, stream=None):
     _check(lib().nttm_ntt_fwd(prime_idx, _ptr(x), logn, batch, _stream(stream)))
+
+
+def ntt_fwd_mfma(prime_idx, x, logn, batch=1, stream=None):
+    """nttm_ntt_fwd with its first four levels on INT8 matrix cores (identical output)"""
+    _check(lib().nttm_ntt_fwd_mfma(prime_idx, _ptr(x), logn, batch, _stream(stream)))
 
 
 def ntt_inv(prime_idx, x, logn, batch=1, stream=None):

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libnttm.so")
-SOURCES = ["api.hip", "ntt.hip", "crt.hip", "epipe.hip", "digits.hip"]
+SOURCES = ["api.hip", "ntt.hip", "crt.hip", "epipe.hip", "digits.hip", "mfma.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]

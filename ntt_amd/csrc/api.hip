@@ -37,6 +37,8 @@ uint64_t nttm_crt_blocks(uint64_t nc);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
 int nttm_run_fill_x(const void *desc_dev, uint64_t nprod, hipStream_t s);
+int nttm_run_dft16_mfma(const uint64_t *x, double *y, int logn, uint64_t batch, const void *etab, const double *tlo,
+                        const double *thi, const double *consts, double p, hipStream_t s);
 int nttm_run_add_carry(uint64_t *c, uint64_t n, const uint64_t *carry, hipStream_t s);
 int nttm_run_r3(bool inv, double *x, uint64_t batch, int logk, const PrimeK &pk, const double2 *b3,
                 const uint64_t *limbs, uint64_t nlimbs, const LoadDesc *ldesc, int which, const double2 *scale,
@@ -905,6 +907,48 @@ extern "C" int nttm_ntt_inv_pw(int prime, uint64_t *x, const uint64_t *y, int ym
     RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
     RCK(inv_planes(prime, (double *)x, (const double *)g.ws, ymode, logn, batch, 1, s));
     return nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s);
+}
+
+/* the forward transform with its first four levels as one radix-16 stage on INT8 matrix cores (mfma.hip), the
+ * remaining levels on the FP64 engine as 16 transforms of n/16 points each: the same canonical output as nttm_ntt_fwd */
+extern "C" int nttm_ntt_fwd_mfma(int prime, uint64_t *x, int logn, size_t batch, void *stream)
+{
+    RCK(prim_check(prime, x, logn, batch));
+    if (logn < 8) return nttm_fail(NTTM_EINVAL, "nttm_ntt_fwd_mfma: 2^%d points (at least 2^8)", logn);
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t p = NTTM_PRIMES[prime], n = (uint64_t)1 << logn, cnt = (uint64_t)batch << logn;
+    const uint64_t w = nttm_root(prime, logn), W = nttm_root(prime, 4);
+    /* host tables: the 15 x 2 A operands (30720 bytes), w^i and w^(4096 i), the recombination constants */
+    const uint64_t nhi = ((n >> 4) + 4095) / 4096;
+    std::vector<uint8_t> et(15 * 2 * 64 * 16, 0);
+    uint64_t Wp[16];
+    for (int i = 0; i < 16; i++) Wp[i] = h_powmod(W, i, p);
+    for (int sh = 0; sh < 15; sh++)
+        for (int h = 0; h < 2; h++)
+            for (int l = 0; l < 64; l++)
+                for (int by = 0; by < 16; by++) {
+                    const int k = l & 15, t = 2 * (l >> 4) + by / 8 + 8 * h, b = by % 8, c = sh - b;
+                    if (c < 0 || c > 7) continue;
+                    et[((sh * 2 + h) * 64 + l) * 16 + by] = (uint8_t)((Wp[(t * k) & 15] >> (7 * c)) & 127);
+                }
+    std::vector<double> tw(4096 + nhi + 4);
+    const uint64_t w4096 = h_powmod(w, 4096, p);
+    for (uint64_t i = 0, v = 1; i < 4096; i++, v = h_mulmod(v, w, p)) tw[i] = h_bal(v, p);
+    for (uint64_t i = 0, v = 1; i < nhi; i++, v = h_mulmod(v, w4096, p)) tw[4096 + i] = h_bal(v, p);
+    const double2 c35 = h_shoup(h_powmod(2, 35, p), p), c70 = h_shoup(h_powmod(2, 70, p), p);
+    double *cs = tw.data() + 4096 + nhi;
+    cs[0] = c35.x; cs[1] = c35.y; cs[2] = c70.x; cs[3] = c70.y;
+    const size_t off_e = align_up(cnt * 8, 256), off_t = align_up(off_e + et.size(), 256);
+    RCK(ws_need(off_t + tw.size() * 8 + 256));
+    HIPCK(hipMemcpyAsync(g.ws + off_e, et.data(), et.size(), hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(g.ws + off_t, tw.data(), tw.size() * 8, hipMemcpyHostToDevice, s));
+    double *y = (double *)g.ws;
+    const double *dt = (const double *)(g.ws + off_t);
+    RCK(nttm_run_dft16_mfma(x, y, logn, batch, g.ws + off_e, dt, dt + 4096, cs, (double)p, s));   /* cs: host copy */
+    RCK(fwd_planes(prime, y, logn - 4, 16 * (uint64_t)batch, s));
+    HIPCK(hipMemcpyAsync(x, y, cnt * 8, hipMemcpyDeviceToDevice, s));
+    RCK(nttm_run_f64_to_canon((double *)x, cnt, g.pk[prime], s));
+    return nttm_hip_check(hipStreamSynchronize(s), "nttm_ntt_fwd_mfma");                 /* the host tables */
 }
 
 /* transforms of 3 2^logk points: the radix-3 layer (nttm_run_r3) and the 2^logk engine on the thirds (batch 3) */
