@@ -216,6 +216,68 @@ PROF_NAMES = (("k_fused2", "k_fused2"), ("k_pass_stream<0", "k_pass<fwd>"), ("k_
               ("k_crt_fix", "k_crt_fix"))
 
 
+def bench_config4(nttm, da, db, reps=3):
+    """the split-precision (INT8 matrix-core) radix-16 stage on the large product's shape: one prime's forward
+    transform of 2^28 points with and without it, and the whole product with every forward transform through it
+    (limbs -> nttm_ntt_fwd_mfma -> pointwise inverse -> CRT), checked bit-identical to nttm_mul's product"""
+    import torch
+    n = da.numel()
+    logn = (2 * n - 1).bit_length()
+    x = torch.empty(1 << logn, dtype=torch.int64, device="cuda")
+    nttm.load(0, x, da)
+    base = x.clone()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(f):
+        best = 1e30
+        for _ in range(reps):
+            x.copy_(base)
+            torch.cuda.synchronize()
+            ev[0].record()
+            f()
+            ev[1].record()
+            torch.cuda.synchronize()
+            best = min(best, ev[0].elapsed_time(ev[1]))
+        return best
+    t_fp64 = timed(lambda: nttm.ntt_fwd(0, x, logn))
+    y64 = x.clone()
+    nttm.profile(True)
+    t_mfma = timed(lambda: nttm.ntt_fwd_mfma(0, x, logn))
+    ks = nttm.profile_read()
+    nttm.profile(False)
+    same_fwd = bool(torch.equal(x, y64))
+    del y64
+    stage = ks.get("k_dft16_mfma", (0, 0.0, 0.0))
+    stage_ms = stage[1] / max(stage[0], 1)
+    int8_ops = 30 * 2 * 16 * 16 * 64 * ((1 << logn) // 256)           # 30 MFMA per 256 points
+    # the whole product with the stage in every forward transform
+    planes = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(3):
+        pa = torch.empty(1 << logn, dtype=torch.int64, device="cuda")
+        pb = torch.empty_like(pa)
+        nttm.load(i, pa, da)
+        nttm.load(i, pb, db)
+        nttm.ntt_fwd_mfma(i, pa, logn)
+        nttm.ntt_fwd_mfma(i, pb, logn)
+        nttm.ntt_inv_pw(i, pa, pb, 0, logn)
+        del pb
+        planes.append(pa)
+    c = nttm.crt(planes, 2 * n - 1, 2 * n)
+    torch.cuda.synchronize()
+    t_prod = time.perf_counter() - t0
+    del planes
+    same_prod = bool(torch.equal(c, nttm.mul(da, db)))
+    return {"transform_points": 1 << logn, "fwd_fp64_ms": round(t_fp64, 3), "fwd_mfma_ms": round(t_mfma, 3),
+            "mfma_stage_kernel_ms": round(stage_ms, 3), "mfma_stage_int8_ops": int8_ops,
+            "mfma_stage_int8_TOPs": round(int8_ops / (stage_ms * 1e-3) / 1e12, 1) if stage_ms else None,
+            "fwd_bit_identical": same_fwd,
+            "product_via_mfma_stage_s": round(t_prod, 4), "product_bit_identical_to_nttm_mul": same_prod,
+            "note": "primitive path (canonical residue planes, separate load / forward / inverse / CRT calls): "
+                    "compare fwd_mfma_ms with fwd_fp64_ms; the default product keeps the FP64 stage"}
+
+
 def bench_e(nttm, digits):
     """e to `digits` fractional digits through nttm_e_fraction (one run, wall clock), checked against the reference's
     sha1 of its ASCII output and its 50-digit windows (tests/golden/e_windows.json)"""
@@ -356,6 +418,8 @@ def main():
     ap.add_argument("--leaf-reps", type=int, default=3)
     ap.add_argument("--e-digits", type=int, default=10 ** 9, help="digits of e for the pipeline line (0: skip)")
     ap.add_argument("--e-only", action="store_true", help="only the e pipeline, one JSON object")
+    ap.add_argument("--no-config4", action="store_true", help="skip the INT8 matrix-core stage measurement")
+    ap.add_argument("--config4-only", action="store_true", help="only the INT8 matrix-core stage measurement")
     ap.add_argument("--leaves-only", action="store_true", help="only the batched-leaf workloads, one JSON object")
     ap.add_argument("--batched-only", action="store_true", help="with --leaves-only: the 4096-leaf batch alone")
     ap.add_argument("--tile-log", type=int, default=0, help="strided-pass tile, log2 points (12 or 13; 0 = default)")
@@ -388,6 +452,9 @@ def main():
     a = gen_limbs(n, 0x5EED0003)
     b = gen_limbs(n, 0x5EED0004)
     nttm.init(0, tile_log=args.tile_log, pts=args.pts)
+    if args.config4_only:                                        # the matrix-core stage alone (for profiling)
+        print(json.dumps({"split_precision_stage": bench_config4(nttm, to_dev(a), to_dev(b))}))
+        return
     if args.copy_probe:
         src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
@@ -430,6 +497,11 @@ def main():
             "launches": launches, "avg_launch_ms": round(kms / launches, 4),
             "alg_bytes_per_launch": kbytes / launches}
 
+    cfg4 = None
+    if not args.no_config4:
+        print("[bench] split-precision stage (INT8 matrix cores)", file=sys.stderr, flush=True)
+        cfg4 = bench_config4(nttm, da, db)
+
     leaves = None
     if not args.no_leaves:
         print("[bench] leaves", file=sys.stderr, flush=True)
@@ -467,7 +539,7 @@ def main():
                                     "crt": round(phase[2], 3)},
                        "kernels_per_step": kernels, "deterministic": deterministic,
                        "vs_baseline_ref": "time ratio to 0.1263 s, 1 MI300A (results/D13b_strategy_e0.txt:26)",
-                       "leaves": leaves, "e_pipeline": epipe},
+                       "leaves": leaves, "e_pipeline": epipe, "split_precision_stage": cfg4},
             "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     nttm.shutdown()
