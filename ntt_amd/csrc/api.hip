@@ -431,10 +431,16 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
  * transforms, the double-fused pass, the inverse; limb reads and the CRT excluded: equal for both lengths) */
 static double sched_bytes(int logn, int r3, int tile_log)
 {
+    static double memo[NTTM_LOGN_MAX + 1][2][2];                 /* per length, radix, tile 12 / 13 (0: not yet) */
+    const bool cacheable = logn >= 0 && logn <= NTTM_LOGN_MAX && (tile_log == 12 || tile_log == 13);
+    double *m = cacheable ? &memo[logn][r3 ? 1 : 0][tile_log - 12] : nullptr;
+    if (m && *m > 0) return *m;
     Plan pl; plan_make2(&pl, logn, 1, product_tile_log(logn, tile_log), NTTM_FUSED_LOG);
     const int P = pl.npass;
-    if (r3) return 3.0 * (2 * (8 + 16 * (P - 1)) + 24 + 16 * (P - 1) + 16);
-    return P >= 2 ? 2 * (8 + 16 * (P - 2)) + 24 + 16 * (P - 1) : 8;
+    const double v = r3 ? 3.0 * (2 * (8 + 16 * (P - 1)) + 24 + 16 * (P - 1) + 16)
+                        : (P >= 2 ? 2 * (8 + 16 * (P - 2)) + 24 + 16 * (P - 1) : 8);
+    if (m) *m = v;
+    return v;
 }
 /* the transform length of a product with ncoef coefficients: 2^logn, or 3 2^(logn - 2) when it fits and the rule
  * (NTTM_R3_*) takes it (the reference takes it whenever it fits, E/rns_mul.c:270-277) */
@@ -713,8 +719,7 @@ static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<si
 /* the group's descriptors as one host image (planes at A), to copy to the group's descriptor base */
 static void group_image(GroupBuild &gb, double *A, char *img)
 {
-    for (uint64_t j = 0; j < gb.cnt; j++) gb.cd[j].pl = A + j * gb.n;
-    memset(img, 0, gb.desc_bytes);
+    for (uint64_t j = 0; j < gb.cnt; j++) gb.cd[j].pl = A + j * gb.n;          /* (the alignment gaps stay unread) */
     memcpy(img + gb.off_cd, gb.cd.data(), gb.cnt * sizeof(CrtDesc));
     memcpy(img + gb.off_ld, gb.ld.data(), gb.cnt * sizeof(LoadDesc));
     if (gb.share) {

@@ -78,6 +78,78 @@ __global__ __launch_bounds__(256) void k_e_seed(uint64_t nterms, uint32_t G, uin
     lens[2 * leaf + 1] = qn;
 }
 
+/* ---- the lowest tree levels by schoolbook products ------------------------------------------------------------------
+ * One wave per pair of adjacent nodes (fixed strides: node i of the input holds P at in + 2 w i and Q at in + 2 w i + w,
+ * lengths lens_in[2 i], lens_in[2 i + 1]): P = P_L Q_R + P_R, Q = Q_L Q_R into node i of the output (stride wo = 2 w + 1);
+ * an odd last node is copied.  Column k of a product, sum_i a_i b_(k-i) (+ P_R's limb k), is below w B^2 + B < 2^128
+ * for w <= 256; it splits into three base-10^18 digits (div_b), and limb k = d0[k] + d1[k-1] + d2[k-2] + carry (one
+ * lane, sequentially; the carry is at most 2). */
+#define BS_MAXW 256
+__device__ __forceinline__ void split3(unsigned __int128 v, uint64_t *d0, uint64_t *d1, uint64_t *d2)
+{
+    const uint64_t hi = (uint64_t)(v >> 64), lo = (uint64_t)v;
+    const uint64_t h1 = hi / NTTM_B, h0 = hi - h1 * NTTM_B;
+    uint64_t q, r;
+    div_b(h0, lo, &q, &r);                                         /* v = (h1 2^64 + q) B + r */
+    *d0 = r;
+    div_b(h1, q, d2, d1);                                          /* h1 2^64 + q = d2 B + d1 */
+}
+__device__ void school_one(const uint64_t *a, uint32_t na, const uint64_t *b, uint32_t nb, const uint64_t *x, uint32_t nx,
+                           uint64_t *s0, uint64_t *s1, uint64_t *s2, uint64_t *out, uint32_t wo, uint32_t *len)
+{
+    const int lane = threadIdx.x;
+    const uint32_t ncol = na && nb ? na + nb - 1 : 0, nout = (na && nb ? na + nb : 0) > nx ? na + nb : nx;
+    const uint32_t ntot = nout + 1 < wo ? nout + 1 : wo;            /* one limb for the carry of + x */
+    for (uint32_t k = lane; k < ntot + 2; k += 64) {
+        unsigned __int128 acc = k < nx ? x[k] : 0;
+        if (k < ncol) {
+            const uint32_t i0 = k + 1 > nb ? k + 1 - nb : 0, i1 = k < na - 1 ? k : na - 1;
+            for (uint32_t i = i0; i <= i1; i++) acc += (unsigned __int128)a[i] * b[k - i];
+        }
+        split3(acc, &s0[k], &s1[k], &s2[k]);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        uint64_t c = 0;
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < ntot; k++) {
+            uint64_t v = s0[k] + c + (k >= 1 ? s1[k - 1] : 0) + (k >= 2 ? s2[k - 2] : 0);
+            c = v >= 2 * NTTM_B ? 2 : v >= NTTM_B ? 1 : 0;
+            v -= c * NTTM_B;
+            out[k] = v;
+            if (v) n = k + 1;
+        }
+        *len = n;
+    }
+    __syncthreads();
+    for (uint32_t k = ntot + lane; k < wo; k += 64) out[k] = 0;
+}
+__global__ __launch_bounds__(64) void k_bs_school(const uint64_t *in, uint32_t w, const uint32_t *lens_in, uint64_t nin,
+                                                  uint64_t *out, uint32_t wo, uint32_t *lens_out)
+{
+    __shared__ uint64_t sPL[BS_MAXW], sQL[BS_MAXW], sPR[BS_MAXW], sQR[BS_MAXW];
+    __shared__ uint64_t s0[2 * BS_MAXW + 4], s1[2 * BS_MAXW + 4], s2[2 * BS_MAXW + 4];
+    const uint64_t pair = blockIdx.x, L = 2 * pair, R = L + 1;
+    const int lane = threadIdx.x;
+    uint64_t *po = out + 2 * (uint64_t)wo * pair, *qo = po + wo;
+    const uint64_t *pl = in + 2 * (uint64_t)w * L, *ql = pl + w;
+    const uint32_t pnL = lens_in[2 * L], qnL = lens_in[2 * L + 1];
+    if (R >= nin) {                                                /* the odd last node moves up */
+        for (uint32_t k = lane; k < wo; k += 64) { po[k] = k < pnL ? pl[k] : 0; qo[k] = k < qnL ? ql[k] : 0; }
+        if (lane == 0) { lens_out[2 * pair] = pnL; lens_out[2 * pair + 1] = qnL; }
+        return;
+    }
+    const uint64_t *pr = in + 2 * (uint64_t)w * R, *qr = pr + w;
+    const uint32_t pnR = lens_in[2 * R], qnR = lens_in[2 * R + 1];
+    for (uint32_t k = lane; k < w; k += 64) {
+        sPL[k] = k < pnL ? pl[k] : 0; sQL[k] = k < qnL ? ql[k] : 0;
+        sPR[k] = k < pnR ? pr[k] : 0; sQR[k] = k < qnR ? qr[k] : 0;
+    }
+    __syncthreads();
+    school_one(sQL, qnL, sQR, qnR, nullptr, 0, s0, s1, s2, qo, wo, &lens_out[2 * pair + 1]);
+    school_one(sPL, pnL, sQR, qnR, sPR, pnR, s0, s1, s2, po, wo, &lens_out[2 * pair]);
+}
+
 /* ---- the device backend of ebig.h ------------------------------------------------------------------------------------ */
 struct DBuf {
     uint64_t *p = nullptr;
@@ -269,26 +341,46 @@ static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
     if (hipMalloc(&lens, nleaf * 2 * 4) != hipSuccess) {
         (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "e pipeline: %llu leaves", (unsigned long long)nleaf);
     }
-    std::vector<uint32_t> hl(2 * nleaf);
     hipLaunchKernelGGL(k_e_seed, dim3((unsigned)((nleaf + 255) / 256)), dim3(256), 0, s, (uint64_t)N, G, nleaf, wl, pq.wptr(), lens);
     int rc = nttm_hip_check(hipGetLastError(), "k_e_seed launch");
-    if (!rc) rc = nttm_hip_check(hipMemcpyAsync(hl.data(), lens, nleaf * 2 * 4, hipMemcpyDeviceToHost, s), "copy leaf lengths");
-    if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "k_e_seed");
+    /* the lowest tree levels by schoolbook products while the nodes fit (30 -> 61 -> 123 -> 247 -> 495 limbs) */
+    uint64_t nodes = nleaf;
+    uint32_t w = wl;
+    while (!rc && nodes > 1 && w <= BS_MAXW) {
+        const uint64_t nout = (nodes + 1) / 2;
+        const uint32_t wo = 2 * w + 1;
+        DNum next;
+        uint32_t *lens2 = nullptr;
+        if ((rc = o.alloc(nout * 2 * wo, next))) break;
+        if (hipMalloc(&lens2, nout * 2 * 4) != hipSuccess) { (void)hipGetLastError(); rc = nttm_fail(NTTM_ENOMEM, "e pipeline: lengths"); break; }
+        hipLaunchKernelGGL(k_bs_school, dim3((unsigned)nout), dim3(64), 0, s, pq.ptr(), w, lens, nodes, next.wptr(), wo, lens2);
+        rc = nttm_hip_check(hipGetLastError(), "k_bs_school launch");
+        if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "k_bs_school");
+        (void)hipFree(lens);
+        lens = lens2;
+        pq = next;
+        nodes = nout;
+        w = wo;
+    }
+    std::vector<uint32_t> hl(2 * nodes);
+    if (!rc) rc = nttm_hip_check(hipMemcpyAsync(hl.data(), lens, nodes * 2 * 4, hipMemcpyDeviceToHost, s), "copy leaf lengths");
+    if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "e leaves");
     (void)hipFree(lens);
     if (rc) return rc;
     g_e_ms[0] = ms_since(t0);
     t0 = eclk::now();
     size_t cap = 2;
-    std::vector<nttm_bs_node> leaves(nleaf);
-    for (uint64_t i = 0; i < nleaf; i++) {
-        leaves[i].p = pq.ptr() + 2 * wl * i; leaves[i].pn = hl[2 * i];
-        leaves[i].q = pq.ptr() + 2 * wl * i + wl; leaves[i].qn = hl[2 * i + 1];
+    std::vector<nttm_bs_node> leaves(nodes);
+    for (uint64_t i = 0; i < nodes; i++) {
+        leaves[i].p = pq.ptr() + 2 * (uint64_t)w * i; leaves[i].pn = hl[2 * i];
+        leaves[i].q = pq.ptr() + 2 * (uint64_t)w * i + w; leaves[i].qn = hl[2 * i + 1];
         cap += hl[2 * i + 1];                                      /* |Q| <= sum of the leaves' |Q|; P < 2 Q */
     }
+    const uint64_t nleaves = nodes;
     RCK(o.alloc(cap, P));
     RCK(o.alloc(cap, Q));
     size_t pn = 0, qn = 0;
-    RCK(nttm_bs_tree(leaves.data(), nleaf, P.wptr(), cap, &pn, Q.wptr(), cap, &qn, s));
+    RCK(nttm_bs_tree(leaves.data(), nleaves, P.wptr(), cap, &pn, Q.wptr(), cap, &qn, s));
     P.n = pn; Q.n = qn;                                            /* normalized by the tree */
     g_e_ms[1] = ms_since(t0);
     return 0;
