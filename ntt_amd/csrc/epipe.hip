@@ -124,11 +124,13 @@ __device__ void school_one(const uint64_t *a, uint32_t na, const uint64_t *b, ui
     __syncthreads();
     for (uint32_t k = ntot + lane; k < wo; k += 64) out[k] = 0;
 }
+/* MAXW: the largest input node width the instance holds in LDS (the small instance keeps more waves per CU) */
+template <int MAXW>
 __global__ __launch_bounds__(64) void k_bs_school(const uint64_t *in, uint32_t w, const uint32_t *lens_in, uint64_t nin,
                                                   uint64_t *out, uint32_t wo, uint32_t *lens_out)
 {
-    __shared__ uint64_t sPL[BS_MAXW], sQL[BS_MAXW], sPR[BS_MAXW], sQR[BS_MAXW];
-    __shared__ uint64_t s0[2 * BS_MAXW + 4], s1[2 * BS_MAXW + 4], s2[2 * BS_MAXW + 4];
+    __shared__ uint64_t sPL[MAXW], sQL[MAXW], sPR[MAXW], sQR[MAXW];
+    __shared__ uint64_t s0[2 * MAXW + 4], s1[2 * MAXW + 4], s2[2 * MAXW + 4];
     const uint64_t pair = blockIdx.x, L = 2 * pair, R = L + 1;
     const int lane = threadIdx.x;
     uint64_t *po = out + 2 * (uint64_t)wo * pair, *qo = po + wo;
@@ -353,7 +355,10 @@ static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
         uint32_t *lens2 = nullptr;
         if ((rc = o.alloc(nout * 2 * wo, next))) break;
         if (hipMalloc(&lens2, nout * 2 * 4) != hipSuccess) { (void)hipGetLastError(); rc = nttm_fail(NTTM_ENOMEM, "e pipeline: lengths"); break; }
-        hipLaunchKernelGGL(k_bs_school, dim3((unsigned)nout), dim3(64), 0, s, pq.ptr(), w, lens, nodes, next.wptr(), wo, lens2);
+        if (w <= 64)
+            hipLaunchKernelGGL(k_bs_school<64>, dim3((unsigned)nout), dim3(64), 0, s, pq.ptr(), w, lens, nodes, next.wptr(), wo, lens2);
+        else
+            hipLaunchKernelGGL(k_bs_school<BS_MAXW>, dim3((unsigned)nout), dim3(64), 0, s, pq.ptr(), w, lens, nodes, next.wptr(), wo, lens2);
         rc = nttm_hip_check(hipGetLastError(), "k_bs_school launch");
         if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "k_bs_school");
         (void)hipFree(lens);
