@@ -109,18 +109,31 @@ __device__ void school_one(const uint64_t *a, uint32_t na, const uint64_t *b, ui
         split3(acc, &s0[k], &s1[k], &s2[k]);
     }
     __syncthreads();
-    if (lane == 0) {
-        uint64_t c = 0;
-        uint32_t n = 0;
-        for (uint32_t k = 0; k < ntot; k++) {
-            uint64_t v = s0[k] + c + (k >= 1 ? s1[k - 1] : 0) + (k >= 2 ? s2[k - 2] : 0);
-            c = v >= 2 * NTTM_B ? 2 : v >= NTTM_B ? 1 : 0;
-            v -= c * NTTM_B;
-            out[k] = v;
-            if (v) n = k + 1;
-        }
-        *len = n;
+    /* v_k = d0[k] + d1[k-1] + d2[k-2] < 2B + 351: q_k = floor(v_k / B) <= 2 goes to limb k+1 directly, and with
+     * w_k = v_k - q_k B + q_(k-1) <= B + 1 the rest is a one-bit carry o_k = [w_k + o_(k-1) >= B]: a 64-lane adder of
+     * the ballots X = [w >= B], Y = X | [w = B - 1] per word of 64 limbs, words in order (as in the CRT) */
+    uint32_t cin = 0, qprev = 0, n = 0;
+    for (uint32_t base = 0; base < ntot; base += 64) {
+        const uint32_t k = base + lane;
+        const uint64_t v = k < ntot ? s0[k] + (k >= 1 ? s1[k - 1] : 0) + (k >= 2 ? s2[k - 2] : 0) : 0;
+        const uint32_t q = v >= 2 * NTTM_B ? 2u : v >= NTTM_B ? 1u : 0u;
+        uint32_t qp = __shfl_up(q, 1, 64);
+        if (lane == 0) qp = qprev;
+        qprev = __builtin_amdgcn_readlane(q, 63);
+        const uint64_t wv = v - (uint64_t)q * NTTM_B + qp;
+        const uint64_t X = __ballot(k < ntot && wv >= NTTM_B), Y = X | __ballot(k < ntot && wv == NTTM_B - 1);
+        const uint64_t XY = X + Y, S = XY + cin;
+        const uint32_t cout = (uint32_t)(XY < X) | (uint32_t)(S < XY);            /* carry out of X + Y + cin */
+        const uint64_t CI = S ^ X ^ Y;                                               /* bit b: the carry into bit b */
+        const uint32_t oin = (uint32_t)(CI >> lane) & 1u;
+        const uint32_t oout = lane == 63 ? cout : (uint32_t)(CI >> (lane + 1)) & 1u;
+        const uint64_t limb = wv + oin - (oout ? NTTM_B : 0);
+        if (k < ntot) out[k] = limb;
+        const uint64_t nz = __ballot(k < ntot && limb != 0);
+        if (nz) n = base + 64 - __clzll((long long)nz);
+        cin = cout;
     }
+    if (lane == 0) *len = n;
     __syncthreads();
     for (uint32_t k = ntot + lane; k < wo; k += 64) out[k] = 0;
 }
