@@ -716,6 +716,9 @@ static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<si
     gb.off_crt = align_up(gb.off_ncn + cnt * 8, 256);
     gb.desc_bytes = align_up(gb.off_crt + nttm_crt_scratch_words(gb.nblk) * 4, 256);
 }
+/* points per thread of a batch group's passes: the configured value, else 8 up to 2^14 points and 16 above (what the
+ * batch plans' autotuner chose for every such group of the 4096-leaf workload) */
+static int batch_pts(int logn) { return g.cfg.pts ? g.cfg.pts : logn <= 14 ? 8 : 16; }
 /* the group's descriptors as one host image (planes at A), to copy to the group's descriptor base */
 static void group_image(GroupBuild &gb, double *A, char *img)
 {
@@ -778,7 +781,7 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
         if (G.empty()) continue;
         const int r3 = gi > NTTM_LOGN_MAX, logn = (int)(gi % (NTTM_LOGN_MAX + 1));
         GroupBuild gb;
-        group_build(gb, P, G, np, r3, logn, g.cfg.tile_log, nttm_get_pts());
+        group_build(gb, P, G, np, r3, logn, g.cfg.tile_log, batch_pts(logn));
         const size_t off_d = align_up(gb.planes_bytes, 256);
         RCK(ws_need(off_d + gb.desc_bytes + 256));
         RCK(hpin_need(gb.desc_bytes + 4096));
@@ -1169,7 +1172,7 @@ static int tune_group(const nttm_prod *P, const std::vector<size_t> &G, int np, 
 {
     static const int cand[4][2] = {{12, 16}, {13, 16}, {12, 8}, {13, 8}};
     *ms = 0;
-    if (!tune) { group_build(best, P, G, np, r3, logn, g.cfg.tile_log, nttm_get_pts()); return 0; }
+    if (!tune) { group_build(best, P, G, np, r3, logn, g.cfg.tile_log, batch_pts(logn)); return 0; }
     double best_ms = 1e30;
     hipEvent_t e0 = g.ev[0], e1 = g.ev[1];
     for (int c = 0; c < 4; c++) {
