@@ -145,3 +145,14 @@ def test_product_plan_shape():
     # the 2^28 transform of the 2^27-limb product: two strided 8-bit passes (32-column tiles) and a 12-bit last pass
     assert ntt_amd.debug_plan(28, 1, 12, 13) == [(20, 8, 5, 0), (12, 8, 5, 0), (0, 12, 0, 0)]
     assert ntt_amd.debug_plan(28, 1, 12, 12) == [(20, 8, 4, 0), (12, 8, 4, 0), (0, 12, 0, 0)]
+
+
+def test_limbs_to_ascii_formats_18_digits_per_limb():
+    """the digit formatter of the wrapper (the text form of e's digits in the tests and the bench): top limb first, 18
+    zero-padded digits per limb, against Python's own formatting"""
+    rng = np.random.default_rng(4)
+    limbs = np.concatenate([np.array([0, 1, 10 ** 17, 10 ** 18 - 1, 999999999, 10 ** 9], dtype=np.uint64),
+                            rng.integers(0, 10 ** 18, 1000, dtype=np.uint64)])
+    want = "".join("%018d" % int(v) for v in limbs[::-1]).encode()
+    assert ntt_amd.limbs_to_ascii(limbs) == want
+    assert ntt_amd.limbs_to_ascii(np.zeros(0, dtype=np.uint64)) == b""
