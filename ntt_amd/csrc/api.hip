@@ -32,8 +32,8 @@ int nttm_run_load_planes(double *dst, uint64_t plane, const uint64_t *src, uint6
 int nttm_run_load_batch(double *dst, uint64_t plane, const void *desc, int which, uint64_t count, int logn, int np,
                         const PrimeK *pk, const double2 *scale, hipStream_t s);
 int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks, int np, const CrtConst &cc,
-                          uint32_t *scratch, hipStream_t s, double alg_bytes);
-uint64_t nttm_crt_blocks(uint64_t nc);
+                          uint32_t *scratch, hipStream_t s, double alg_bytes, int per = 8);
+uint64_t nttm_crt_blocks(uint64_t nc, int per = 8);      /* per: CRT limbs per thread, 8 (2048-limb blocks) or 1 (256) */
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 int nttm_run_norm(const void *desc_dev, uint64_t nprod, uint64_t *ncn_dev, hipStream_t s);
 int nttm_run_fill_x(const void *desc_dev, uint64_t nprod, hipStream_t s);
@@ -657,6 +657,7 @@ extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
  * descriptor base), for a given schedule (length, strided-pass tile, points per thread) */
 struct GroupBuild {
     int r3 = 0, logn = 0, tile = 0, pts = 0;
+    int crt_per = 8;                                             /* CRT limbs per thread: 1 when every product is small */
     uint64_t n = 0, cnt = 0, cnt_b = 0, nblk = 0;
     bool share = false;
     std::vector<CrtDesc> cd;
@@ -672,6 +673,7 @@ static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<si
     gb.r3 = r3; gb.logn = logn; gb.tile = tile; gb.pts = pts;
     const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn, cnt = G.size();          /* points per product plane */
     gb.n = n; gb.cnt = cnt; gb.nblk = 0; gb.crt_bytes = 0;
+    gb.crt_per = n <= 1024 ? 1 : 8;                              /* products of at most n + 1 limbs: 256-limb blocks */
     gb.cd.assign(cnt, CrtDesc()); gb.ld.assign(cnt, LoadDesc()); gb.ldb.clear(); gb.bmap.clear();
     for (uint64_t j = 0; j < cnt; j++) {
         const nttm_prod &q = P[G[j]];
@@ -680,7 +682,7 @@ static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<si
         l.a = q.a; l.na = q.na; l.b = q.b; l.nb = q.nb;
         c.pl = nullptr; c.plane = cnt * n; c.ncoef = q.na + q.nb - 1; c.x = q.x; c.nx = q.x ? q.nx : 0;
         c.c = q.c; c.nc = q.na + q.nb + (q.x ? 1 : 0); c.first_block = gb.nblk; c.trunc = 0;
-        gb.nblk += nttm_crt_blocks(c.nc);
+        gb.nblk += nttm_crt_blocks(c.nc, gb.crt_per);
         gb.crt_bytes += 8.0 * np * c.ncoef + 8.0 * c.nc + 8.0 * c.nx;
     }
     /* products that share their B operand (the same b and nb: the binary-splitting tree's P1 Q2 + P2 and Q1 Q2,
@@ -741,7 +743,7 @@ static int group_launch(const GroupBuild &gb, int np, double *A, char *dbase, hi
                                 s, nullptr, gb.cnt_b, gb.share ? (const LoadDesc *)(dbase + gb.off_ldb) : nullptr,
                                 gb.share ? (const uint32_t *)(dbase + gb.off_bmap) : nullptr);
     if (!rc) rc = nttm_crt_launch_batch(dbase + gb.off_cd, gb.cnt, gb.nblk, np, g.cc, (uint32_t *)(dbase + gb.off_crt), s,
-                                        gb.crt_bytes);
+                                        gb.crt_bytes, gb.crt_per);
     if (!rc) rc = nttm_run_norm(dbase + gb.off_cd, gb.cnt, (uint64_t *)(dbase + gb.off_ncn), s);
     g.cfg.tile_log = tile0; nttm_set_pts(pts0);
     return rc;

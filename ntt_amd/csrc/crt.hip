@@ -22,8 +22,8 @@
 #include "internal.h"
 
 #define CRT_T 256
-#define CRT_PER 8
-#define CRT_CH (CRT_T * CRT_PER)
+#define CRT_WIDE 8                       /* limbs per thread of a block (2048-limb blocks); groups of small products
+                                            use 1 (256-limb blocks, less of a block idle past a product's end) */
 #define CRT_HALO 4                       /* digits of coefficients k0-4 .. k0-1: v_{k0-1} with four primes */
 #define CRT_ID_MAP 0x2u                  /* one-bit carry maps: bit c = cout(cin = c); the identity is 0b10 */
 #define CRT_CTL_WORDS 4                  /* control words before the block status words (16-byte aligned) */
@@ -116,10 +116,11 @@ __device__ __forceinline__ void crt_digits(const CrtConst &cc, const double *r, 
     d[NP - 1] = acc[NP - 1] + c;
 }
 
-template <int NP>
+template <int NP, int PER>
 __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
 {
-    __shared__ uint64_t D[NP][CRT_CH + CRT_HALO];
+    constexpr int CH = CRT_T * PER;                                  /* limbs per block */
+    __shared__ uint64_t D[NP][CH + CRT_HALO];
     __shared__ uint32_t s_prod, s_err;
     __shared__ uint32_t s_wave[CRT_T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -132,20 +133,20 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
     __syncthreads();
     const CrtDesc dsc = a.desc[s_prod];
     const uint32_t first = (uint32_t)dsc.first_block, local = blk - first;
-    const uint64_t k0 = (uint64_t)local * CRT_CH;
+    const uint64_t k0 = (uint64_t)local * CH;
 
     /* digits of coefficients k0 - HALO .. k0 + CH - 1 into LDS (coalesced plane reads, all of a thread's residues
      * requested before the first is used) */
-    uint64_t xk[CRT_PER];
+    uint64_t xk[PER];
     {
         const bool halo = tid < CRT_HALO && k0 + tid >= CRT_HALO;   /* coefficient k0 - HALO + tid exists */
         /* the planes as global-space pointers: generic (flat) loads would also count against the LDS counter, whose
          * limit of 15 outstanding operations would serialize the prefetch */
         const __attribute__((address_space(1))) double *pl = (const __attribute__((address_space(1))) double *)dsc.pl;
         const __attribute__((address_space(1))) uint64_t *xg = (const __attribute__((address_space(1))) uint64_t *)dsc.x;
-        double rr[CRT_PER + 1][NP];
+        double rr[PER + 1][NP];
 #pragma unroll
-        for (int i = 0; i < CRT_PER; i++) {
+        for (int i = 0; i < PER; i++) {
             const uint64_t kk = k0 + i * CRT_T + tid;
 #pragma unroll
             for (int j = 0; j < NP; j++) rr[i][j] = kk < dsc.ncoef ? pl[j * dsc.plane + kk] : 0.0;
@@ -153,15 +154,15 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         {
             const uint64_t kk = k0 + tid - CRT_HALO;
 #pragma unroll
-            for (int j = 0; j < NP; j++) rr[CRT_PER][j] = halo && kk < dsc.ncoef ? pl[j * dsc.plane + kk] : 0.0;
+            for (int j = 0; j < NP; j++) rr[PER][j] = halo && kk < dsc.ncoef ? pl[j * dsc.plane + kk] : 0.0;
         }
 #pragma unroll
-        for (int i = 0; i < CRT_PER; i++) {                          /* x at the limbs of the limb phase below */
-            const uint64_t k = k0 + wave * 64 * CRT_PER + i * 64 + lane;
+        for (int i = 0; i < PER; i++) {                          /* x at the limbs of the limb phase below */
+            const uint64_t k = k0 + wave * 64 * PER + i * 64 + lane;
             xk[i] = k < dsc.nx && k < dsc.nc ? xg[k] : 0;
         }
 #pragma unroll
-        for (int i = 0; i < CRT_PER; i++) {
+        for (int i = 0; i < PER; i++) {
             uint64_t d[NP];
             crt_digits<NP>(a.cc, rr[i], d);                          /* zero residues give zero digits */
 #pragma unroll
@@ -169,14 +170,14 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         }
         if (tid < CRT_HALO) {
             uint64_t d[NP];
-            crt_digits<NP>(a.cc, rr[CRT_PER], d);
+            crt_digits<NP>(a.cc, rr[PER], d);
 #pragma unroll
             for (int j = 0; j < NP; j++) D[j][tid] = d[j];
         }
     }
     __syncthreads();
 
-    /* limb k = k0 + wave 512 + i 64 + lane for i < 8: one ballot per i gives 64 consecutive limbs */
+    /* limb k = k0 + wave 64 PER + i 64 + lane for i < PER: one ballot per i gives 64 consecutive limbs */
     /* v_k = sum_j d_j[k - j] + x_k < NP B + 2^31: its quotient by B is at most NP */
     auto vsum = [&](uint64_t k, int li, uint64_t xv) -> uint64_t {
         uint64_t v = 0;
@@ -193,12 +194,12 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         for (int c = 1; c <= NP; c++) q += v >= (uint64_t)c * NTTM_B;
         return q;
     };
-    uint64_t wv[CRT_PER];
-    uint64_t GX[CRT_PER], GY[CRT_PER];
+    uint64_t wv[PER];
+    uint64_t GX[PER], GY[PER];
     uint32_t qlast = 0, qprev = 0;
 #pragma unroll
-    for (int i = 0; i < CRT_PER; i++) {
-        const int off = wave * 64 * CRT_PER + i * 64 + lane;          /* offset of limb k inside the block */
+    for (int i = 0; i < PER; i++) {
+        const int off = wave * 64 * PER + i * 64 + lane;          /* offset of limb k inside the block */
         const uint64_t k = k0 + off;
         const int li = CRT_HALO + off;
         const uint64_t v = vsum(k, li, xk[i]);
@@ -214,10 +215,10 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
         GX[i] = g; GY[i] = g | pr;
         if (k + 1 == dsc.nc) qlast = q;
     }
-    /* the wave's map cin -> cout over its 8 words */
+    /* the wave's map cin -> cout over its PER words */
     uint32_t F = CRT_ID_MAP;
 #pragma unroll
-    for (int i = 0; i < CRT_PER; i++) {
+    for (int i = 0; i < PER; i++) {
         const uint32_t f = add_cout(GX[i], GY[i], 0) | (add_cout(GX[i], GY[i], 1) << 1);
         F = map_compose(f, F);
     }
@@ -234,13 +235,13 @@ __global__ __launch_bounds__(CRT_T) void k_crt(CrtArgs a)
     __attribute__((address_space(1))) uint64_t *cg = (__attribute__((address_space(1))) uint64_t *)dsc.c;
     uint32_t c = map_apply(P, 0), c1 = map_apply(P, 1);              /* the carries into this wave's first word */
 #pragma unroll
-    for (int i = 0; i < CRT_PER; i++) {
+    for (int i = 0; i < PER; i++) {
         const uint64_t S = GX[i] + GY[i] + c;
         const uint32_t cout = add_cout(GX[i], GY[i], c);
         const uint64_t CI = S ^ GX[i] ^ GY[i];                      /* bit b: the carry into bit b (o_{k-1}) */
         const uint32_t oin = (uint32_t)(CI >> lane) & 1u;
         const uint32_t oout = lane == 63 ? cout : (uint32_t)(CI >> (lane + 1)) & 1u;
-        const int off = wave * 64 * CRT_PER + i * 64 + lane;
+        const int off = wave * 64 * PER + i * 64 + lane;
         const uint64_t k = k0 + off;
         if (k < dsc.nc) cg[k] = wv[i] + oin - (oout ? NTTM_B : 0);
         const uint32_t cout1 = add_cout(GX[i], GY[i], c1);
@@ -353,10 +354,12 @@ __global__ __launch_bounds__(CRT_SCAN_T) void k_crt_scan(uint32_t *status, uint6
 
 /* the limbs of a block whose carry in is 1: its stored limbs (for carry 0) plus one, i.e. the leading limbs equal to
  * B - 1 become 0 and the first other one grows by 1.  A workgroup scouts 64 status words; each of its 4 waves takes
- * every 4th block found and rewrites it (32 limbs per lane, all loads in flight at once). */
+ * every 4th block found and rewrites it (CH / 64 limbs per lane, all loads in flight at once). */
 #define CRT_FIX_W 64
+template <int PER>
 __global__ __launch_bounds__(256) void k_crt_fix(const CrtDesc *desc, uint64_t nprod, const uint32_t *status, uint64_t n)
 {
+    constexpr int CH = CRT_T * PER;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t b0 = (uint64_t)blockIdx.x * CRT_FIX_W;
     const uint32_t st = b0 + lane < n ? status[b0 + lane] : 0u;
@@ -369,18 +372,18 @@ __global__ __launch_bounds__(256) void k_crt_fix(const CrtDesc *desc, uint64_t n
         uint64_t lo = 0, hi = nprod - 1;
         while (lo < hi) { const uint64_t mid = (lo + hi + 1) >> 1; if (desc[mid].first_block <= blk) lo = mid; else hi = mid - 1; }
         const CrtDesc d = desc[lo];
-        const uint64_t k0 = (blk - d.first_block) * CRT_CH, end = k0 + CRT_CH < d.nc ? k0 + CRT_CH : d.nc;
-        uint64_t v[CRT_CH / 64];
+        const uint64_t k0 = (blk - d.first_block) * CH, end = k0 + CH < d.nc ? k0 + CH : d.nc;
+        uint64_t v[CH / 64];
 #pragma unroll
-        for (int j = 0; j < CRT_CH / 64; j++) { const uint64_t k = k0 + j * 64 + lane; v[j] = k < end ? d.c[k] : 0; }
-        int first_ne = CRT_CH;                                      /* offset of the first limb below B - 1 */
+        for (int j = 0; j < CH / 64; j++) { const uint64_t k = k0 + j * 64 + lane; v[j] = k < end ? d.c[k] : 0; }
+        int first_ne = CH;                                      /* offset of the first limb below B - 1 */
 #pragma unroll
-        for (int j = CRT_CH / 64 - 1; j >= 0; j--) {
+        for (int j = CH / 64 - 1; j >= 0; j--) {
             const uint64_t ne = __ballot(k0 + j * 64 + lane < end && v[j] != NTTM_B - 1);
             if (ne) first_ne = j * 64 + __ffsll((long long)ne) - 1;
         }
 #pragma unroll
-        for (int j = 0; j < CRT_CH / 64; j++) {
+        for (int j = 0; j < CH / 64; j++) {
             const int o = j * 64 + lane;
             const uint64_t k = k0 + o;
             if (k < end && o <= first_ne) d.c[k] = o < first_ne ? 0 : v[j] + 1;
@@ -435,15 +438,17 @@ __global__ void k_norm(const CrtDesc *d, uint64_t nprod, uint64_t *ncn)
     ncn[i] = n;
 }
 
-uint64_t nttm_crt_blocks(uint64_t nc) { return (nc + CRT_CH - 1) / CRT_CH; }
+/* blocks of a product of nc limbs at per limbs per thread (CRT_WIDE, or 1) */
+uint64_t nttm_crt_blocks(uint64_t nc, int per) { const uint64_t ch = (uint64_t)CRT_T * (per == 1 ? 1 : CRT_WIDE); return (nc + ch - 1) / ch; }
 
 uint64_t nttm_crt_scratch_words(uint64_t nblocks) { return nblocks + CRT_CTL_WORDS; }
 
 /* desc_dev: nprod descriptors in device memory whose block ranges tile [0, nblocks); scratch: 16-byte aligned,
  * nttm_crt_scratch_words(nblocks) words, of which word 1 receives the error flags */
 int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks, int np, const CrtConst &cc,
-                          uint32_t *scratch, hipStream_t s, double alg_bytes)
+                          uint32_t *scratch, hipStream_t s, double alg_bytes, int per)
 {
+    if (per != 1 && per != CRT_WIDE) return nttm_fail(NTTM_EINVAL, "crt: %d limbs per thread", per);
     if (np < 3 || np > NTTM_MAX_PRIMES) return nttm_fail(NTTM_EINVAL, "crt: np = %d", np);
     if (!nprod || !nblocks) return 0;
     if (nblocks > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "crt: too many blocks");
@@ -452,8 +457,13 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
     a.desc = (const CrtDesc *)desc_dev; a.nprod = nprod;
     a.ctl = scratch; a.status = scratch + CRT_CTL_WORDS; a.cc = cc;
     prof_begin("k_crt", alg_bytes, s);
-    if (np == 3) hipLaunchKernelGGL(k_crt<3>, dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
-    else hipLaunchKernelGGL(k_crt<4>, dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
+    if (per == 1) {
+        if (np == 3) hipLaunchKernelGGL((k_crt<3, 1>), dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
+        else hipLaunchKernelGGL((k_crt<4, 1>), dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
+    } else {
+        if (np == 3) hipLaunchKernelGGL((k_crt<3, CRT_WIDE>), dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
+        else hipLaunchKernelGGL((k_crt<4, CRT_WIDE>), dim3((unsigned)nblocks), dim3(CRT_T), 0, s, a);
+    }
     prof_end(s);
     int rc = nttm_hip_check(hipGetLastError(), "k_crt launch");
     if (rc) return rc;
@@ -463,8 +473,9 @@ int nttm_crt_launch_batch(const void *desc_dev, uint64_t nprod, uint64_t nblocks
     rc = nttm_hip_check(hipGetLastError(), "k_crt_scan launch");
     if (rc) return rc;
     prof_begin("k_crt_fix", 4.0 * (double)nblocks, s);
-    hipLaunchKernelGGL(k_crt_fix, dim3((unsigned)((nblocks + CRT_FIX_W - 1) / CRT_FIX_W)), dim3(256), 0, s, a.desc, nprod,
-                       (const uint32_t *)a.status, nblocks);
+    const dim3 gfix((unsigned)((nblocks + CRT_FIX_W - 1) / CRT_FIX_W));
+    if (per == 1) hipLaunchKernelGGL((k_crt_fix<1>), gfix, dim3(256), 0, s, a.desc, nprod, (const uint32_t *)a.status, nblocks);
+    else hipLaunchKernelGGL((k_crt_fix<CRT_WIDE>), gfix, dim3(256), 0, s, a.desc, nprod, (const uint32_t *)a.status, nblocks);
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_crt_fix launch");
 }
@@ -490,13 +501,14 @@ __device__ __forceinline__ uint64_t add_limb(const uint64_t *p, uint64_t n, uint
 }
 __global__ __launch_bounds__(CRT_T) void k_addsub(AddArgs a)
 {
+    constexpr int PER = CRT_WIDE, CH = CRT_T * PER;
     __shared__ uint32_t s_wave[CRT_T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t blk = blockIdx.x, k0 = blk * CRT_CH;
-    uint64_t wv[CRT_PER], GX[CRT_PER], GY[CRT_PER];
+    const uint64_t blk = blockIdx.x, k0 = blk * CH;
+    uint64_t wv[PER], GX[PER], GY[PER];
 #pragma unroll
-    for (int i = 0; i < CRT_PER; i++) {
-        const uint64_t k = k0 + wave * 64 * CRT_PER + i * 64 + lane;
+    for (int i = 0; i < PER; i++) {
+        const uint64_t k = k0 + wave * 64 * PER + i * 64 + lane;
         uint64_t w = NTTM_B - 1;
         if (k < a.nout) {
             const uint64_t x = a.apow >= 0 ? (k == (uint64_t)a.apow ? 1u : 0u) : add_limb(a.a, a.na, a.sa, k);
@@ -509,7 +521,7 @@ __global__ __launch_bounds__(CRT_T) void k_addsub(AddArgs a)
     }
     uint32_t F = CRT_ID_MAP;
 #pragma unroll
-    for (int i = 0; i < CRT_PER; i++) F = map_compose(add_cout(GX[i], GY[i], 0) | (add_cout(GX[i], GY[i], 1) << 1), F);
+    for (int i = 0; i < PER; i++) F = map_compose(add_cout(GX[i], GY[i], 0) | (add_cout(GX[i], GY[i], 1) << 1), F);
     if (lane == 0) s_wave[wave] = F;
     __syncthreads();
     uint32_t P = CRT_ID_MAP, A = CRT_ID_MAP;
@@ -519,13 +531,13 @@ __global__ __launch_bounds__(CRT_T) void k_addsub(AddArgs a)
     }
     uint32_t c = map_apply(P, 0);
 #pragma unroll
-    for (int i = 0; i < CRT_PER; i++) {
+    for (int i = 0; i < PER; i++) {
         const uint64_t S = GX[i] + GY[i] + c;
         const uint32_t cout = add_cout(GX[i], GY[i], c);
         const uint64_t CI = S ^ GX[i] ^ GY[i];
         const uint32_t oin = (uint32_t)(CI >> lane) & 1u;
         const uint32_t oout = lane == 63 ? cout : (uint32_t)(CI >> (lane + 1)) & 1u;
-        const uint64_t k = k0 + wave * 64 * CRT_PER + i * 64 + lane;
+        const uint64_t k = k0 + wave * 64 * PER + i * 64 + lane;
         if (k < a.nout) a.r[k] = wv[i] + oin - (oout ? NTTM_B : 0);
         c = cout;
     }
@@ -538,10 +550,11 @@ __global__ __launch_bounds__(CRT_T) void k_addsub(AddArgs a)
 /* the blocks whose carry in is 1: their limbs plus one (a ripple through the leading limbs equal to B - 1) */
 __global__ __launch_bounds__(64) void k_addsub_fix(uint64_t *r, uint64_t nout, const uint32_t *status, uint64_t nblocks)
 {
+    constexpr int CH = CRT_T * CRT_WIDE;
     const int lane = threadIdx.x;
     const uint64_t blk = blockIdx.x;
     if (blk >= nblocks || !(status[blk] & CRT_ST_CIN)) return;
-    const uint64_t k0 = blk * CRT_CH, end = k0 + CRT_CH < nout ? k0 + CRT_CH : nout;
+    const uint64_t k0 = blk * CH, end = k0 + CH < nout ? k0 + CH : nout;
     for (uint64_t base = k0; base < end; base += 64) {
         const uint64_t k = base + lane;
         const uint64_t v = k < end ? r[k] : 0;
@@ -555,7 +568,7 @@ int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, 
                     uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s)
 {
     if (!nout) return 0;
-    const uint64_t nblocks = (nout + CRT_CH - 1) / CRT_CH;
+    const uint64_t nblocks = nttm_crt_blocks(nout, CRT_WIDE);
     if (nblocks > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "addsub: too many blocks");
     if ((uintptr_t)scratch & 15) return nttm_fail(NTTM_EINVAL, "addsub: scratch not 16-byte aligned");
     AddArgs A;

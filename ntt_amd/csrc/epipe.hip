@@ -30,7 +30,7 @@
 int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, uint64_t sa, int64_t apow, const uint64_t *b,
                     uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s);
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
-uint64_t nttm_crt_blocks(uint64_t nc);
+uint64_t nttm_crt_blocks(uint64_t nc, int per = 8);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 
 /* leaf i folds the terms [i G, min((i + 1) G, nterms)): P = sum_j prod_{k > j} (k + 1), Q = prod (k + 1); written as
