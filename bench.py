@@ -114,6 +114,23 @@ def cpu_baseline(a, b, log_hi, reps):
             "sample_s": {"2^%d" % k: round(v, 4) for k, v in t.items()}}
 
 
+def cpu_leaves_baseline(ops, gpu_products_per_s):
+    """GMP mpz_mul over the whole leaf workload on 1 thread and on every host thread (one product per thread at a
+    time, dynamic schedule), conversions off the clock"""
+    import oracle
+    if not oracle.have_gmp():
+        return None
+    model, ncpu = host_info()
+    threads = min(ncpu or 1, 16)
+    t1 = oracle.gmp_time_batch(ops, 1, 1)
+    tn = oracle.gmp_time_batch(ops, threads, 1)
+    return {"kind": "port", "products": len(ops), "gmp_1_thread_s": round(t1, 4),
+            "gmp_1_thread_products_per_s": round(len(ops) / t1, 1), "threads": threads,
+            "gmp_all_threads_s": round(tn, 4), "gmp_all_threads_products_per_s": round(len(ops) / tn, 1),
+            "gpu_graph_over_gmp_all_threads": round(gpu_products_per_s * tn / len(ops), 1),
+            "sample": "the full 4096-product leaf workload, mpz_mul only, host %s" % model}
+
+
 def dominant(ks):
     name = max(ks, key=lambda k: ks[k][1])
     launches, ms, byts = ks[name]
@@ -129,7 +146,7 @@ def load_traffic(name):
         return None
 
 
-def bench_leaves(nttm, reps, each=True):
+def bench_leaves(nttm, reps, each=True, cpu=False):
     import torch
     ops = leaf_workload()
     items = []
@@ -165,6 +182,9 @@ def bench_leaves(nttm, reps, each=True):
            "graph_products_per_s": round(len(ops) / t_graph, 1), "graph_ms": round(1e3 * t_graph, 3),
            "graph_plan_autotune_s": round(t_plan_make, 3), "graph_equals_batch": bool(graph_same),
            "graph_schedule": sched}
+    if cpu:
+        print("[bench] leaves cpu baseline (GMP)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_leaves_baseline(ops, len(ops) / t_graph)
     if not each:
         return out
     t0 = time.perf_counter()
@@ -507,7 +527,7 @@ def main():
         print("[bench] leaves", file=sys.stderr, flush=True)
         del dc
         torch.cuda.empty_cache()
-        leaves = bench_leaves(nttm, args.leaf_reps)
+        leaves = bench_leaves(nttm, args.leaf_reps, cpu=not args.no_cpu_baseline)
         print("[bench] binary-splitting level", file=sys.stderr, flush=True)
         leaves["binsplit_level"] = bench_binsplit(nttm, args.leaf_reps)
 

@@ -76,6 +76,9 @@ def lib():
         L.orc_gmp_tref_edges.restype = C.c_int
         L.orc_gmp_check.argtypes = [_u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t]
         L.orc_gmp_check.restype = C.c_int
+        L.orc_gmp_time_batch.argtypes = [C.POINTER(_u64p), C.POINTER(C.c_size_t), C.POINTER(_u64p), C.POINTER(C.c_size_t),
+                                         C.c_size_t, C.c_int, C.c_int]
+        L.orc_gmp_time_batch.restype = C.c_double
         L.orc_pack60_bytes.argtypes = [C.c_size_t]; L.orc_pack60_bytes.restype = C.c_uint64
         L.orc_pack60.argtypes = [_u64p, C.c_size_t, C.c_void_p]; L.orc_pack60.restype = None
         L.orc_unpack60.argtypes = [C.c_void_p, C.c_size_t, _u64p]; L.orc_unpack60.restype = None
@@ -290,6 +293,17 @@ def gmp_check(a, b, c, x=None):
 def gmp_time_mul(a, b, reps=3):
     a = np.ascontiguousarray(a, dtype=np.uint64); b = np.ascontiguousarray(b, dtype=np.uint64)
     return float(lib().orc_gmp_time_mul(_p(a), len(a), _p(b), len(b), reps))
+
+
+def gmp_time_batch(pairs, threads=1, reps=1):
+    """seconds of mpz_mul over every (a, b) of pairs on `threads` OpenMP threads, conversions off the clock"""
+    arrs = [(np.ascontiguousarray(a, dtype=np.uint64), np.ascontiguousarray(b, dtype=np.uint64)) for a, b in pairs]
+    n = len(arrs)
+    pa = (_u64p * n)(*[_p(a) for a, _ in arrs])
+    pb = (_u64p * n)(*[_p(b) for _, b in arrs])
+    na = (C.c_size_t * n)(*[len(a) for a, _ in arrs])
+    nb = (C.c_size_t * n)(*[len(b) for _, b in arrs])
+    return float(lib().orc_gmp_time_batch(pa, na, pb, nb, n, threads, reps))
 
 
 def tref_res(a, b, c):

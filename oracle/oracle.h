@@ -102,6 +102,10 @@ int orc_have_gmp(void);
 int orc_gmp_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb);
 /* seconds of mpz_mul alone on the converted operands, median of reps; -1 without GMP */
 double orc_gmp_time_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, int reps);
+/* seconds of n mpz_mul calls a[i] b[i] on `threads` OpenMP threads, conversions off the clock, median of reps; -1
+ * without GMP */
+double orc_gmp_time_batch(const uint64_t *const *a, const size_t *na, const uint64_t *const *b, const size_t *nb, size_t n,
+                          int threads, int reps);
 /* the residue form's parts (3) low 64 limbs exact and (4) top limbs within [AhBh, AhBh + Ah + Bh]; 1 if c passes */
 int orc_gmp_tref_edges(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc);
 /* 1 if c equals a b + x exactly (x optional), by GMP */

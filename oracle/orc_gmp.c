@@ -122,6 +122,37 @@ double orc_gmp_time_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t 
     return t[reps / 2];
 }
 
+/* n products a[i] b[i]: seconds of the mpz_mul loop alone (operands converted and results sized beforehand), on
+ * `threads` OpenMP threads (dynamic schedule), median of reps */
+double orc_gmp_time_batch(const uint64_t *const *a, const size_t *na, const uint64_t *const *b, const size_t *nb, size_t n,
+                          int threads, int reps)
+{
+    if (reps < 1) reps = 1;
+    if (reps > 64) reps = 64;
+    if (threads < 1) threads = 1;
+    mpz_t *x = (mpz_t *)malloc(3 * n * sizeof(mpz_t));
+    if (!x) return -1.0;
+    mpz_t *y = x + n, *z = y + n;
+#pragma omp parallel for schedule(dynamic, 1)    /* conversions on every host thread (the power cache is locked) */
+    for (long i = 0; i < (long)n; i++) {
+        mpz_inits(x[i], y[i], z[i], NULL);
+        mpz_from_limbs(x[i], a[i], na[i]);
+        mpz_from_limbs(y[i], b[i], nb[i]);
+        mpz_mul(z[i], x[i], y[i]);                              /* warm: every result allocated at full size */
+    }
+    double t[64];
+    for (int r = 0; r < reps; r++) {
+        const double t0 = now_s();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+        for (long i = 0; i < (long)n; i++) mpz_mul(z[i], x[i], y[i]);
+        t[r] = now_s() - t0;
+    }
+    for (size_t i = 0; i < n; i++) mpz_clears(x[i], y[i], z[i], NULL);
+    free(x);
+    qsort(t, reps, sizeof t[0], cmp_d);
+    return t[reps / 2];
+}
+
 int orc_gmp_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *xx, size_t nx,
                   const uint64_t *c, size_t nc)
 {
@@ -170,6 +201,9 @@ int orc_gmp_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, si
 { (void)c; (void)a; (void)na; (void)b; (void)nb; return -1; }
 double orc_gmp_time_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, int reps)
 { (void)a; (void)na; (void)b; (void)nb; (void)reps; return -1.0; }
+double orc_gmp_time_batch(const uint64_t *const *a, const size_t *na, const uint64_t *const *b, const size_t *nb, size_t n,
+                          int threads, int reps)
+{ (void)a; (void)na; (void)b; (void)nb; (void)n; (void)threads; (void)reps; return -1.0; }
 int orc_gmp_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
                   const uint64_t *c, size_t nc)
 { (void)a; (void)na; (void)b; (void)nb; (void)x; (void)nx; (void)c; (void)nc; return -1; }

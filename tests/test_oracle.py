@@ -299,6 +299,9 @@ def test_gmp_bridge_and_checker():
     assert O.gmp_check(a, b, O.mul_school(a, b, x), x)
     assert O.tref_res(a, b, c) and O.tref_edges(a, b, c)
     assert O.gmp_time_mul(a, b, 3) > 0
+    pairs = [O.gen_limbs(40 + 30 * i, 100 + i)[0:1] + O.gen_limbs(500 - 20 * i, 200 + i)[0:1] for i in range(20)]
+    assert O.gmp_time_batch(pairs, 1, 3) > 0 and O.gmp_time_batch(pairs, 4, 1) > 0
+    assert O.gmp_time_batch([], 2, 1) >= 0
 
 
 def test_checker_catches_single_limb_faults():
