@@ -351,7 +351,9 @@ def replay_summary(db, replays=3):
     ks = [r for r in rows if r[0].startswith("k_") or r[0].startswith("void k_")]
     names = [r[0] for r in ks]
     period = None
-    for R in range(1, len(ks) // replays + 1):
+    # the longest period whose last `replays` runs repeat: a shorter one can be a run of identical length groups inside
+    # one replay, while twice the true period needs twice as many replays in the trace
+    for R in range(len(ks) // replays, 0, -1):
         tail = names[len(names) - replays * R:]
         if all(tail[i * R:(i + 1) * R] == tail[:R] for i in range(replays)):
             period = R
