@@ -130,6 +130,7 @@ double orc_gmp_time_batch(const uint64_t *const *a, const size_t *na, const uint
     if (reps < 1) reps = 1;
     if (reps > 64) reps = 64;
     if (threads < 1) threads = 1;
+    if (!n) return 0.0;
     mpz_t *x = (mpz_t *)malloc(3 * n * sizeof(mpz_t));
     if (!x) return -1.0;
     mpz_t *y = x + n, *z = y + n;
