@@ -418,7 +418,31 @@ def traffic_summary(fetch_db, write_db):
     print(json.dumps(out))
 
 
+def counter_summary(db):
+    """every counter of a rocprofv3 --pmc run averaged per dispatch of each kernel (summed over the counter's
+    instances), one row per kernel"""
+    import collections
+    import sqlite3
+    con = sqlite3.connect(db)
+    per = collections.defaultdict(float)
+    disp = collections.defaultdict(set)
+    names = set()
+    for d, k, c, v in con.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+        per[(k, c)] += v
+        disp[k].add(d)
+        names.add(c)
+    cols = sorted(names)
+    print("| kernel | dispatches | " + " | ".join(cols) + " |")
+    print("|---|---|" + "---|" * len(cols))
+    for k in sorted(disp, key=lambda k: -per[(k, cols[0])] if cols else 0):
+        n = len(disp[k])
+        print("| %s | %d | " % (k[:60], n) + " | ".join("%.4g" % (per[(k, c)] / n) for c in cols) + " |")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--counter-summary":
+        counter_summary(sys.argv[2])
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--trace-summary":
         trace_summary(sys.argv[2])
         return
