@@ -1,10 +1,25 @@
 /* nttm.h - the C ABI of the MI355X multi-prime NTT product library (libnttm.so).
  *
  * Numbers are arrays of base-10^18 limbs (uint64_t, each < 10^18), least significant first, not necessarily
- * normalized -- the reference's convention (E/bigint.h:12, E/rns_mul.h:48-51).  Every pointer an entry point takes
- * for limbs, residues or products is DEVICE memory (hipMalloc) unless stated; the caller owns it.  Calls run
- * asynchronously on the given HIP stream (a hipStream_t passed as void *, NULL = the null stream) and on the device
- * selected by nttm_init.  One host thread per library instance; calls are not reentrant.
+ * normalized -- the reference's convention (E/bigint.h:12, E/rns_mul.h:48-51).  The caller owns every array it passes.
+ *
+ * Memory: the products (nttm_mul, nttm_mul_add, nttm_mul_cut, nttm_mul_batch, nttm_mul_each) take a, b, x and c in
+ * device memory of the library's device or in host memory -- pageable, pinned (hipHostMalloc) or managed -- as the
+ * reference's rns_mul_batch does (E/rns_mul.h:42-47).  Host arrays are staged: copied into library-owned HBM through a
+ * pinned bounce buffer before the product and copied back after it (E/rns_mul.c:807-817 stages the same way); device
+ * arrays are used in place.  Every other entry point takes DEVICE memory (hipMalloc) for its limb and residue arrays
+ * unless stated, and returns NTTM_EINVAL, before any launch, for host memory, memory of another device or a range
+ * that runs past the end of its device allocation.
+ *
+ * Streams and synchronization: work runs on the given HIP stream (a hipStream_t passed as void *, NULL = the null
+ * stream) on the device selected by nttm_init.  These entries return with the stream drained (they read flags or
+ * lengths back): nttm_mul, nttm_mul_add, nttm_mul_cut, nttm_mul_batch, nttm_mul_each, nttm_mul_each_async,
+ * nttm_bs_tree, nttm_e_fraction, nttm_e_write_packed, nttm_limbs_mod, nttm_crt, nttm_ntt_fwd_mfma,
+ * nttm_batch_plan_create, nttm_batch_plan_ncn.  These only enqueue work and return: nttm_ntt_fwd, nttm_ntt_inv,
+ * nttm_ntt_inv_pw, nttm_ntt_fwd3, nttm_ntt_inv3, nttm_ntt_inv3_pw, nttm_pw, nttm_load, nttm_pack60,
+ * nttm_batch_plan_run.  (The reference's NTT calls are asynchronous on the caller's stream, E/ntt.h:1-2; its products
+ * return complete, E/rns_mul.c:792-1028.)  Growing the library's workspace synchronizes the device.  One host thread
+ * per library instance; calls are not reentrant.
  *
  * Errors: every entry returns 0 on success or a negative NTTM_E* code, and nttm_last_error() describes the last
  * failure.  The reference instead ends the process through ec_fatal (E/fatal.h:19-23); its limits are kept as
@@ -86,7 +101,7 @@ uint64_t nttm_root(int i, int logn);
 /* largest min(na, nb) a product on np primes reconstructs exactly: floor((p_0 ... p_{np-1} - 1) / (10^18 - 1)^2) */
 uint64_t nttm_max_terms(int np);
 
-/* c[0 .. na+nb) = a b; device pointers.  na or nb == 0 gives c = 0.  A product that no single transform holds
+/* c[0 .. na+nb) = a b; device or host memory (see the top).  na or nb == 0 gives c = 0.  A product that no single transform holds
  * (more than 2^32 points, more terms than the prime set reconstructs, or more workspace than nttm_config caps) is
  * formed as a grid of piece products added into c (the reference's grid, E/rns_dist.c:610-700). */
 int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream);
@@ -101,8 +116,9 @@ int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, s
  * No carry beyond limb w-1 is checked. */
 int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut, size_t w,
                  void *stream);
-/* N independent products; P is HOST memory holding device pointers.  Products are grouped by transform length and
- * run batched; ncn is filled on return (the call synchronizes the stream). */
+/* N independent products; P is HOST memory; its a, b, x, c may each be device or host memory (host arrays staged, see
+ * the top; a batch with more than 1 GiB of host arrays runs as consecutive sub-batches).  Products are grouped by
+ * transform length and run batched; ncn is filled on return (the call synchronizes the stream). */
 int nttm_mul_batch(nttm_prod *P, size_t N, void *stream);
 /* a node of a binary-splitting tree: P and Q as device limb arrays */
 typedef struct nttm_bs_node {
@@ -153,7 +169,8 @@ int nttm_e_write_packed(const char *path, const uint64_t *frac, size_t m, uint64
  * nttm_shutdown.  No counterpart in the reference (it plans per call). */
 typedef struct nttm_batch_plan nttm_batch_plan;
 int nttm_batch_plan_create(const nttm_prod *P, size_t N, int autotune, nttm_batch_plan **plan, void *stream);
-/* all N products, one graph launch on the stream (asynchronous) */
+/* all N products, one graph launch on the stream (asynchronous).  NTTM_ESTATE for a plan made under an earlier
+ * nttm_init (the library's tables it captured are gone); device memory only (a plan refuses host operands). */
 int nttm_batch_plan_run(nttm_batch_plan *plan, void *stream);
 /* after a run: the normalized lengths (ncn[i] for product i; HOST memory, N entries); synchronizes the stream and
  * checks the CRT's carry flags */
@@ -161,8 +178,13 @@ int nttm_batch_plan_ncn(nttm_batch_plan *plan, size_t *ncn, void *stream);
 /* the chosen schedule as text, one line per group */
 int nttm_batch_plan_info(const nttm_batch_plan *plan, char *buf, size_t cap);
 void nttm_batch_plan_destroy(nttm_batch_plan *plan);
-/* the same N products, one product call each (the unbatched baseline of the batch tier) */
+/* the same N products, one product call each (the unbatched baseline of the batch tier): every call synchronizes and
+ * the normalized lengths are read back limb by limb.  No counterpart in the reference. */
 int nttm_mul_each(nttm_prod *P, size_t N, void *stream);
+/* the same N products, one product call each, launched back to back with no synchronization between them: one at the
+ * end for the CRT flags, and the normalized lengths by one kernel over all N.  Device memory only.  No counterpart in
+ * the reference (the one-launch-per-product baseline without host round trips). */
+int nttm_mul_each_async(nttm_prod *P, size_t N, void *stream);
 
 /* ---- transform primitives on canonical residues in [0, p) (uint64_t), batch transforms of 2^logn contiguous ---- */
 /* natural order in -> bit-reversed out */

@@ -85,6 +85,7 @@ def lib():
         L.nttm_e_write_packed.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), sz, u64, vp]
         L.nttm_e_write_packed.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
+        L.nttm_mul_each_async.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each_async.restype = i32
         L.nttm_batch_plan_create.argtypes = [C.POINTER(Prod), sz, i32, C.POINTER(vp), vp]
         L.nttm_batch_plan_create.restype = i32
         L.nttm_batch_plan_run.argtypes = [vp, vp]; L.nttm_batch_plan_run.restype = i32
@@ -130,6 +131,48 @@ def _stream(stream):
 
 def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _dev_limbs(t, name, need=0):
+    """check that t is a contiguous int64 CUDA tensor of at least `need` limbs (ValueError before any library call)"""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("%s: a torch tensor is required, got %s" % (name, type(t).__name__))
+    if not t.is_cuda:
+        raise ValueError("%s: a CUDA tensor is required (device memory), got one on %s" % (name, t.device))
+    if t.dtype != torch.int64:
+        raise ValueError("%s: int64 limbs are required, got %s" % (name, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s: a contiguous tensor is required" % name)
+    if t.numel() < need:
+        raise ValueError("%s: %d limbs, at least %d needed" % (name, t.numel(), need))
+    return t
+
+
+def _any_limbs(t, name, need=0):
+    """(address, limbs) of a limb array for the products, which take device or host memory: an int64 torch tensor (CUDA
+    or CPU) or an int64 / uint64 numpy array; contiguous, at least `need` limbs (ValueError otherwise)"""
+    import numpy as np
+    if isinstance(t, np.ndarray):
+        if t.dtype not in (np.uint64, np.int64):
+            raise ValueError("%s: uint64 or int64 limbs are required, got %s" % (name, t.dtype))
+        if not t.flags.c_contiguous:
+            raise ValueError("%s: a contiguous array is required" % name)
+        n = t.size
+        addr = t.ctypes.data
+    else:
+        import torch
+        if not isinstance(t, torch.Tensor):
+            raise ValueError("%s: a torch tensor or numpy array is required, got %s" % (name, type(t).__name__))
+        if t.dtype != torch.int64:
+            raise ValueError("%s: int64 limbs are required, got %s" % (name, t.dtype))
+        if not t.is_contiguous():
+            raise ValueError("%s: a contiguous tensor is required" % name)
+        n = t.numel()
+        addr = t.data_ptr()
+    if n < need:
+        raise ValueError("%s: %d limbs, at least %d needed" % (name, n, need))
+    return (addr if n else None), n
 
 
 # ---- host-side queries (no device needed) ---------------------------------------------------------------------------
@@ -181,8 +224,9 @@ def shutdown():
 def mul(a, b, out=None, stream=None):
     """c = a b for int64 device tensors of limbs; returns c (na + nb limbs)."""
     import torch
+    _dev_limbs(a, "a"); _dev_limbs(b, "b")
     na, nb = a.numel(), b.numel()
-    c = out if out is not None else torch.empty(max(na + nb, 1), dtype=torch.int64, device=a.device)
+    c = _dev_limbs(out, "out", na + nb) if out is not None else torch.empty(max(na + nb, 1), dtype=torch.int64, device=a.device)
     _check(lib().nttm_mul(_ptr(c), _ptr(a), na, _ptr(b), nb, _stream(stream)))
     return c[:na + nb]
 
@@ -190,8 +234,11 @@ def mul(a, b, out=None, stream=None):
 def mul_add(a, b, x, out=None, stream=None):
     """c = a b + x (na + nb + 1 limbs, the top one the final carry)."""
     import torch
+    _dev_limbs(a, "a"); _dev_limbs(b, "b"); _dev_limbs(x, "x")
     na, nb = a.numel(), b.numel()
-    c = out if out is not None else torch.empty(na + nb + 1, dtype=torch.int64, device=a.device)
+    if x.numel() > na + nb:
+        raise ValueError("x: %d limbs, at most na + nb = %d" % (x.numel(), na + nb))
+    c = _dev_limbs(out, "out", na + nb + 1) if out is not None else torch.empty(na + nb + 1, dtype=torch.int64, device=a.device)
     _check(lib().nttm_mul_add(_ptr(c), _ptr(a), na, _ptr(b), nb, _ptr(x), x.numel(), _stream(stream)))
     return c[:na + nb + 1]
 
@@ -202,9 +249,10 @@ WHOLE = 2 ** 64 - 1          # w of a cut product that keeps every limb (SIZE_MA
 def mul_cut(a, b, lowcut=0, w=WHOLE, out=None, stream=None):
     """rns_mul_low_db / high_db / band_db analog: the low min(na + nb, w) limbs of a b (operands cut to w limbs)."""
     import torch
+    _dev_limbs(a, "a"); _dev_limbs(b, "b")
     na, nb = a.numel(), b.numel()
     n = min(na + nb, w)
-    c = out if out is not None else torch.empty(max(n, 1), dtype=torch.int64, device=a.device)
+    c = _dev_limbs(out, "out", n) if out is not None else torch.empty(max(n, 1), dtype=torch.int64, device=a.device)
     _check(lib().nttm_mul_cut(_ptr(c), _ptr(a) if na else None, na, _ptr(b) if nb else None, nb, lowcut, w,
                               _stream(stream)))
     return c[:n]
@@ -303,11 +351,21 @@ def e_digits(m, stream=None):
 
 
 def make_prods(items):
-    """items: list of (a, b, x_or_None, c) device tensors -> ctypes array of Prod."""
+    """items: list of (a, b, x_or_None, c) -> ctypes array of Prod.  Each array is an int64 torch tensor (CUDA or CPU)
+    or an int64 / uint64 numpy array (the products stage host memory through HBM); c needs na + nb limbs, + 1 with x.
+    The array keeps the items alive (``arr._keep``): the records hold their addresses."""
     arr = (Prod * len(items))()
     for i, (a, b, x, c) in enumerate(items):
-        arr[i] = Prod(a.data_ptr() if a.numel() else None, a.numel(), b.data_ptr() if b.numel() else None, b.numel(),
-                      c.data_ptr(), None if x is None else x.data_ptr(), 0 if x is None else x.numel(), 0)
+        pa, na = _any_limbs(a, "product %d: a" % i)
+        pb, nb = _any_limbs(b, "product %d: b" % i)
+        px, nx = (None, 0) if x is None else _any_limbs(x, "product %d: x" % i)
+        if x is not None and nx > na + nb:
+            raise ValueError("product %d: x has %d limbs, at most na + nb = %d" % (i, nx, na + nb))
+        pc, _ = _any_limbs(c, "product %d: c" % i, na + nb + (0 if x is None else 1))
+        if x is not None and px is None:
+            px = pc                      # an empty x still marks the carry limb: any valid address, never read
+        arr[i] = Prod(pa, na, pb, nb, pc, px, nx, 0)
+    arr._keep = list(items)
     return arr
 
 
@@ -318,7 +376,14 @@ def mul_batch(prods, stream=None):
 
 
 def mul_each(prods, stream=None):
+    """one product call each (synchronizing per product)"""
     _check(lib().nttm_mul_each(prods, len(prods), _stream(stream)))
+    return prods
+
+
+def mul_each_async(prods, stream=None):
+    """one product call each, launched back to back, one synchronization at the end (device memory only)"""
+    _check(lib().nttm_mul_each_async(prods, len(prods), _stream(stream)))
     return prods
 
 
@@ -327,7 +392,8 @@ class BatchPlan:
     uploaded, launches captured as one hipGraph -- and run many times"""
 
     def __init__(self, prods, autotune=True, stream=None):
-        self.prods = prods                                       # keeps the descriptors' tensors' owner alive
+        self.prods = prods                                       # the records (their graph has the addresses built in)
+        self._keep = getattr(prods, "_keep", None)               # and the arrays they point to, kept alive with the plan
         self.n = len(prods)
         h = C.c_void_p()
         _check(lib().nttm_batch_plan_create(prods, self.n, 1 if autotune else 0, C.byref(h), _stream(stream)))

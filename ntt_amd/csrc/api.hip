@@ -22,6 +22,7 @@ void nttm_set_stream(int on, int ncu);
 #define NTTM_FUSED_LOG 12          /* tile of the fused forward + pointwise + inverse pass (k_fused) */
 #define NTTM_TILE_LOG_DEFAULT 13   /* tile of the strided passes: 2^13 points (64 KiB of LDS), runs of 32 columns */
 #define NTTM_PRODUCT_LOGN_MIN 4    /* products use at least 2^4 points, so every fused tile has >= 16 */
+#define NTTM_STAGE_CHUNK ((size_t)1 << 30)   /* staged bytes per sub-batch of a batch with host operands (default) */
 struct LoadArgs;
 int nttm_run_u64_to_f64(double *x, uint64_t count, hipStream_t s);
 int nttm_run_cvt_u64_f64(double *dst, const uint64_t *src, uint64_t count, hipStream_t s);
@@ -99,10 +100,17 @@ struct Ctx {
     uint64_t *tmp; size_t tmp_limbs;         /* a piece product of a grid (above one transform) */
     uint64_t *bsp[2]; size_t bsp_limbs[2];   /* the inner levels of a binary-splitting tree (even / odd levels) */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
+    char *stage; size_t stage_bytes;         /* device copies of host-resident operands and results */
+    size_t stage_chunk;                      /* staged bytes per sub-batch of a batch */
+    char *bounce; size_t bounce_half;        /* pinned bounce buffer of the host copies: two halves */
+    hipEvent_t bev[2]; int bev_live[2];      /* the last copy through each half */
+    uint64_t gen;                            /* which nttm_init this context is (plans record it) */
     hipEvent_t ev[4];
+    hipEvent_t pev[2 * NTTM_MAX_PRIMES];      /* around B's transform of each prime (phase times of nttm_mul) */
     double last_ms[3];
 };
 static Ctx g;
+static uint64_t g_init_count;                /* nttm_init calls so far: a plan made under an earlier one is stale */
 
 static uint64_t max_terms_np(int np)
 {
@@ -244,6 +252,60 @@ static int hpin_need(size_t bytes)
     return 0;
 }
 
+/* ---- where an operand lives -------------------------------------------------------------------------------------------
+ * Kernels read and write HBM only (XNACK is off: a kernel touching host memory faults the GPU).  Every pointer an entry
+ * point takes is classed before any launch: device memory of this device is used in place; host memory -- pageable,
+ * pinned (hipHostMalloc) or managed -- is copied through library-owned HBM by the entry points that stage (the
+ * products, as the reference's batch stages products outside registered memory, E/rns_mul.h:42-47,
+ * E/rns_mul.c:807-817) and refused with NTTM_EINVAL by the others.  One MemMap lives for one call: it remembers the
+ * device allocations it has seen, so the operands of a large batch carved from a few allocations cost one runtime
+ * query per allocation. */
+enum { MEM_DEV = 0, MEM_HOST = 1, MEM_PINNED = 2, MEM_BAD = 3 };
+struct MemMap {
+    std::map<uintptr_t, uintptr_t> dev;                          /* base -> end of the device allocations seen */
+    uintptr_t last_b = 0, last_e = 0;                            /* the last one hit */
+    const char *why = "";
+    int kind(const void *p, size_t bytes)
+    {
+        if (!p || !bytes) return MEM_DEV;                        /* nothing is read or written */
+        const uintptr_t a = (uintptr_t)p, e = a + bytes;
+        if (e < a) { why = "wraps the address space"; return MEM_BAD; }
+        if (a >= last_b && a < last_e) return e <= last_e ? MEM_DEV : (why = "runs past the end of its device allocation", MEM_BAD);
+        auto it = dev.upper_bound(a);
+        if (it != dev.begin()) {
+            --it;
+            if (a < it->second) {
+                last_b = it->first; last_e = it->second;
+                return e <= it->second ? MEM_DEV : (why = "runs past the end of its device allocation", MEM_BAD);
+            }
+        }
+        hipPointerAttribute_t at;
+        memset(&at, 0, sizeof at);
+        if (hipPointerGetAttributes(&at, p) != hipSuccess) { (void)hipGetLastError(); return MEM_HOST; }   /* pageable */
+        if (at.type == hipMemoryTypeDevice) {
+            if (at.device != g.dev) { why = "is device memory of another device"; return MEM_BAD; }
+            hipDeviceptr_t base = nullptr; size_t sz = 0;
+            if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)p) == hipSuccess && base && sz) {
+                const uintptr_t b = (uintptr_t)base;
+                dev[b] = b + sz; last_b = b; last_e = b + sz;
+                if (e > b + sz) { why = "runs past the end of its device allocation"; return MEM_BAD; }
+            } else (void)hipGetLastError();
+            return MEM_DEV;
+        }
+        if (at.type == hipMemoryTypeHost) return MEM_PINNED;
+        return MEM_HOST;                                         /* managed, unified or unregistered */
+    }
+};
+/* NTTM_EINVAL unless [p, p + bytes) is device memory of the library's device (entry points that do not stage) */
+int nttm_require_device(const void *p, size_t bytes, const char *what)
+{
+    MemMap mm;
+    const int k = mm.kind(p, bytes);
+    if (k == MEM_DEV) return 0;
+    return nttm_fail(NTTM_EINVAL, "%s %s", what, k == MEM_BAD ? mm.why : "is host memory (this entry point takes device memory)");
+}
+#define DEVCK(p, bytes, what) RCK(nttm_require_device((p), (bytes), (what)))
+
 extern "C" int nttm_init(int device, const nttm_config *cfg)
 {
     if (g.init) return nttm_fail(NTTM_ESTATE, "nttm_init: already initialised");
@@ -276,7 +338,13 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     build_crt_const(g.cc);
     RCK(build_tables());
     for (int i = 0; i < 4; i++) HIPCK(hipEventCreate(&g.ev[i]));
+    for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) HIPCK(hipEventCreate(&g.pev[i]));
     RCK(hpin_need(1 << 16));
+    {   /* NTTM_STAGE_CHUNK=<bytes>: staged bytes per sub-batch (default 1 GiB; smaller values exercise the split) */
+        const char *e = getenv("NTTM_STAGE_CHUNK");
+        g.stage_chunk = e && atoll(e) > 0 ? (size_t)atoll(e) : NTTM_STAGE_CHUNK;
+    }
+    g.gen = ++g_init_count;
     g.init = 1;
     return 0;
 }
@@ -292,7 +360,11 @@ extern "C" void nttm_shutdown(void)
     for (int k = 0; k < 2; k++) if (g.bsp[k]) (void)hipFree(g.bsp[k]);
     if (g.tables) (void)hipFree(g.tables);
     if (g.hpin) (void)hipHostFree(g.hpin);
+    if (g.stage) (void)hipFree(g.stage);
+    if (g.bounce) (void)hipHostFree(g.bounce);
+    for (int k = 0; k < 2; k++) if (g.bev[k]) (void)hipEventDestroy(g.bev[k]);
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
+    for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) if (g.pev[i]) (void)hipEventDestroy(g.pev[i]);
     memset(&g, 0, sizeof g);
 }
 
@@ -341,6 +413,7 @@ struct CrtDesc {                 /* must match crt.hip */
     uint64_t first_block;
     uint64_t trunc;
 };
+static_assert(sizeof(CrtDesc) <= 96, "a product's pinned slot holds its CRT descriptor below byte 96");
 
 static int choose_np(size_t mn, int *np)
 {
@@ -373,38 +446,39 @@ static int product_tile_log(int logn, int tl = 0)
 
 /* the transforms of cnt products of one length: 2^logn points (r3 = 0), or 3 2^logn points (r3 = 1: a radix-3 layer,
  * reading the limbs, and the 2^logn engine on the 3 cnt thirds).  Plane i of A holds the cnt transforms of prime i back
- * to back.  After it, A holds the cyclic convolutions mod each prime, natural order.
- * Shared B operands (bmap non-null): B's planes hold the cnt_b distinct operands of ldesc_b; A's engine transform t
- * pairs with B's engine transform bmap[t] in the double-fused pass. */
+ * to back.  After it, A holds the cyclic convolutions mod each prime, natural order.  B has one plane, reused by every
+ * prime: B's transform of prime i is made just before A's (np + 1 planes in all instead of 2 np).
+ * Shared B operands (bmap non-null): B's plane holds the cnt_b distinct operands of ldesc_b; A's engine transform t
+ * pairs with B's engine transform bmap[t] in the double-fused pass.
+ * pev (or null): 2 np events recorded around B's transform of each prime. */
 static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A, double *Bp, const uint64_t *a,
                               uint64_t na, const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s,
-                              hipEvent_t mid, uint64_t cnt_b = 0, const LoadDesc *ldesc_b = nullptr,
+                              hipEvent_t *pev, uint64_t cnt_b = 0, const LoadDesc *ldesc_b = nullptr,
                               const uint32_t *bmap = nullptr)
 {
     if (!bmap) { cnt_b = cnt; ldesc_b = ldesc; }
     const uint64_t tb = r3 ? 3 * cnt : cnt, tbb = r3 ? 3 * cnt_b : cnt_b;   /* engine transforms of 2^logn points */
     Plan pl; plan_make2(&pl, logn, tb, product_tile_log(logn), NTTM_FUSED_LOG);
     Plan plb; plan_make2(&plb, logn, tbb, product_tile_log(logn), NTTM_FUSED_LOG);
-    const uint64_t n = (uint64_t)1 << logn, plane = tb * n, plane_b = tbb * n, ntot = r3 ? 3 * n : n;
+    const uint64_t n = (uint64_t)1 << logn, plane = tb * n, ntot = r3 ? 3 * n : n;
     const int last = pl.npass - 1;
     if (bmap && (plb.npass != pl.npass || pl.p[last].m + pl.p[last].logh > logn || (last == 0 && !r3)))
         return nttm_fail(NTTM_EINVAL, "internal: shared B operands on this length");
-    for (int i = 0; i < np; i++) {                               /* B, every prime, all passes but the last */
-        double *Bi = Bp + (size_t)i * plane_b;
+    for (int i = 0; i < np; i++) {
+        double *Ai = A + (size_t)i * plane;
         const uint64_t p = NTTM_PRIMES[i];
+        /* B, prime i, all passes but the last */
+        if (pev) HIPCK(hipEventRecord(pev[2 * i], s));
         const double2 ninv = h_shoup(h_inv(ntot % p, p), p);     /* the inverse's 1/length, folded into B */
-        PassArgs fb = base_args(i, Bi, logn, tbb, false);
-        if (r3) RCK(nttm_run_r3(false, Bi, cnt_b, logn, g.pk[i], g.twf[i].b3, ldesc_b ? nullptr : b, nb, ldesc_b, 1, &ninv, s));
+        PassArgs fb = base_args(i, Bp, logn, tbb, false);
+        if (r3) RCK(nttm_run_r3(false, Bp, cnt_b, logn, g.pk[i], g.twf[i].b3, ldesc_b ? nullptr : b, nb, ldesc_b, 1, &ninv, s));
         else { fb.limbs = ldesc_b ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc_b; fb.which = 1; fb.in_scale = ninv; fb.do_in_scale = 1; }
         for (int j = 0; j < last; j++) {
             RCK(nttm_run_pass(false, fb, plb.p[j], s));
             fb.limbs = nullptr; fb.ldesc = nullptr; fb.do_in_scale = 0;
         }
-    }
-    if (mid) HIPCK(hipEventRecord(mid, s));
-    for (int i = 0; i < np; i++) {                               /* A, every prime */
-        double *Ai = A + (size_t)i * plane, *Bi = Bp + (size_t)i * plane_b;
-        const uint64_t p = NTTM_PRIMES[i];
+        if (pev) HIPCK(hipEventRecord(pev[2 * i + 1], s));
+        /* A, prime i */
         PassArgs fa = base_args(i, Ai, logn, tb, false);
         if (r3) RCK(nttm_run_r3(false, Ai, cnt, logn, g.pk[i], g.twf[i].b3, ldesc ? nullptr : a, na, ldesc, 0, nullptr, s));
         else { fa.limbs = ldesc ? nullptr : a; fa.nlimbs = na; fa.ldesc = ldesc; fa.which = 0; }
@@ -413,13 +487,13 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
             fa.limbs = nullptr; fa.ldesc = nullptr;
         }
         /* B's last forward pass, A's last forward pass, the pointwise product and A's first inverse pass */
-        PassArgs fb = base_args(i, Bi, logn, tb, false);
-        fb.bmap = bmap;
+        PassArgs fbl = base_args(i, Bp, logn, tb, false);
+        fbl.bmap = bmap;
         if (last == 0 && !r3) {                                  /* a single-pass transform reads B's limbs here */
-            fb.limbs = ldesc ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc; fb.which = 1;
-            fb.in_scale = h_shoup(h_inv(n % p, p), p); fb.do_in_scale = 1;
+            fbl.limbs = ldesc ? nullptr : b; fbl.nlimbs = nb; fbl.ldesc = ldesc; fbl.which = 1;
+            fbl.in_scale = h_shoup(h_inv(n % p, p), p); fbl.do_in_scale = 1;
         }
-        RCK(nttm_run_fused2(fa, fb, pl.p[last], s));
+        RCK(nttm_run_fused2(fa, fbl, pl.p[last], s));
         PassArgs ia = base_args(i, Ai, logn, tb, true);
         for (int j = last - 1; j >= 0; j--) RCK(nttm_run_pass(true, ia, pl.p[j], s));
         if (r3) RCK(nttm_run_r3(true, Ai, cnt, logn, g.pk[i], g.twi[i].b3, nullptr, 0, nullptr, 0, nullptr, s));
@@ -471,7 +545,7 @@ static int one_plan(size_t na, size_t nb, size_t nc, OnePlan *o)
     choose_len(na + nb - 1, &o->logn, &o->r3);
     if (o->logn > NTTM_LOGN_MAX) return 1;
     o->n = (uint64_t)(o->r3 ? 3 : 1) << o->logn;
-    o->planes_bytes = (size_t)2 * o->np * o->n * 8;
+    o->planes_bytes = (size_t)(o->np + 1) * o->n * 8;
     o->crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
     o->ws_bytes = align_up(o->planes_bytes, 256) + 256 + o->crt_words * 4 + 256;
     return 0;
@@ -482,9 +556,32 @@ static bool fits_one(size_t na, size_t nb, size_t nc)
     return !one_plan(na, nb, nc, &o) && (!g.cfg.workspace_bytes || o.ws_bytes <= g.cfg.workspace_bytes);
 }
 
-/* wcut: write only the low wcut limbs of the result (the value mod B^wcut; no check of what lies above) */
+/* the workspace of a one-transform product: the planes, its CRT descriptor and the CRT's scratch (bytes from ws) */
+struct CoreLayout { int np, logn, r3; uint64_t n, ncoef; size_t nc, nc_full, off_desc, off_crt, bytes; };
+static int core_layout(size_t na, size_t nb, bool has_x, size_t wcut, CoreLayout *L)
+{
+    L->nc_full = na + nb + (has_x ? 1 : 0); L->nc = L->nc_full < wcut ? L->nc_full : wcut;
+    RCK(choose_np(na < nb ? na : nb, &L->np));
+    const uint64_t ncoef_full = na + nb - 1;
+    L->ncoef = ncoef_full < L->nc ? ncoef_full : L->nc;           /* coefficients read */
+    choose_len(ncoef_full, &L->logn, &L->r3);
+    if (L->logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", L->logn, NTTM_LOGN_MAX);
+    L->n = (uint64_t)(L->r3 ? 3 : 1) << L->logn;                  /* points per plane */
+    /* np planes of A and one of B: B's transform of each prime is made just before A's of that prime */
+    const size_t planes_bytes = (size_t)(L->np + 1) * L->n * 8;
+    const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(L->nc));
+    L->off_desc = align_up(planes_bytes, 256); L->off_crt = L->off_desc + 256;
+    L->bytes = L->off_crt + crt_words * 4 + 256;
+    return 0;
+}
+
+/* c = a b (+ x), device pointers.  wcut: write only the low wcut limbs of the result (the value mod B^wcut; no check of
+ * what lies above).  ws: the workspace (null: the library's, grown to fit; else ws_cap bytes of the caller's).
+ * sync = 1: returns after the stream has drained and the CRT's flags are checked; sync = 0: returns after the launches,
+ * the flags copied to pinned slot `slot` (g.hpin + 128 slot + 64, read by the caller after a synchronization). */
 static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
-                    hipStream_t s, size_t wcut = SIZE_MAX)
+                    hipStream_t s, size_t wcut = SIZE_MAX, char *ws = nullptr, size_t ws_cap = 0, int sync = 1,
+                    size_t slot = 0)
 {
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     const size_t nc_full = na + nb + (x ? 1 : 0), nc = nc_full < wcut ? nc_full : wcut;
@@ -494,40 +591,43 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     if (!na || !nb) {
         if (nc) HIPCK(hipMemsetAsync(c, 0, nc * 8, s));
         if (x && nx && nc) HIPCK(hipMemcpyAsync(c, x, (nx < nc ? nx : nc) * 8, hipMemcpyDeviceToDevice, s));
-        HIPCK(hipStreamSynchronize(s));
+        if (sync) HIPCK(hipStreamSynchronize(s));
+        else ((uint32_t *)((char *)g.hpin + 128 * slot + 96))[1] = 0;
         return 0;
     }
     if (!nc) return 0;
-    int np; RCK(choose_np(na < nb ? na : nb, &np));
-    const uint64_t ncoef_full = na + nb - 1, ncoef = ncoef_full < nc ? ncoef_full : nc;   /* coefficients read */
-    int logn, r3;
-    choose_len(ncoef_full, &logn, &r3);
-    if (logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", logn, NTTM_LOGN_MAX);
-    const uint64_t n = (uint64_t)(r3 ? 3 : 1) << logn;             /* points per plane */
-    const size_t planes_bytes = (size_t)2 * np * n * 8;
-    const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
-    const size_t off_desc = align_up(planes_bytes, 256), off_crt = off_desc + 256;
-    RCK(ws_need(off_crt + crt_words * 4 + 256));
-    double *A = (double *)g.ws, *Bp = A + (size_t)np * n;
-    CrtDesc *ddesc = (CrtDesc *)(g.ws + off_desc);
-    uint32_t *scratch = (uint32_t *)(g.ws + off_crt);
+    CoreLayout L; RCK(core_layout(na, nb, x != nullptr, wcut, &L));
+    if (ws) {
+        if (L.bytes > ws_cap) return nttm_fail(NTTM_EINVAL, "the product needs %zu bytes of workspace, above the %zu given", L.bytes, ws_cap);
+    } else {
+        RCK(ws_need(L.bytes));
+        ws = g.ws;
+    }
+    double *A = (double *)ws, *Bp = A + (size_t)L.np * L.n;
+    CrtDesc *ddesc = (CrtDesc *)(ws + L.off_desc);
+    uint32_t *scratch = (uint32_t *)(ws + L.off_crt);
+    char *hs = (char *)g.hpin + 128 * slot;                       /* pinned: the descriptor, then (at 96) the flags */
 
-    HIPCK(hipEventRecord(g.ev[0], s));
-    RCK(product_transforms(np, logn, r3, 1, A, Bp, a, na, b, nb, nullptr, s, g.ev[1]));
-    HIPCK(hipEventRecord(g.ev[2], s));
+    if (sync) HIPCK(hipEventRecord(g.ev[0], s));
+    RCK(product_transforms(L.np, L.logn, L.r3, 1, A, Bp, a, na, b, nb, nullptr, s, sync ? g.pev : nullptr));
+    if (sync) HIPCK(hipEventRecord(g.ev[2], s));
     CrtDesc hd; memset(&hd, 0, sizeof hd);
-    hd.pl = A; hd.plane = n; hd.ncoef = ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
+    hd.pl = A; hd.plane = L.n; hd.ncoef = L.ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
     hd.trunc = nc < nc_full;
-    memcpy(g.hpin, &hd, sizeof hd);
-    HIPCK(hipMemcpyAsync(ddesc, g.hpin, sizeof hd, hipMemcpyHostToDevice, s));
-    RCK(nttm_crt_launch_batch(ddesc, 1, nttm_crt_blocks(nc), np, g.cc, scratch, s,
-                              8.0 * np * (double)ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? (nx < nc ? nx : nc) : 0)));
-    HIPCK(hipEventRecord(g.ev[3], s));
-    HIPCK(hipMemcpyAsync(g.hpin + 64, scratch, 8, hipMemcpyDeviceToHost, s));
+    memcpy(hs, &hd, sizeof hd);
+    HIPCK(hipMemcpyAsync(ddesc, hs, sizeof hd, hipMemcpyHostToDevice, s));
+    RCK(nttm_crt_launch_batch(ddesc, 1, nttm_crt_blocks(nc), L.np, g.cc, scratch, s,
+                              8.0 * L.np * (double)L.ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? (nx < nc ? nx : nc) : 0)));
+    if (sync) HIPCK(hipEventRecord(g.ev[3], s));
+    HIPCK(hipMemcpyAsync(hs + 96, scratch, 8, hipMemcpyDeviceToHost, s));
+    if (!sync) return 0;
     HIPCK(hipStreamSynchronize(s));
-    float ms;
-    for (int k = 0; k < 3; k++) { HIPCK(hipEventElapsedTime(&ms, g.ev[k], g.ev[k + 1])); g.last_ms[k] = ms; }
-    const uint32_t flags = ((const uint32_t *)(g.hpin + 64))[1];
+    float ms, tb = 0, t;
+    for (int i = 0; i < L.np; i++) { HIPCK(hipEventElapsedTime(&t, g.pev[2 * i], g.pev[2 * i + 1])); tb += t; }
+    HIPCK(hipEventElapsedTime(&ms, g.ev[0], g.ev[2]));
+    g.last_ms[0] = tb; g.last_ms[1] = ms - tb;
+    HIPCK(hipEventElapsedTime(&ms, g.ev[2], g.ev[3])); g.last_ms[2] = ms;
+    const uint32_t flags = ((const uint32_t *)(hs + 96))[1];
     if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond the top limb (wrong residues)");
     return 0;
 }
@@ -602,26 +702,6 @@ static int mul_any(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b,
     return mul_grid(c, a, na, b, nb, x, nx, s, lowcut, w);
 }
 
-extern "C" int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream)
-{
-    return mul_any(c, a, na, b, nb, nullptr, 0, (hipStream_t)stream);
-}
-extern "C" int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
-                            size_t nx, void *stream)
-{
-    if (!x) return nttm_fail(NTTM_EINVAL, "nttm_mul_add: null x");
-    return mul_any(c, a, na, b, nb, x, nx, (hipStream_t)stream);
-}
-/* the cut product (E/rns_dist.c:581-589,684-693; E/rns_mul.c rns_mul_local): the operands cut to their low w limbs,
- * the low min(na + nb, w) limbs of the product written.  A product that fits one transform is formed whole (the
- * reference skips no piece of it either); a grid skips the pieces ending at or below lowcut. */
-extern "C" int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut,
-                            size_t w, void *stream)
-{
-    const size_t nout = na + nb < w ? na + nb : w;
-    if (lowcut > nout) return nttm_fail(NTTM_EINVAL, "nttm_mul_cut: lowcut %zu above the %zu limbs written", lowcut, nout);
-    return mul_any(c, a, na < w ? na : w, b, nb < w ? nb : w, nullptr, 0, (hipStream_t)stream, lowcut, w);
-}
 extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
 {
     if (f) *f = g.last_ms[0];
@@ -631,28 +711,6 @@ extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
 }
 
 /* ---- batches ---------------------------------------------------------------------------------------------------------- */
-extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
-{
-    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
-    hipStream_t s = (hipStream_t)stream;
-    for (size_t i = 0; i < N; i++) {
-        RCK(mul_any(P[i].c, P[i].a, P[i].na, P[i].b, P[i].nb, P[i].x, P[i].nx, s));
-        const size_t nc = P[i].na + P[i].nb + (P[i].x ? 1 : 0);
-        P[i].ncn = nc;
-    }
-    /* normalized lengths: read back the top limbs */
-    for (size_t i = 0; i < N; i++) {
-        size_t n = P[i].ncn;
-        while (n) {
-            uint64_t v; HIPCK(hipMemcpy(&v, P[i].c + n - 1, 8, hipMemcpyDeviceToHost));
-            if (v) break;
-            n--;
-        }
-        P[i].ncn = n;
-    }
-    return 0;
-}
-
 /* one length group of a batch: its products' descriptors and their layout in device memory (offsets from the group's
  * descriptor base), for a given schedule (length, strided-pass tile, points per thread) */
 struct GroupBuild {
@@ -663,7 +721,7 @@ struct GroupBuild {
     std::vector<CrtDesc> cd;
     std::vector<LoadDesc> ld, ldb;
     std::vector<uint32_t> bmap;
-    size_t planes_bytes = 0;                                     /* np (cnt + cnt_b) n doubles */
+    size_t planes_bytes = 0;                                     /* (np cnt + cnt_b) n doubles */
     size_t off_cd = 0, off_ld = 0, off_ldb = 0, off_bmap = 0, off_ncn = 0, off_crt = 0, desc_bytes = 0;
     double crt_bytes = 0;
 };
@@ -709,7 +767,7 @@ static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<si
         gb.bmap.resize(tb);
         for (uint64_t t = 0; t < tb; t++) gb.bmap[t] = r3 ? 3 * slot[t / 3] + (uint32_t)(t % 3) : slot[t];
     }
-    gb.planes_bytes = (size_t)np * (cnt + gb.cnt_b) * n * 8;
+    gb.planes_bytes = ((size_t)np * cnt + gb.cnt_b) * n * 8;
     gb.off_cd = 0;
     gb.off_ld = align_up(gb.off_cd + cnt * sizeof(CrtDesc), 256);
     gb.off_ldb = align_up(gb.off_ld + cnt * sizeof(LoadDesc), 256);
@@ -749,10 +807,9 @@ static int group_launch(const GroupBuild &gb, int np, double *A, char *dbase, hi
     return rc;
 }
 
-extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
+/* the batch on device memory (every pointer of P classed MEM_DEV) */
+static int mul_batch_dev(nttm_prod *P, size_t N, hipStream_t s)
 {
-    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
-    hipStream_t s = (hipStream_t)stream;
     if (!N) return 0;
     /* validate; zero-operand products are written directly */
     size_t mn_max = 0;
@@ -811,6 +868,354 @@ extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
     return 0;
 }
 
+/* ---- host-resident operands: staging through HBM -----------------------------------------------------------------------
+ * The products accept operands and results in host memory, as the reference's batch does: it reads registered host
+ * memory in place and stages anything else through device buffers (E/rns_mul.h:42-47, E/rns_mul.c:807-817).  Kernels
+ * here read HBM only, so every host input is copied into a library-owned device region (g.stage) before the products
+ * and every host result copied back after them.  Copies go through a pinned bounce buffer of two halves, so that the
+ * host copy into one half overlaps the DMA out of the other; pinned inputs and results of at least a half move by
+ * DMA directly.  A host operand named by several products of one call (the shared B of a binary-splitting pair) is
+ * copied once, so those products still share its transform.  A batch whose staged bytes exceed g.stage_chunk (1 GiB)
+ * runs as consecutive sub-batches (a product larger than that is staged alone). */
+#define NTTM_BOUNCE_HALF ((size_t)4 << 20)       /* bytes per half of the pinned bounce buffer */
+#define NTTM_STAGE_ALIGN 256                     /* every staged array starts on a 256-byte boundary */
+
+static int stage_need(size_t bytes)
+{
+    if (bytes <= g.stage_bytes) return 0;
+    if (g.stage) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.stage)); g.stage = nullptr; g.stage_bytes = 0; }
+    if (hipMalloc(&g.stage, bytes) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "staging: hipMalloc of %zu bytes failed", bytes); }
+    g.stage_bytes = bytes;
+    return 0;
+}
+static int bounce_need(void)
+{
+    if (g.bounce) return 0;
+    HIPCK(hipHostMalloc((void **)&g.bounce, 2 * NTTM_BOUNCE_HALF, hipHostMallocDefault));
+    g.bounce_half = NTTM_BOUNCE_HALF;
+    for (int k = 0; k < 2; k++) { HIPCK(hipEventCreateWithFlags(&g.bev[k], hipEventDisableTiming)); g.bev_live[k] = 0; }
+    return 0;
+}
+static int bounce_wait(int h)                    /* the last copy through half h has completed */
+{
+    if (g.bev_live[h]) { HIPCK(hipEventSynchronize(g.bev[h])); g.bev_live[h] = 0; }
+    return 0;
+}
+
+/* the host arrays of one staged call and their offsets in g.stage: the inputs first, then the results */
+struct StageSet {
+    struct Item { const void *p; size_t bytes; int kind; size_t off; };
+    std::vector<Item> in, out;
+    std::map<std::pair<const void *, size_t>, size_t> seen;
+    size_t bytes = 0;
+    size_t add_in(const void *p, size_t n, int kind)
+    {
+        const auto key = std::make_pair(p, n);
+        const auto it = seen.find(key);
+        if (it != seen.end()) return it->second;
+        in.push_back({p, n, kind, 0});
+        seen.emplace(key, in.size() - 1);
+        return in.size() - 1;
+    }
+    size_t add_out(void *p, size_t n, int kind) { out.push_back({p, n, kind, 0}); return out.size() - 1; }
+    void layout()
+    {
+        size_t o = 0;
+        for (Item &i : in) { i.off = o; o = align_up(o + i.bytes, NTTM_STAGE_ALIGN); }
+        for (Item &i : out) { i.off = o; o = align_up(o + i.bytes, NTTM_STAGE_ALIGN); }
+        bytes = o ? o : NTTM_STAGE_ALIGN;
+    }
+    const uint64_t *din(size_t k) const { return (const uint64_t *)(g.stage + in[k].off); }
+    uint64_t *dout(size_t k) const { return (uint64_t *)(g.stage + out[k].off); }
+};
+
+/* the inputs into g.stage: consecutive inputs are packed into a bounce half (the alignment gaps zero-filled), and a
+ * full half goes to the device by one DMA while the host fills the other half */
+static int stage_upload(const StageSet &st, hipStream_t s)
+{
+    if (st.in.empty()) return 0;
+    RCK(bounce_need());
+    const size_t half = g.bounce_half;
+    int h = 0;
+    size_t fill = 0, base = 0;                                   /* half h holds fill bytes for g.stage + base */
+    RCK(bounce_wait(0));
+    auto flush = [&]() -> int {
+        if (!fill) return 0;
+        HIPCK(hipMemcpyAsync(g.stage + base, g.bounce + h * half, fill, hipMemcpyHostToDevice, s));
+        HIPCK(hipEventRecord(g.bev[h], s)); g.bev_live[h] = 1;
+        h ^= 1; fill = 0;
+        return bounce_wait(h);
+    };
+    for (const StageSet::Item &i : st.in) {
+        if (!i.bytes) continue;
+        if (i.kind == MEM_PINNED && i.bytes >= half) {           /* pinned and large: one direct DMA */
+            RCK(flush());
+            HIPCK(hipMemcpyAsync(g.stage + i.off, i.p, i.bytes, hipMemcpyHostToDevice, s));
+            continue;
+        }
+        const char *src = (const char *)i.p;
+        size_t left = i.bytes, off = i.off;
+        while (left) {
+            if (!fill) base = off;
+            else if (base + fill != off) {                       /* an alignment gap: zero-filled while it fits */
+                const size_t pad = off - (base + fill);
+                if (fill + pad < half) { memset(g.bounce + h * half + fill, 0, pad); fill += pad; }
+                else { RCK(flush()); base = off; }
+            }
+            const size_t take = std::min(left, half - fill);
+            memcpy(g.bounce + h * half + fill, src, take);
+            fill += take; src += take; off += take; left -= take;
+            if (fill == half) RCK(flush());
+        }
+    }
+    return flush();
+}
+
+/* the results out of g.stage: windows of at most a half DMA'd into alternating halves, each landed window copied to
+ * its host arrays while the next one is in flight; pinned results of at least a half by DMA directly.  Returns after
+ * the stream has drained. */
+static int stage_download(const StageSet &st, hipStream_t s)
+{
+    if (st.out.empty()) return nttm_hip_check(hipStreamSynchronize(s), "staging");
+    RCK(bounce_need());
+    const size_t half = g.bounce_half;
+    struct Piece { char *dst; size_t off, bytes; };
+    struct Win { size_t w0, w1, p0, p1; };                       /* device range, pieces [p0, p1) */
+    std::vector<Piece> pc;
+    for (const StageSet::Item &o : st.out) {
+        if (!o.bytes) continue;
+        if (o.kind == MEM_PINNED && o.bytes >= half) {
+            HIPCK(hipMemcpyAsync((void *)o.p, g.stage + o.off, o.bytes, hipMemcpyDeviceToHost, s));
+            continue;
+        }
+        for (size_t k = 0; k < o.bytes; k += half) pc.push_back({(char *)o.p + k, o.off + k, std::min(half, o.bytes - k)});
+    }
+    std::vector<Win> win;
+    for (size_t k = 0; k < pc.size(); k++) {
+        if (!win.empty() && pc[k].off + pc[k].bytes - win.back().w0 <= half) { win.back().w1 = pc[k].off + pc[k].bytes; win.back().p1 = k + 1; }
+        else win.push_back({pc[k].off, pc[k].off + pc[k].bytes, k, k + 1});
+    }
+    RCK(bounce_wait(0)); RCK(bounce_wait(1));
+    auto scatter = [&](size_t w) {
+        const Win &x = win[w];
+        const char *hb = g.bounce + (w & 1) * half;
+        for (size_t k = x.p0; k < x.p1; k++) memcpy(pc[k].dst, hb + (pc[k].off - x.w0), pc[k].bytes);
+    };
+    for (size_t w = 0; w < win.size(); w++) {
+        const int h = (int)(w & 1);
+        HIPCK(hipMemcpyAsync(g.bounce + h * half, g.stage + win[w].w0, win[w].w1 - win[w].w0, hipMemcpyDeviceToHost, s));
+        HIPCK(hipEventRecord(g.bev[h], s)); g.bev_live[h] = 1;
+        if (w) { RCK(bounce_wait(h ^ 1)); scatter(w - 1); }
+    }
+    if (!win.empty()) { RCK(bounce_wait((int)((win.size() - 1) & 1))); scatter(win.size() - 1); }
+    return nttm_hip_check(hipStreamSynchronize(s), "staging");
+}
+
+/* the four pointers of a product classed (a, b, x, c with their byte counts); NTTM_EINVAL for memory no entry can use */
+static int prod_kinds(MemMap &mm, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
+                      const uint64_t *c, size_t nout, int *k, size_t idx)
+{
+    static const char *const nm[4] = {"a", "b", "x", "c"};
+    k[0] = mm.kind(a, na * 8); k[1] = mm.kind(b, nb * 8); k[2] = x ? mm.kind(x, nx * 8) : MEM_DEV; k[3] = mm.kind(c, nout * 8);
+    for (int j = 0; j < 4; j++)
+        if (k[j] == MEM_BAD) return nttm_fail(NTTM_EINVAL, "product %zu: operand %s %s", idx, nm[j], mm.why);
+    return 0;
+}
+static bool any_host(const int *k) { return k[0] != MEM_DEV || k[1] != MEM_DEV || k[2] != MEM_DEV || k[3] != MEM_DEV; }
+
+/* one product with host memory among its operands (kinds k): staged, then formed on the device copies */
+static int mul_staged(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
+                      size_t nx, hipStream_t s, size_t lowcut, size_t w, const int *k)
+{
+    const size_t nc_full = na + nb + (x ? 1 : 0), nout = nc_full < w ? nc_full : w;
+    StageSet st;
+    const size_t ia = k[0] != MEM_DEV ? st.add_in(a, na * 8, k[0]) : SIZE_MAX;
+    const size_t ib = k[1] != MEM_DEV ? st.add_in(b, nb * 8, k[1]) : SIZE_MAX;
+    const size_t ix = k[2] != MEM_DEV ? st.add_in(x, nx * 8, k[2]) : SIZE_MAX;
+    const size_t ic = k[3] != MEM_DEV ? st.add_out(c, nout * 8, k[3]) : SIZE_MAX;
+    st.layout();
+    RCK(stage_need(st.bytes));
+    RCK(stage_upload(st, s));
+    RCK(mul_any(ic == SIZE_MAX ? c : st.dout(ic), ia == SIZE_MAX ? a : st.din(ia), na, ib == SIZE_MAX ? b : st.din(ib), nb,
+                ix == SIZE_MAX ? x : st.din(ix), nx, s, lowcut, w));
+    return stage_download(st, s);
+}
+
+/* the entry of the single products: device operands in place, host ones staged */
+static int mul_entry(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
+                     size_t nx, hipStream_t s, size_t lowcut = 0, size_t w = SIZE_MAX)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (x && nx > na + nb) return nttm_fail(NTTM_EINVAL, "nx = %zu exceeds na + nb = %zu", nx, na + nb);
+    const size_t nc_full = na + nb + (x ? 1 : 0), nout = nc_full < w ? nc_full : w;
+    MemMap mm;
+    int k[4];
+    RCK(prod_kinds(mm, a, na, b, nb, x, nx, c, nout, k, 0));
+    if (!any_host(k)) return mul_any(c, a, na, b, nb, x, nx, s, lowcut, w);
+    return mul_staged(c, a, na, b, nb, x, nx, s, lowcut, w, k);
+}
+
+extern "C" int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream)
+{
+    return mul_entry(c, a, na, b, nb, nullptr, 0, (hipStream_t)stream);
+}
+extern "C" int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x,
+                            size_t nx, void *stream)
+{
+    if (!x) return nttm_fail(NTTM_EINVAL, "nttm_mul_add: null x");
+    return mul_entry(c, a, na, b, nb, x, nx, (hipStream_t)stream);
+}
+/* the cut product (E/rns_dist.c:581-589,684-693; E/rns_mul.c rns_mul_local): the operands cut to their low w limbs,
+ * the low min(na + nb, w) limbs of the product written.  A product that fits one transform is formed whole (the
+ * reference skips no piece of it either); a grid skips the pieces ending at or below lowcut. */
+extern "C" int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut,
+                            size_t w, void *stream)
+{
+    const size_t nout = na + nb < w ? na + nb : w;
+    if (lowcut > nout) return nttm_fail(NTTM_EINVAL, "nttm_mul_cut: lowcut %zu above the %zu limbs written", lowcut, nout);
+    return mul_entry(c, a, na < w ? na : w, b, nb < w ? nb : w, nullptr, 0, (hipStream_t)stream, lowcut, w);
+}
+
+extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (!N) return 0;
+    if (!P) return nttm_fail(NTTM_EINVAL, "nttm_mul_batch: null P");
+    hipStream_t s = (hipStream_t)stream;
+    MemMap mm;
+    std::vector<int> k(4 * N);
+    std::vector<size_t> est(N, 0);                               /* staged bytes per product */
+    bool host = false;
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        if (q.x && q.nx > q.na + q.nb) return nttm_fail(NTTM_EINVAL, "product %zu: nx exceeds na + nb", i);
+        const size_t nc = q.na + q.nb + (q.x ? 1 : 0);
+        int *ki = &k[4 * i];
+        RCK(prod_kinds(mm, q.a, q.na, q.b, q.nb, q.x, q.nx, q.c, nc, ki, i));
+        if (!any_host(ki)) continue;
+        host = true;
+        const size_t by[4] = {q.na * 8, q.nb * 8, q.x ? q.nx * 8 : 0, nc * 8};
+        for (int j = 0; j < 4; j++) if (ki[j] != MEM_DEV) est[i] += align_up(by[j], NTTM_STAGE_ALIGN);
+    }
+    if (!host) return mul_batch_dev(P, N, s);
+    /* sub-batches of at most NTTM_STAGE_CHUNK staged bytes */
+    std::vector<size_t> idx;
+    auto run = [&]() -> int {
+        StageSet st;
+        std::vector<size_t> ia(idx.size()), ib(idx.size()), ix(idx.size()), ic(idx.size());
+        for (size_t j = 0; j < idx.size(); j++) {
+            const nttm_prod &q = P[idx[j]];
+            const int *ki = &k[4 * idx[j]];
+            const size_t nc = q.na + q.nb + (q.x ? 1 : 0);
+            ia[j] = ki[0] != MEM_DEV ? st.add_in(q.a, q.na * 8, ki[0]) : SIZE_MAX;
+            ib[j] = ki[1] != MEM_DEV ? st.add_in(q.b, q.nb * 8, ki[1]) : SIZE_MAX;
+            ix[j] = ki[2] != MEM_DEV ? st.add_in(q.x, q.nx * 8, ki[2]) : SIZE_MAX;
+            ic[j] = ki[3] != MEM_DEV ? st.add_out(q.c, nc * 8, ki[3]) : SIZE_MAX;
+        }
+        st.layout();
+        RCK(stage_need(st.bytes));
+        RCK(stage_upload(st, s));
+        std::vector<nttm_prod> D(idx.size());
+        for (size_t j = 0; j < idx.size(); j++) {
+            D[j] = P[idx[j]];
+            if (ia[j] != SIZE_MAX) D[j].a = st.din(ia[j]);
+            if (ib[j] != SIZE_MAX) D[j].b = st.din(ib[j]);
+            if (ix[j] != SIZE_MAX) D[j].x = st.din(ix[j]);
+            if (ic[j] != SIZE_MAX) D[j].c = st.dout(ic[j]);
+        }
+        RCK(mul_batch_dev(D.data(), D.size(), s));
+        RCK(stage_download(st, s));
+        for (size_t j = 0; j < idx.size(); j++) P[idx[j]].ncn = D[j].ncn;
+        idx.clear();
+        return 0;
+    };
+    size_t bytes = 0;
+    for (size_t i = 0; i < N; i++) {
+        if (!idx.empty() && bytes + est[i] > g.stage_chunk) { RCK(run()); bytes = 0; }
+        idx.push_back(i); bytes += est[i];
+    }
+    return idx.empty() ? 0 : run();
+}
+
+/* the same N products, one product call each (the unbatched baseline of the batch tier); host memory staged per
+ * product.  Every call synchronizes, and the normalized lengths are read back limb by limb. */
+extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (N && !P) return nttm_fail(NTTM_EINVAL, "nttm_mul_each: null P");
+    hipStream_t s = (hipStream_t)stream;
+    MemMap mm;
+    std::vector<int> k(4 * N);
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        if (q.x && q.nx > q.na + q.nb) return nttm_fail(NTTM_EINVAL, "product %zu: nx exceeds na + nb", i);
+        const size_t nc = q.na + q.nb + (q.x ? 1 : 0);
+        int *ki = &k[4 * i];
+        RCK(prod_kinds(mm, q.a, q.na, q.b, q.nb, q.x, q.nx, q.c, nc, ki, i));
+        if (any_host(ki)) RCK(mul_staged(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s, 0, SIZE_MAX, ki));
+        else RCK(mul_any(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s));
+        P[i].ncn = nc;
+    }
+    /* normalized lengths: the top limbs read back */
+    for (size_t i = 0; i < N; i++) {
+        size_t n = P[i].ncn;
+        if (k[4 * i + 3] != MEM_DEV) { while (n && !P[i].c[n - 1]) n--; }
+        else while (n) {
+            uint64_t v; HIPCK(hipMemcpy(&v, P[i].c + n - 1, 8, hipMemcpyDeviceToHost));
+            if (v) break;
+            n--;
+        }
+        P[i].ncn = n;
+    }
+    return 0;
+}
+
+/* the same N products, one product call each, launched back to back: no synchronization between products, one at the
+ * end for every product's CRT flags, and the normalized lengths by one k_norm launch over all N.  Device memory only;
+ * a product above one transform (a grid) runs synchronously inside the sequence. */
+extern "C" int nttm_mul_each_async(nttm_prod *P, size_t N, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (!N) return 0;
+    if (!P) return nttm_fail(NTTM_EINVAL, "nttm_mul_each_async: null P");
+    hipStream_t s = (hipStream_t)stream;
+    MemMap mm;
+    size_t ws_max = 0;
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        if (q.x && q.nx > q.na + q.nb) return nttm_fail(NTTM_EINVAL, "product %zu: nx exceeds na + nb", i);
+        const size_t nc = q.na + q.nb + (q.x ? 1 : 0);
+        int ki[4];
+        RCK(prod_kinds(mm, q.a, q.na, q.b, q.nb, q.x, q.nx, q.c, nc, ki, i));
+        if (any_host(ki)) return nttm_fail(NTTM_EINVAL, "nttm_mul_each_async: product %zu: host memory (device memory only)", i);
+        if (q.na && q.nb && fits_one(q.na, q.nb, nc)) { CoreLayout L; RCK(core_layout(q.na, q.nb, q.x != nullptr, SIZE_MAX, &L)); ws_max = std::max(ws_max, L.bytes); }
+    }
+    const size_t off_ncn = align_up(N * sizeof(CrtDesc), 256);
+    RCK(ws_need(ws_max));                                        /* sized once: no reallocation between launches */
+    RCK(hpin_need(128 * N + 8 * N + 256));
+    RCK(stage_need(off_ncn + N * 8));
+    std::vector<char> fl(N, 0);                                  /* 1: flags in pinned slot i */
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        const size_t nc = q.na + q.nb + (q.x ? 1 : 0);
+        if (q.na && q.nb && !fits_one(q.na, q.nb, nc)) RCK(mul_any(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s));
+        else { RCK(mul_core(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s, SIZE_MAX, nullptr, 0, 0, i)); fl[i] = 1; }
+    }
+    std::vector<CrtDesc> cd(N);
+    for (size_t i = 0; i < N; i++) { memset(&cd[i], 0, sizeof(CrtDesc)); cd[i].c = P[i].c; cd[i].nc = P[i].na + P[i].nb + (P[i].x ? 1 : 0); }
+    HIPCK(hipMemcpyAsync(g.stage, cd.data(), N * sizeof(CrtDesc), hipMemcpyHostToDevice, s));
+    RCK(nttm_run_norm(g.stage, N, (uint64_t *)(g.stage + off_ncn), s));
+    uint64_t *hn = (uint64_t *)((char *)g.hpin + 128 * N);
+    HIPCK(hipMemcpyAsync(hn, g.stage + off_ncn, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (size_t i = 0; i < N; i++) {
+        if (fl[i] && (((const uint32_t *)((char *)g.hpin + 128 * i + 96))[1] & 1))
+            return nttm_fail(NTTM_ECHECK, "product %zu: crt: non-zero carry beyond the top limb (wrong residues)", i);
+        P[i].ncn = hn[i];
+    }
+    return 0;
+}
+
 /* ---- the binary-splitting product tree ----------------------------------------------------------------------------------- */
 static int bsp_need(int k, size_t limbs)
 {
@@ -826,8 +1231,14 @@ extern "C" int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, s
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     if (!leaves || !n || !pn || !qn || (p_cap && !p) || (q_cap && !q)) return nttm_fail(NTTM_EINVAL, "nttm_bs_tree arguments");
     hipStream_t s = (hipStream_t)stream;
-    for (size_t i = 0; i < n; i++)
+    MemMap mm;
+    for (size_t i = 0; i < n; i++) {
         if ((leaves[i].pn && !leaves[i].p) || (leaves[i].qn && !leaves[i].q)) return nttm_fail(NTTM_EINVAL, "leaf %zu: null limbs", i);
+        if (mm.kind(leaves[i].p, leaves[i].pn * 8) != MEM_DEV || mm.kind(leaves[i].q, leaves[i].qn * 8) != MEM_DEV)
+            return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: leaf %zu is not device memory of this device", i);
+    }
+    if (mm.kind(p, p_cap * 8) != MEM_DEV || mm.kind(q, q_cap * 8) != MEM_DEV)
+        return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: p or q is not device memory of this device");
     std::vector<nttm_bs_node> cur(leaves, leaves + n), nxt;
     std::vector<nttm_prod> prods;
     for (int level = 0; cur.size() > 1; level++) {
@@ -877,12 +1288,14 @@ extern "C" int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, s
 }
 
 /* ---- transform primitives on canonical uint64 residues ---------------------------------------------------------------- */
-static int prim_check(int prime, const void *x, int logn, size_t batch)
+/* the arguments of a transform primitive: x holds batch transforms of (3 if r3) 2^logn residues in device memory */
+static int prim_check(int prime, const void *x, int logn, size_t batch, int r3 = 0)
 {
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     if (prime < 0 || prime >= NTTM_MAX_PRIMES) return nttm_fail(NTTM_EINVAL, "prime %d", prime);
     if (logn < 1 || logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EINVAL, "logn %d outside [1, %d]", logn, NTTM_LOGN_MAX);
     if (!x || !batch) return nttm_fail(NTTM_EINVAL, "empty transform");
+    DEVCK(x, ((uint64_t)batch << logn) * (r3 ? 3 : 1) * 8, "x");
     return 0;
 }
 extern "C" int nttm_ntt_fwd(int prime, uint64_t *x, int logn, size_t batch, void *stream)
@@ -912,6 +1325,7 @@ extern "C" int nttm_ntt_inv_pw(int prime, uint64_t *x, const uint64_t *y, int ym
     const uint64_t cnt = (uint64_t)batch << logn;
     const uint64_t ycnt = ymode == NTTM_Y_FULL ? cnt : ymode == NTTM_Y_BCAST ? ((uint64_t)1 << logn)
                                                      : ((uint64_t)((batch + 1) / 2) << logn);
+    DEVCK(y, ycnt * 8, "y");
     RCK(ws_need(ycnt * 8));
     RCK(nttm_run_cvt_u64_f64((double *)g.ws, y, ycnt, s));
     RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
@@ -964,7 +1378,7 @@ extern "C" int nttm_ntt_fwd_mfma(int prime, uint64_t *x, int logn, size_t batch,
 /* transforms of 3 2^logk points: the radix-3 layer (nttm_run_r3) and the 2^logk engine on the thirds (batch 3) */
 extern "C" int nttm_ntt_fwd3(int prime, uint64_t *x, int logk, size_t batch, void *stream)
 {
-    RCK(prim_check(prime, x, logk, batch));
+    RCK(prim_check(prime, x, logk, batch, 1));
     hipStream_t s = (hipStream_t)stream;
     const uint64_t cnt = 3 * ((uint64_t)batch << logk);
     RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
@@ -982,7 +1396,7 @@ static int inv3_planes(int prime, double *x, const double *y, int ymode, int log
 }
 extern "C" int nttm_ntt_inv3(int prime, uint64_t *x, int logk, size_t batch, void *stream)
 {
-    RCK(prim_check(prime, x, logk, batch));
+    RCK(prim_check(prime, x, logk, batch, 1));
     hipStream_t s = (hipStream_t)stream;
     const uint64_t cnt = 3 * ((uint64_t)batch << logk);
     RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
@@ -991,12 +1405,13 @@ extern "C" int nttm_ntt_inv3(int prime, uint64_t *x, int logk, size_t batch, voi
 }
 extern "C" int nttm_ntt_inv3_pw(int prime, uint64_t *x, const uint64_t *y, int ymode, int logk, size_t batch, void *stream)
 {
-    RCK(prim_check(prime, x, logk, batch));
+    RCK(prim_check(prime, x, logk, batch, 1));
     if (!y) return nttm_fail(NTTM_EINVAL, "null y");
     if (ymode < NTTM_Y_FULL || ymode > NTTM_Y_PAIR) return nttm_fail(NTTM_EINVAL, "ymode %d", ymode);
     hipStream_t s = (hipStream_t)stream;
     const uint64_t n3 = (uint64_t)3 << logk, cnt = n3 * batch;
     const uint64_t ycnt = ymode == NTTM_Y_FULL ? cnt : ymode == NTTM_Y_BCAST ? n3 : (uint64_t)((batch + 1) / 2) * n3;
+    DEVCK(y, ycnt * 8, "y");
     RCK(ws_need(ycnt * 8));
     RCK(nttm_run_cvt_u64_f64((double *)g.ws, y, ycnt, s));
     RCK(nttm_run_u64_to_f64((double *)x, cnt, s));
@@ -1008,6 +1423,8 @@ extern "C" int nttm_pw(int prime, uint64_t *x, const uint64_t *y, size_t count, 
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     if (prime < 0 || prime >= NTTM_MAX_PRIMES || !x || !y) return nttm_fail(NTTM_EINVAL, "nttm_pw arguments");
     if (!count) return 0;
+    DEVCK(x, count * 8, "x");
+    DEVCK(y, count * 8, "y");
     return nttm_run_pw_u64(x, y, count, g.pk[prime], (hipStream_t)stream);
 }
 extern "C" int nttm_load(int prime, uint64_t *dst, const uint64_t *src, size_t nlimbs, size_t npoints, void *stream)
@@ -1016,6 +1433,8 @@ extern "C" int nttm_load(int prime, uint64_t *dst, const uint64_t *src, size_t n
     if (prime < 0 || prime >= NTTM_MAX_PRIMES || !dst || (nlimbs && !src) || nlimbs > npoints)
         return nttm_fail(NTTM_EINVAL, "nttm_load arguments");
     if (!npoints) return 0;
+    DEVCK(dst, npoints * 8, "dst");
+    DEVCK(src, nlimbs * 8, "src");
     return nttm_run_load_planes((double *)dst, npoints, src, nlimbs, npoints, 1, &g.pk[prime], nullptr, 1, (hipStream_t)stream);
 }
 extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, int np, size_t ncoef, const uint64_t *x,
@@ -1025,6 +1444,9 @@ extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, i
     if (np < 3 || np > NTTM_MAX_PRIMES || !planes || (nc && !c) || (nx && !x)) return nttm_fail(NTTM_EINVAL, "nttm_crt arguments");
     if (ncoef > nc) return nttm_fail(NTTM_EINVAL, "nttm_crt: ncoef %zu exceeds nc %zu", ncoef, nc);
     if (!nc) return 0;
+    DEVCK(c, nc * 8, "c");
+    DEVCK(x, nx * 8, "x");
+    for (int i = 0; i < np; i++) DEVCK(planes[i], ncoef * 8, "plane");
     hipStream_t s = (hipStream_t)stream;
     const size_t pbytes = align_up((size_t)np * ncoef * 8, 256);
     const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(nc));
@@ -1125,6 +1547,7 @@ extern "C" int nttm_profile_read(nttm_kstat *out, int cap)
  * launches with one call. */
 struct nttm_batch_plan {
     size_t N = 0;
+    uint64_t gen = 0;                                            /* the nttm_init it was made under */
     int np = 3;
     std::vector<GroupBuild> gb;
     std::vector<std::vector<size_t>> gidx;
@@ -1208,8 +1631,19 @@ extern "C" int nttm_batch_plan_create(const nttm_prod *P, size_t N, int autotune
     if (!out || (N && !P)) return nttm_fail(NTTM_EINVAL, "nttm_batch_plan_create arguments");
     *out = nullptr;
     hipStream_t s = (hipStream_t)stream;
+    /* the graph has the operands' addresses built in: device memory only */
+    {
+        MemMap mm;
+        for (size_t i = 0; i < N; i++) {
+            const nttm_prod &q = P[i];
+            int k[4];
+            RCK(prod_kinds(mm, q.a, q.na, q.b, q.nb, q.x, q.nx, q.c, q.na + q.nb + (q.x ? 1 : 0), k, i));
+            if (any_host(k)) return nttm_fail(NTTM_EINVAL, "plan: product %zu: host memory (a plan takes device memory)", i);
+        }
+    }
     nttm_batch_plan *pl = new nttm_batch_plan();
     pl->N = N;
+    pl->gen = g.gen;
     auto fail = [&](int rc) { plan_free(pl); return rc; };
     /* validate and class the products by their 2^k length */
     size_t mn_max = 0;
@@ -1315,13 +1749,16 @@ extern "C" int nttm_batch_plan_run(nttm_batch_plan *pl, void *stream)
 {
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     if (!pl) return nttm_fail(NTTM_EINVAL, "null plan");
+    if (pl->gen != g.gen) return nttm_fail(NTTM_ESTATE, "plan made before the last nttm_shutdown: its tables are gone");
     if (!pl->exec) return 0;
     return nttm_hip_check(hipGraphLaunch(pl->exec, (hipStream_t)stream), "graph launch");
 }
 
 extern "C" int nttm_batch_plan_ncn(nttm_batch_plan *pl, size_t *ncn, void *stream)
 {
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     if (!pl || !ncn) return nttm_fail(NTTM_EINVAL, "nttm_batch_plan_ncn arguments");
+    if (pl->gen != g.gen) return nttm_fail(NTTM_ESTATE, "plan made before the last nttm_shutdown: its tables are gone");
     hipStream_t s = (hipStream_t)stream;
     uint64_t *h = pl->hstage;
     size_t w = 0;
@@ -1366,7 +1803,7 @@ extern "C" int nttm_batch_plan_info(const nttm_batch_plan *pl, char *buf, size_t
 extern "C" void nttm_batch_plan_destroy(nttm_batch_plan *pl)
 {
     if (pl && pl->cs) (void)hipStreamSynchronize(pl->cs);
-    if (g.init) (void)hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();                                /* no replay still reading the plan's memory */
     plan_free(pl);
 }
 

@@ -159,6 +159,7 @@ static int lmod_prepare(LmodArgs *A, const uint64_t *q, int nq, uint64_t T)
 extern "C" int nttm_limbs_mod(uint64_t *res, const uint64_t *x, size_t n, const uint64_t *q, int nq, void *stream)
 {
     if (!res || !q || (n && !x)) return nttm_fail(NTTM_EINVAL, "nttm_limbs_mod arguments");
+    RCK(nttm_require_device(x, n * 8, "x"));
     hipStream_t s = (hipStream_t)stream;
     const unsigned blocks = n ? (unsigned)std::min<uint64_t>(1024, (n + LMOD_T - 1) / LMOD_T) : 1;
     LmodArgs A;
@@ -187,6 +188,8 @@ extern "C" int nttm_pack60(uint8_t *out, const uint64_t *x, size_t lo, size_t hi
     if ((uintptr_t)out & 3) return nttm_fail(NTTM_EINVAL, "nttm_pack60: the output must be 4-byte aligned");
     const uint64_t n = hi - lo;
     if (!n) return 0;
+    RCK(nttm_require_device(x + lo, n * 8, "x"));
+    RCK(nttm_require_device(out, nttm_pack60_bytes(n), "out"));
     const uint64_t threads = n / 8 + 1;
     hipLaunchKernelGGL(k_pack60, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, (hipStream_t)stream, out, x,
                        (uint64_t)hi, n);

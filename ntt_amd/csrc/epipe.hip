@@ -444,6 +444,9 @@ extern "C" int nttm_debug_addsub(uint64_t *r, size_t nout, const uint64_t *a, si
                                  size_t nb, size_t sb, int sub, int *ge, void *stream)
 {
     if (!r || !ge || (na && !a) || (nb && !b)) return nttm_fail(NTTM_EINVAL, "nttm_debug_addsub arguments");
+    RCK(nttm_require_device(r, nout * 8, "r"));
+    RCK(nttm_require_device(a, na * 8, "a"));
+    RCK(nttm_require_device(b, nb * 8, "b"));
     hipStream_t s = (hipStream_t)stream;
     uint32_t *scratch = nullptr;
     const size_t words = nttm_crt_scratch_words(nttm_crt_blocks(nout ? nout : 1));

@@ -156,3 +156,54 @@ def test_limbs_to_ascii_formats_18_digits_per_limb():
     want = "".join("%018d" % int(v) for v in limbs[::-1]).encode()
     assert ntt_amd.limbs_to_ascii(limbs) == want
     assert ntt_amd.limbs_to_ascii(np.zeros(0, dtype=np.uint64)) == b""
+
+
+REF_ECALC = "/root/reference/ecalc"
+SHIM = os.path.join(ROOT, "ntt_amd", "ecalc_shim", "rns_nttm.c")
+
+
+def _compile_shim(src, out):
+    import subprocess
+    cmd = ["gcc", "-std=c11", "-Wall", "-Werror", "-c", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + REF_ECALC, "-I" + os.path.join(ROOT, "include"), src, "-o", out]
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_ECALC), reason="the reference tree is absent")
+def test_ecalc_shim_compiles_against_the_reference_headers(tmp_path):
+    """the drop-in shim (rns_init / rns_mul_batch / rns_local_core / the cut products over libnttm) compiles with the
+    reference's own rns_mul.h, dbig.h and fatal.h; its static assertions prove rns_prod and nttm_prod have the same
+    size and field offsets, so the batch passes the caller's records through unchanged"""
+    r = _compile_shim(SHIM, str(tmp_path / "shim.o"))
+    assert r.returncode == 0, r.stderr
+    # the layout check has teeth: a shim that compares a field against the wrong one does not compile
+    bad = tmp_path / "bad.c"
+    bad.write_text(open(SHIM).read().replace("offsetof(rns_prod, ncn) == offsetof(nttm_prod, ncn)",
+                                             "offsetof(rns_prod, ncn) == offsetof(nttm_prod, nx)"))
+    r = _compile_shim(str(bad), str(tmp_path / "bad.o"))
+    assert r.returncode != 0 and "field ncn" in r.stderr
+
+
+def test_wrappers_refuse_bad_tensors():
+    """the Python wrappers check residence, dtype, contiguity and output size before any library call"""
+    import torch
+    a = torch.zeros(8, dtype=torch.int64)
+    with pytest.raises(ValueError, match="CUDA"):
+        ntt_amd.mul(a, a)
+    with pytest.raises(ValueError, match="CUDA"):
+        ntt_amd.mul_add(a, a, a)
+    with pytest.raises(ValueError, match="CUDA"):
+        ntt_amd.mul_cut(a, a, 0, 4)
+    with pytest.raises(ValueError, match="torch tensor"):
+        ntt_amd.mul(np.zeros(4, dtype=np.uint64), a)
+    h = np.zeros(8, dtype=np.uint64)
+    with pytest.raises(ValueError, match="at least 17"):
+        ntt_amd.make_prods([(h, h, h[:3], np.zeros(16, dtype=np.uint64))])
+    with pytest.raises(ValueError, match="int64"):
+        ntt_amd.make_prods([(a.to(torch.int32), a, None, np.zeros(16, dtype=np.uint64))])
+    with pytest.raises(ValueError, match="contiguous"):
+        ntt_amd.make_prods([(torch.zeros(16, dtype=torch.int64)[::2], a, None, np.zeros(16, dtype=np.uint64))])
+    with pytest.raises(ValueError, match="at most na \\+ nb"):
+        ntt_amd.make_prods([(h[:2], h[:2], h, np.zeros(16, dtype=np.uint64))])
+    arr = ntt_amd.make_prods([(h, h, None, np.zeros(16, dtype=np.uint64))])
+    assert arr._keep is not None and arr[0].na == 8 and arr[0].nb == 8
