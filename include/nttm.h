@@ -105,6 +105,12 @@ uint64_t nttm_max_terms(int np);
  * (more than 2^32 points, more terms than the prime set reconstructs, or more workspace than nttm_config caps) is
  * formed as a grid of piece products added into c (the reference's grid, E/rns_dist.c:610-700). */
 int nttm_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream);
+/* c[0 .. na+nb) = a b with no allocation, for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Device memory only; one transform
+ * of at most 2^20 points on three primes (e.g. 2^19 x 2^19 limbs; larger products, four primes or a grid give
+ * NTTM_EINVAL).  The planes and CRT scratch live in a 40 MiB file-scope device array, one per GPU, held until the process
+ * ends; calls that overlap on one GPU share it and race.  Synchronizes the stream. */
+int nttm_mul_noalloc(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream);
 /* c[0 .. na+nb+1) = a b + x, nx <= na + nb; the top limb is the final carry */
 int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
                  const uint64_t *x, size_t nx, void *stream);

@@ -71,6 +71,7 @@ def lib():
         L.nttm_max_terms.argtypes = [i32]; L.nttm_max_terms.restype = u64
         L.nttm_mul.argtypes = [vp, vp, sz, vp, sz, vp]; L.nttm_mul.restype = i32
         L.nttm_mul_add.argtypes = [vp, vp, sz, vp, sz, vp, sz, vp]; L.nttm_mul_add.restype = i32
+        L.nttm_mul_noalloc.argtypes = [vp, vp, sz, vp, sz, vp]; L.nttm_mul_noalloc.restype = i32
         L.nttm_mul_batch.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_batch.restype = i32
         L.nttm_mul_cut.argtypes = [vp, vp, sz, vp, sz, sz, sz, vp]; L.nttm_mul_cut.restype = i32
         L.nttm_bs_tree.argtypes = [C.POINTER(BsNode), sz, vp, sz, C.POINTER(C.c_size_t), vp, sz,
@@ -229,6 +230,16 @@ def mul(a, b, out=None, stream=None):
     c = _dev_limbs(out, "out", na + nb) if out is not None else torch.empty(max(na + nb, 1), dtype=torch.int64, device=a.device)
     _check(lib().nttm_mul(_ptr(c), _ptr(a), na, _ptr(b), nb, _stream(stream)))
     return c[:na + nb]
+
+
+def mul_noalloc(a, b, out, stream=None):
+    """c = a b into out (na + nb limbs, device) through nttm_mul_noalloc: no allocation, products of at most 2^20
+    transform points"""
+    _dev_limbs(a, "a"); _dev_limbs(b, "b")
+    na, nb = a.numel(), b.numel()
+    _dev_limbs(out, "out", na + nb)
+    _check(lib().nttm_mul_noalloc(_ptr(out), _ptr(a) if na else None, na, _ptr(b) if nb else None, nb, _stream(stream)))
+    return out[:na + nb]
 
 
 def mul_add(a, b, x, out=None, stream=None):

@@ -110,7 +110,16 @@ struct Ctx {
     double last_ms[3];
 };
 static Ctx g;
-static uint64_t g_init_count;                /* nttm_init calls so far: a plan made under an earlier one is stale */
+static uint64_t g_init_count;
+static int g_prof_on;                        /* live kernel timing on (nttm_profile) */
+
+/* the workspace of nttm_mul_noalloc: a file-scope device array, one copy per GPU, held until the process ends.  Host
+ * code never uses its name as an address (on the host it is a stand-in, not the device address): nttm_init looks up
+ * the address on its device with hipGetSymbolAddress and keeps it in g_scratch_dev[device]. */
+#define NTTM_NOALLOC_BYTES ((size_t)40 << 20)
+#define NTTM_NOALLOC_MAX_DEV 64
+static __device__ __align__(256) char scratch[NTTM_NOALLOC_BYTES];
+static char *g_scratch_dev[NTTM_NOALLOC_MAX_DEV];                /* nttm_init calls so far: a plan made under an earlier one is stale */
 
 static uint64_t max_terms_np(int np)
 {
@@ -328,6 +337,12 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     if (device < 0 || device >= nd) return nttm_fail(NTTM_EINVAL, "nttm_init: device %d of %d", device, nd);
     HIPCK(hipSetDevice(device));
     g.dev = device;
+    if (device < NTTM_NOALLOC_MAX_DEV && !g_scratch_dev[device]) {   /* once per device, with it current */
+        void *ptr = nullptr;
+        HIPCK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return nttm_fail(NTTM_EHIP, "nttm_init: no device address for the no-allocation workspace");
+        g_scratch_dev[device] = (char *)ptr;
+    }
     {   /* NTTM_STREAM=0: strided passes without the LDS-streaming kernel (for comparison); one streaming block per CU */
         const char *st = getenv("NTTM_STREAM");
         int ncu = 0;
@@ -605,7 +620,7 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     }
     double *A = (double *)ws, *Bp = A + (size_t)L.np * L.n;
     CrtDesc *ddesc = (CrtDesc *)(ws + L.off_desc);
-    uint32_t *scratch = (uint32_t *)(ws + L.off_crt);
+    uint32_t *crt_scr = (uint32_t *)(ws + L.off_crt);
     char *hs = (char *)g.hpin + 128 * slot;                       /* pinned: the descriptor, then (at 96) the flags */
 
     if (sync) HIPCK(hipEventRecord(g.ev[0], s));
@@ -616,10 +631,10 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     hd.trunc = nc < nc_full;
     memcpy(hs, &hd, sizeof hd);
     HIPCK(hipMemcpyAsync(ddesc, hs, sizeof hd, hipMemcpyHostToDevice, s));
-    RCK(nttm_crt_launch_batch(ddesc, 1, nttm_crt_blocks(nc), L.np, g.cc, scratch, s,
+    RCK(nttm_crt_launch_batch(ddesc, 1, nttm_crt_blocks(nc), L.np, g.cc, crt_scr, s,
                               8.0 * L.np * (double)L.ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? (nx < nc ? nx : nc) : 0)));
     if (sync) HIPCK(hipEventRecord(g.ev[3], s));
-    HIPCK(hipMemcpyAsync(hs + 96, scratch, 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(hs + 96, crt_scr, 8, hipMemcpyDeviceToHost, s));
     if (!sync) return 0;
     HIPCK(hipStreamSynchronize(s));
     float ms, tb = 0, t;
@@ -1076,6 +1091,48 @@ extern "C" int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uin
     return mul_entry(c, a, na < w ? na : w, b, nb < w ? nb : w, nullptr, 0, (hipStream_t)stream, lowcut, w);
 }
 
+/* c = a b with no allocation: the same product as nttm_mul on device memory, its planes, CRT descriptor and CRT scratch
+ * in the file-scope array `scratch` of this GPU (NTTM_NOALLOC_BYTES = 40 MiB: one transform of at most 2^20 points on
+ * three primes, e.g. 2^19 x 2^19 limbs).  No hipMalloc, hipFree or other allocating call: the operands are classed by
+ * pointer queries without the per-call cache, live kernel timing is paused for the call, and a product that would
+ * need more (a larger transform, four primes beyond the array, a grid, host operands) is refused with NTTM_EINVAL.
+ * Nothing in the array must start at zero: the limb passes write every plane point before any read, and every CRT
+ * block writes its status word and the carry scan its control words.  Calls that overlap on one GPU share the array
+ * and race; calls in one stream or one after another are safe.  Synchronizes the stream. */
+static int noalloc_dev_ok(const void *p, size_t bytes)
+{
+    if (!p || !bytes) return 1;
+    hipPointerAttribute_t at;
+    memset(&at, 0, sizeof at);
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    if (at.type != hipMemoryTypeDevice || at.device != g.dev) return 0;
+    hipDeviceptr_t base = nullptr; size_t sz = 0;
+    if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)p) != hipSuccess) { (void)hipGetLastError(); return 1; }
+    return (uintptr_t)p + bytes <= (uintptr_t)base + sz;
+}
+extern "C" int nttm_mul_noalloc(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    hipStream_t s = (hipStream_t)stream;
+    if (g.dev >= NTTM_NOALLOC_MAX_DEV || !g_scratch_dev[g.dev]) return nttm_fail(NTTM_ESTATE, "nttm_mul_noalloc: no workspace on device %d", g.dev);
+    if ((na && !a) || (nb && !b) || (na + nb && !c)) return nttm_fail(NTTM_EINVAL, "nttm_mul_noalloc: null pointer");
+    if (!noalloc_dev_ok(a, na * 8) || !noalloc_dev_ok(b, nb * 8) || !noalloc_dev_ok(c, (na + nb) * 8))
+        return nttm_fail(NTTM_EINVAL, "nttm_mul_noalloc: operands must be device memory of this device, inside their allocations");
+    if (na && nb) {
+        if (!fits_one(na, nb, na + nb)) return nttm_fail(NTTM_EINVAL, "nttm_mul_noalloc: a %zu x %zu-limb product needs a grid", na, nb);
+        CoreLayout L; RCK(core_layout(na, nb, false, SIZE_MAX, &L));
+        if (L.bytes > NTTM_NOALLOC_BYTES)
+            return nttm_fail(NTTM_EINVAL, "nttm_mul_noalloc: a %zu x %zu-limb product needs %zu bytes of workspace, above the "
+                             "%zu-byte maximum (one transform of at most 2^20 points on three primes)", na, nb, L.bytes,
+                             (size_t)NTTM_NOALLOC_BYTES);
+    }
+    const int prof0 = g_prof_on;
+    g_prof_on = 0;                                               /* event records could create events */
+    const int rc = mul_core(c, a, na, b, nb, nullptr, 0, s, SIZE_MAX, g_scratch_dev[g.dev], NTTM_NOALLOC_BYTES);
+    g_prof_on = prof0;
+    return rc;
+}
+
 extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
 {
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
@@ -1461,11 +1518,11 @@ extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, i
     HIPCK(hipStreamSynchronize(s));
     memcpy(g.hpin, &hd, sizeof hd);
     CrtDesc *dd = (CrtDesc *)(g.ws + pbytes);
-    uint32_t *scratch = (uint32_t *)(g.ws + pbytes + 256);
+    uint32_t *crt_scr = (uint32_t *)(g.ws + pbytes + 256);
     HIPCK(hipMemcpyAsync(dd, g.hpin, sizeof hd, hipMemcpyHostToDevice, s));
-    RCK(nttm_crt_launch_batch(dd, 1, nttm_crt_blocks(nc), np, g.cc, scratch, s,
+    RCK(nttm_crt_launch_batch(dd, 1, nttm_crt_blocks(nc), np, g.cc, crt_scr, s,
                               8.0 * np * (double)ncoef + 8.0 * (double)nc + 8.0 * (double)(x ? nx : 0)));
-    HIPCK(hipMemcpyAsync(g.hpin + 64, scratch, 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(g.hpin + 64, crt_scr, 8, hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
     const uint32_t flags = ((const uint32_t *)(g.hpin + 64))[1];
     if (flags & 1) return nttm_fail(NTTM_ECHECK, "crt: the value needs more than nc limbs");
@@ -1476,7 +1533,6 @@ extern "C" int nttm_crt(uint64_t *c, size_t nc, const uint64_t *const *planes, i
 struct ProfRec { const char *name; double bytes; hipEvent_t a, b; };
 static std::vector<ProfRec> g_prof;
 static std::vector<hipEvent_t> g_evpool;
-static int g_prof_on;
 static hipEvent_t ev_get(void)
 {
     if (!g_evpool.empty()) { hipEvent_t e = g_evpool.back(); g_evpool.pop_back(); return e; }
