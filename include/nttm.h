@@ -254,6 +254,9 @@ int nttm_debug_product_len(size_t ncoef, int radix3, int tile_log, int *logn, in
  * the difference is >= 0).  Synchronous. */
 int nttm_debug_addsub(uint64_t *r, size_t nout, const uint64_t *a, size_t na, size_t sa, const uint64_t *b, size_t nb,
                       size_t sb, int sub, int *ge, void *stream);
+/* synthetic operand limbs on the device: x[j] = splitmix64 value j + 1 of the stream seeded `seed`, mod 10^18, a zero
+ * top limb set to 1 (the reference's generator, E/tests/harness.h:55-81).  Asynchronous. */
+int nttm_debug_gen_limbs(uint64_t *x, size_t n, uint64_t seed, void *stream);
 /* the global element indices of tile `tile` of a pass of that geometry; returns the tile's element count */
 int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out);
 

@@ -112,6 +112,7 @@ def lib():
         L.nttm_debug_product_len.restype = i32
         L.nttm_debug_addsub.argtypes = [vp, sz, vp, sz, sz, vp, sz, sz, i32, C.POINTER(C.c_int), vp]
         L.nttm_debug_addsub.restype = i32
+        L.nttm_debug_gen_limbs.argtypes = [vp, sz, u64, vp]; L.nttm_debug_gen_limbs.restype = i32
         L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
         L.nttm_debug_tile_indices.restype = i32
         _lib = L
@@ -309,6 +310,15 @@ def debug_addsub(a, b, nout, sa=0, sb=0, sub=False, stream=None):
                                    _ptr(b) if b.numel() else None, b.numel(), sb, 1 if sub else 0, C.byref(ge),
                                    _stream(stream)))
     return r[:nout], bool(ge.value)
+
+
+def gen_limbs_dev(n, seed, stream=None):
+    """n synthetic limbs generated on the device (an int64 tensor): the splitmix64 stream seeded `seed`, mod 10^18,
+    top limb nonzero -- the same limbs as the oracle's and the bench's host generator"""
+    import torch
+    x = torch.empty(max(n, 1), dtype=torch.int64, device="cuda")
+    _check(lib().nttm_debug_gen_limbs(_ptr(x), n, seed, _stream(stream)))
+    return x[:n]
 
 
 def limbs_to_ascii(limbs):

@@ -17,6 +17,7 @@
 #include <string.h>
 #include <stddef.h>
 #include <vector>
+#include <algorithm>
 #include "internal.h"
 
 #define P60_MASK ((1ULL << 60) - 1)
@@ -178,6 +179,33 @@ extern "C" int nttm_limbs_mod(uint64_t *res, const uint64_t *x, size_t n, const 
     for (unsigned b = 0; b < blocks; b++)
         for (int j = 0; j < nq; j++) { res[j] += hp[(size_t)b * LMOD_MAXQ + j]; if (res[j] >= q[j]) res[j] -= q[j]; }
     return 0;
+}
+
+/* ---- synthetic operands on the device --------------------------------------------------------------------------------
+ * limb j of an operand is value j + 1 of the splitmix64 stream seeded `seed`, reduced mod B, and a zero top limb is
+ * set to 1 -- the reference's operand generator (E/tests/harness.h:55-81: splitmix64, limb = next() % B, top limb
+ * nonzero).  Lets a product of up to 2^31 x 2^31 limbs be made and checked (by residues) without a host copy. */
+__global__ __launch_bounds__(256) void k_gen_limbs(uint64_t *x, uint64_t n, uint64_t seed)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += stride) {
+        uint64_t z = seed + (j + 1) * 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        uint64_t q, r;
+        div_b(0, z, &q, &r);
+        x[j] = (j + 1 == n && r == 0) ? 1 : r;
+    }
+}
+extern "C" int nttm_debug_gen_limbs(uint64_t *x, size_t n, uint64_t seed, void *stream)
+{
+    if (!n) return 0;
+    if (!x) return nttm_fail(NTTM_EINVAL, "nttm_debug_gen_limbs: null x");
+    RCK(nttm_require_device(x, n * 8, "x"));
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_gen_limbs, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, x, (uint64_t)n, seed);
+    return nttm_hip_check(hipGetLastError(), "k_gen_limbs launch");
 }
 
 extern "C" uint64_t nttm_pack60_bytes(size_t n) { return 15 * (uint64_t)(n / 2) + 8 * (uint64_t)(n & 1); }

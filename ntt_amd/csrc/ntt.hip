@@ -117,7 +117,8 @@ __device__ __forceinline__ void for_slots(const PassPos &pp, int s_lo, F f)
         const uint32_t pe = PAD ? eb + (eb >> 5) : eb;
         const uint32_t c = eb & ((1u << pp.logw) - 1), r = (eb >> pp.logw) & ((1u << pp.m) - 1);
         const uint32_t h = eb >> (pp.logw + pp.m);
-        const uint32_t ob = (h << (pp.lo + pp.m)) + (r << pp.lo) + c;
+        /* h << (lo + m) in 64 bits: a tile of the first pass of a 2^32-point transform has lo + m = 32 (and h = 0) */
+        const uint32_t ob = (uint32_t)((uint64_t)h << (pp.lo + pp.m)) + (r << pp.lo) + c;
         const uint32_t kb = pp.m ? (__brev(r) >> (32 - pp.m)) : 0u;
         const int ksh = pp.m - s_lo - LOGR;
 #pragma unroll

@@ -76,6 +76,11 @@ def lib():
         L.orc_gmp_tref_edges.restype = C.c_int
         L.orc_gmp_check.argtypes = [_u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t]
         L.orc_gmp_check.restype = C.c_int
+        L.orc_gmp_full_check.argtypes = [_u64p, C.c_size_t, _u64p, C.c_size_t, _u64p, C.c_size_t, C.POINTER(C.c_double)]
+        L.orc_gmp_full_check.restype = C.c_int
+        L.orc_gmp_check_batch.argtypes = [C.POINTER(_u64p), C.POINTER(C.c_size_t), C.POINTER(_u64p), C.POINTER(C.c_size_t),
+                                          C.POINTER(_u64p), C.POINTER(C.c_size_t), C.c_size_t, C.POINTER(C.c_uint8)]
+        L.orc_gmp_check_batch.restype = C.c_size_t
         L.orc_gmp_time_batch.argtypes = [C.POINTER(_u64p), C.POINTER(C.c_size_t), C.POINTER(_u64p), C.POINTER(C.c_size_t),
                                          C.c_size_t, C.c_int, C.c_int]
         L.orc_gmp_time_batch.restype = C.c_double
@@ -304,6 +309,31 @@ def gmp_time_batch(pairs, threads=1, reps=1):
     na = (C.c_size_t * n)(*[len(a) for a, _ in arrs])
     nb = (C.c_size_t * n)(*[len(b) for _, b in arrs])
     return float(lib().orc_gmp_time_batch(pa, na, pb, nb, n, threads, reps))
+
+
+def gmp_check_batch(triples):
+    """[ok_i]: c_i == a_i b_i exactly for every (a, b, c) of triples, checked on every host thread"""
+    arrs = [tuple(np.ascontiguousarray(v, dtype=np.uint64) for v in t) for t in triples]
+    n = len(arrs)
+    ptr = lambda k: (_u64p * n)(*[_p(t[k]) for t in arrs])
+    ln = lambda k: (C.c_size_t * n)(*[len(t[k]) for t in arrs])
+    ok = (C.c_uint8 * max(n, 1))()
+    r = lib().orc_gmp_check_batch(ptr(0), ln(0), ptr(1), ln(1), ptr(2), ln(2), n, ok)
+    if r == C.c_size_t(-1).value:
+        raise RuntimeError("oracle built without GMP")
+    return [bool(ok[i]) for i in range(n)]
+
+
+def gmp_full_check(a, b, c):
+    """(equal, seconds to convert a and b, seconds of one mpz_mul, seconds to convert c): the exact check of a large
+    product against GMP with the mpz_mul timed on the same operands, progress on stderr"""
+    a = np.ascontiguousarray(a, dtype=np.uint64); b = np.ascontiguousarray(b, dtype=np.uint64)
+    c = np.ascontiguousarray(c, dtype=np.uint64)
+    t = (C.c_double * 3)()
+    r = lib().orc_gmp_full_check(_p(a), len(a), _p(b), len(b), _p(c), len(c), t)
+    if r < 0:
+        raise RuntimeError("oracle built without GMP")
+    return bool(r), t[0], t[1], t[2]
 
 
 def tref_res(a, b, c):

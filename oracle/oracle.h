@@ -108,6 +108,12 @@ double orc_gmp_time_batch(const uint64_t *const *a, const size_t *na, const uint
                           int threads, int reps);
 /* the residue form's parts (3) low 64 limbs exact and (4) top limbs within [AhBh, AhBh + Ah + Bh]; 1 if c passes */
 int orc_gmp_tref_edges(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc);
+/* n products checked exactly on every host thread: ok[i] = 1 when c_i = a_i b_i; returns the failures ((size_t)-1 without
+ * GMP) */
+size_t orc_gmp_check_batch(const uint64_t *const *a, const size_t *na, const uint64_t *const *b, const size_t *nb,
+                           const uint64_t *const *c, const size_t *nc, size_t n, uint8_t *ok);
+/* a and b converted (t[0] s), one mpz_mul timed (t[1] s), c converted (t[2] s); 1 when c = a b, progress on stderr */
+int orc_gmp_full_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc, double *t);
 /* 1 if c equals a b + x exactly (x optional), by GMP */
 int orc_gmp_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
                   const uint64_t *c, size_t nc);

@@ -2,6 +2,7 @@
  * Conversions between base-10^18 limbs and mpz by a product tree: value = lo + hi B^h with h = 2^k, n/2 <= h < n, the
  * powers B^(2^k) cached (the scheme of E/tests/harness.h:104-199); halves run as OpenMP tasks above a size threshold.
  * Built without GMP (no gmp.h), every entry reports its absence. */
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -35,6 +36,8 @@ static const mpz_t *pow2k(int k)                              /* B^(2^k) */
 }
 static int split_k(size_t n) { int k = 0; while (((size_t)2 << k) < n) k++; return k; }
 
+static int g_progress;                                        /* 1: report the top combines on stderr */
+static double now_s(void);
 static void from_rec(mpz_t z, const uint64_t *a, size_t n, int par)
 {
     if (n <= LEAF) {
@@ -48,6 +51,7 @@ static void from_rec(mpz_t z, const uint64_t *a, size_t n, int par)
     from_rec(t, a, h, par);
     from_rec(z, a + h, n - h, par);
 #pragma omp taskwait
+    if (g_progress && n >= ((size_t)1 << 25)) { fprintf(stderr, "[oracle] combine %zu limbs at %.1f s\n", n, now_s()); fflush(stderr); }
     mpz_mul(z, z, *pow2k(k)); mpz_add(z, z, t);
     mpz_clear(t);
 }
@@ -95,6 +99,30 @@ static void mpz_to_limbs_n(uint64_t *a, size_t n, const mpz_t z)
     } else to_rec(a, n, z, 0);
 }
 static double now_s(void) { struct timespec t; clock_gettime(CLOCK_MONOTONIC, &t); return t.tv_sec + 1e-9 * t.tv_nsec; }
+
+/* the full-size CPU baseline and exact check of one large product, reporting each step on stderr: a and b converted
+ * (t[0] seconds, all host threads), one mpz_mul (t[1], one thread), c converted (t[2]); returns 1 when c = a b exactly,
+ * 0 when not (limbs >= B included) */
+int orc_gmp_full_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc, double *t)
+{
+    for (size_t i = 0; i < nc; i++) if (c[i] >= ORC_B) return 0;
+    g_progress = 1;
+    mpz_t x, y, z, w; mpz_inits(x, y, z, w, NULL);
+    double t0 = now_s();
+    fprintf(stderr, "[oracle] converting a and b (%zu, %zu limbs)\n", na, nb); fflush(stderr);
+    mpz_from_limbs(x, a, na); mpz_from_limbs(y, b, nb);
+    t[0] = now_s() - t0;
+    fprintf(stderr, "[oracle] converted in %.1f s; mpz_mul\n", t[0]); fflush(stderr);
+    t0 = now_s(); mpz_mul(z, x, y); t[1] = now_s() - t0;
+    fprintf(stderr, "[oracle] mpz_mul %.1f s; converting c (%zu limbs)\n", t[1], nc); fflush(stderr);
+    mpz_clears(x, y, NULL);
+    t0 = now_s(); mpz_from_limbs(w, c, nc); t[2] = now_s() - t0;
+    const int ok = mpz_cmp(z, w) == 0;
+    fprintf(stderr, "[oracle] c converted in %.1f s; equal %d\n", t[2], ok); fflush(stderr);
+    mpz_clears(z, w, NULL);
+    g_progress = 0;
+    return ok;
+}
 static int cmp_d(const void *x, const void *y) { double a = *(const double *)x, b = *(const double *)y; return (a > b) - (a < b); }
 
 int orc_have_gmp(void) { return 1; }
@@ -167,6 +195,29 @@ int orc_gmp_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, co
     return ok;
 }
 
+/* n products checked on every host thread (one product per thread at a time): ok[i] = c_i == a_i b_i exactly and every
+ * limb of c_i below B; returns the number that fail */
+size_t orc_gmp_check_batch(const uint64_t *const *a, const size_t *na, const uint64_t *const *b, const size_t *nb,
+                           const uint64_t *const *c, const size_t *nc, size_t n, uint8_t *ok)
+{
+    size_t bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+    for (long i = 0; i < (long)n; i++) {
+        int good = 1;
+        for (size_t k = 0; k < nc[i] && good; k++) if (c[i][k] >= ORC_B) good = 0;
+        if (good) {
+            mpz_t x, y, z, w; mpz_inits(x, y, z, w, NULL);
+            from_rec(x, a[i], na[i], 0); from_rec(y, b[i], nb[i], 0); mpz_mul(z, x, y);
+            from_rec(w, c[i], nc[i], 0);
+            good = mpz_cmp(z, w) == 0;
+            mpz_clears(x, y, z, w, NULL);
+        }
+        ok[i] = (uint8_t)good;
+        bad += !good;
+    }
+    return bad;
+}
+
 static size_t norm_len(const uint64_t *a, size_t n) { while (n && a[n - 1] == 0) n--; return n; }
 
 /* E/tests/tref.h:59-78: the low 64 limbs exact; floor(c / B^(ka+kb)) within [Ah Bh, Ah Bh + Ah + Bh] */
@@ -210,4 +261,9 @@ int orc_gmp_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, co
 { (void)a; (void)na; (void)b; (void)nb; (void)x; (void)nx; (void)c; (void)nc; return -1; }
 int orc_gmp_tref_edges(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc)
 { (void)a; (void)na; (void)b; (void)nb; (void)c; (void)nc; return -1; }
+int orc_gmp_full_check(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *c, size_t nc, double *t)
+{ (void)a; (void)na; (void)b; (void)nb; (void)c; (void)nc; (void)t; return -1; }
+size_t orc_gmp_check_batch(const uint64_t *const *a, const size_t *na, const uint64_t *const *b, const size_t *nb,
+                           const uint64_t *const *c, const size_t *nc, size_t n, uint8_t *ok)
+{ (void)a; (void)na; (void)b; (void)nb; (void)c; (void)nc; (void)n; (void)ok; return (size_t)-1; }
 #endif

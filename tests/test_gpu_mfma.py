@@ -62,3 +62,53 @@ def test_product_through_mfma_stage(nttm):
     want = nttm.mul(da, db)
     assert torch.equal(c, want)
     assert stats["k_dft16_mfma"][0] == 6
+
+
+def test_fwd_mfma_equals_fwd_2e28(nttm):
+    """config 4's size: one prime's forward transform of 2^28 points (the 2^27-limb operand, seed 0x5EED0003, loaded
+    mod the prime) through the matrix-core stage equals the FP64 transform bit for bit"""
+    import torch
+    n = 1 << 27
+    a, _ = O.gen_limbs(n, 0x5EED0003)
+    da = dev(a)
+    x = torch.empty(1 << 28, dtype=torch.int64, device="cuda")
+    nttm.load(1, x, da)
+    del da
+    y = x.clone()
+    say("fwd mfma 2^28")
+    nttm.ntt_fwd(1, x, 28)
+    nttm.ntt_fwd_mfma(1, y, 28)
+    torch.cuda.synchronize()
+    assert torch.equal(x, y)
+
+
+def test_product_2e27_through_mfma_stage_residue_form(nttm):
+    """the 2^27 x 2^27-limb product (seeds 0x5EED0003 / 0x5EED0004) with all six forward transforms through the
+    matrix-core stage passes the reference checker's residue form and both 64-limb edges against GMP -- checked
+    against the operands, not against the FP64 path"""
+    import torch
+    n = 1 << 27
+    a, _ = O.gen_limbs(n, 0x5EED0003)
+    b, _ = O.gen_limbs(n, 0x5EED0004)
+    da, db = dev(a), dev(b)
+    planes = []
+    say("2^27 product through the mfma stage")
+    for i in range(3):
+        pa = torch.empty(1 << 28, dtype=torch.int64, device="cuda")
+        pb = torch.empty_like(pa)
+        nttm.load(i, pa, da)
+        nttm.load(i, pb, db)
+        nttm.ntt_fwd_mfma(i, pa, 28)
+        nttm.ntt_fwd_mfma(i, pb, 28)
+        nttm.ntt_inv_pw(i, pa, pb, 0, 28)
+        del pb
+        planes.append(pa)
+    del da, db
+    c = nttm.crt(planes, 2 * n - 1, 2 * n)
+    torch.cuda.synchronize()
+    del planes
+    ch = host(c)
+    del c
+    assert O.tref_res(a, b, ch)
+    if O.have_gmp():
+        assert O.tref_edges(a, b, ch)

@@ -304,6 +304,27 @@ def test_gmp_bridge_and_checker():
     assert O.gmp_time_batch([], 2, 1) >= 0
 
 
+@pytest.mark.skipif(not O.have_gmp(), reason="GMP absent")
+def test_gmp_batch_and_full_checks():
+    """the batch checker and the full-size checker agree with the schoolbook on exact products and reject a product
+    with one limb changed by one, a limb >= B and a truncated result"""
+    trip = []
+    for i in range(12):
+        a, s = O.gen_limbs(10 + 97 * i, 300 + i); b, _ = O.gen_limbs(700 - 31 * i, s)
+        trip.append((a, b, O.mul_school(a, b)))
+    assert all(O.gmp_check_batch(trip))
+    bad = [(a, b, c.copy()) for a, b, c in trip[:3]]
+    bad[0][2][5] += np.uint64(1)
+    bad[1][2][0] = np.uint64(B)
+    bad[2] = (bad[2][0], bad[2][1], bad[2][2][:-1])
+    assert O.gmp_check_batch(bad + trip[3:4]) == [False, False, False, True]
+    a, b, c = trip[7]
+    ok, t_conv, t_mul, t_c = O.gmp_full_check(a, b, c)
+    assert ok and min(t_conv, t_mul, t_c) >= 0
+    d = c.copy(); d[len(d) // 2] ^= np.uint64(1)
+    assert not O.gmp_full_check(a, b, d)[0]
+
+
 def test_checker_catches_single_limb_faults():
     # the reference checker's self-test (E/tests/tref.h:105-126): low, middle and top limbs changed
     a, s = O.gen_limbs(700, 21); b, _ = O.gen_limbs(650, s)
