@@ -738,6 +738,11 @@ def main():
         torch.cuda.empty_cache()
         epipe = bench_e(nttm, args.e_digits)
 
+    sweep = None
+    if not args.no_sweep:
+        torch.cuda.empty_cache()
+        sweep = bench_sweep(nttm, log_hi=args.sweep_max)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (GMP)", file=sys.stderr, flush=True)
@@ -761,7 +766,7 @@ def main():
                        "kernels_per_step": kernels, "deterministic": deterministic,
                        "vs_baseline_ref": "time ratio to 0.1263 s, 1 MI300A (results/D13b_strategy_e0.txt:26)",
                        "plain_product_2e20": cfg1, "leaves": leaves, "e_pipeline": epipe,
-                       "split_precision_stage": cfg4},
+                       "split_precision_stage": cfg4, "size_sweep": sweep},
             "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     nttm.shutdown()
