@@ -246,6 +246,9 @@ int nttm_profile_read(nttm_kstat *out, int cap);
  * tile_log / last_log: log2 of points per tile of the strided passes / of the last (contiguous) pass (0: defaults;
  * the product path uses the configured tile_log and 12) */
 int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_log, int *out, int cap);
+/* the pass plan a product transform of 2^logn points (batch of them) takes: 4 ints (lo, m, logw, logh) per pass, tile_log
+ * the configured strided tile (0: default); the last pass may take fewer than 12 bits in 4096-point tiles */
+int nttm_debug_product_plan(int logn, uint64_t batch, int tile_log, int *out, int cap);
 /* the transform length the products take for ncoef = na + nb - 1 coefficients under a radix3 rule (NTTM_R3_*) and
  * tile_log (0: default): 2^logn (r3 = 0) or 3 2^logn (r3 = 1) points */
 int nttm_debug_product_len(size_t ncoef, int radix3, int tile_log, int *logn, int *r3);

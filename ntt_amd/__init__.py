@@ -11,7 +11,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "libnttm.so")
+LIB_PATH = os.environ.get("NTTM_LIB") or os.path.join(HERE, "libnttm.so")   # NTTM_LIB: another build of the library
 HEADER = os.path.join(ROOT, "include", "nttm.h")
 
 B = 10 ** 18
@@ -108,6 +108,8 @@ def lib():
         L.nttm_profile.argtypes = [i32]; L.nttm_profile.restype = i32
         L.nttm_profile_read.argtypes = [C.POINTER(KStat), i32]; L.nttm_profile_read.restype = i32
         L.nttm_debug_plan.argtypes = [i32, u64, i32, i32, C.POINTER(C.c_int), i32]; L.nttm_debug_plan.restype = i32
+        L.nttm_debug_product_plan.argtypes = [i32, u64, i32, C.POINTER(C.c_int), i32]
+        L.nttm_debug_product_plan.restype = i32
         L.nttm_debug_product_len.argtypes = [sz, i32, i32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.nttm_debug_product_len.restype = i32
         L.nttm_debug_addsub.argtypes = [vp, sz, vp, sz, sz, vp, sz, sz, i32, C.POINTER(C.c_int), vp]
@@ -197,6 +199,15 @@ def max_terms(np_):
 def debug_plan(logn, batch=1, last_log=0, tile_log=13):
     out = (C.c_int * 24)()
     n = lib().nttm_debug_plan(logn, batch, tile_log, last_log, out, 24)
+    return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
+
+
+def debug_product_plan(logn, batch=1, tile_log=0):
+    """the (lo, m, logw, logh) passes of a product transform of 2^logn points"""
+    out = (C.c_int * 24)()
+    n = lib().nttm_debug_product_plan(logn, batch, tile_log, out, 24)
+    if n < 0:
+        _check(n)
     return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
 
 

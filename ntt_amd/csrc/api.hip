@@ -459,6 +459,22 @@ static int product_tile_log(int logn, int tl = 0)
     return k_13 < k_cfg ? 13 : tl;
 }
 
+/* bits of the double-fused last pass of a 2^logn product transform (its tiles stay 2^12 points): 12, or 10 and 11 for
+ * 2^26 and 2^27 points, so that their two strided passes take 8 bits each (8+8+10, 8+8+11, 8+8+12 up to 2^28) and run
+ * as the 8 x 32 streamed shape -- a level moved out of the fused pass (instruction-bound: both forward transforms and
+ * the inverse run there) costs less in an HBM-bound strided pass.  Measured per pass at 2^28 points on one MI355X:
+ * 8-bit streamed passes 0.78-0.85 ms, 9-bit (16-column rows) 0.96-1.14 ms, the fused pass 2.08 ms at 12 bits and 1.68
+ * at 10, so 9+9+10 loses to 8+8+12 there, and 2^29 / 2^30 points keep 9+8+12 / 9+9+12. */
+static int fused_bits(int logn, int tile_log)
+{
+    if (tile_log != 13 || logn < 26 || logn > 28) return NTTM_FUSED_LOG;
+    return logn - 16;
+}
+static void product_plan(Plan *pl, int logn, uint64_t batch, int tile_log)
+{
+    plan_make3(pl, logn, batch, tile_log, fused_bits(logn, tile_log), NTTM_FUSED_LOG);
+}
+
 /* the transforms of cnt products of one length: 2^logn points (r3 = 0), or 3 2^logn points (r3 = 1: a radix-3 layer,
  * reading the limbs, and the 2^logn engine on the 3 cnt thirds).  Plane i of A holds the cnt transforms of prime i back
  * to back.  After it, A holds the cyclic convolutions mod each prime, natural order.  B has one plane, reused by every
@@ -473,8 +489,8 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
 {
     if (!bmap) { cnt_b = cnt; ldesc_b = ldesc; }
     const uint64_t tb = r3 ? 3 * cnt : cnt, tbb = r3 ? 3 * cnt_b : cnt_b;   /* engine transforms of 2^logn points */
-    Plan pl; plan_make2(&pl, logn, tb, product_tile_log(logn), NTTM_FUSED_LOG);
-    Plan plb; plan_make2(&plb, logn, tbb, product_tile_log(logn), NTTM_FUSED_LOG);
+    Plan pl; product_plan(&pl, logn, tb, product_tile_log(logn));
+    Plan plb; product_plan(&plb, logn, tbb, product_tile_log(logn));
     const uint64_t n = (uint64_t)1 << logn, plane = tb * n, ntot = r3 ? 3 * n : n;
     const int last = pl.npass - 1;
     if (bmap && (plb.npass != pl.npass || pl.p[last].m + pl.p[last].logh > logn || (last == 0 && !r3)))
@@ -524,7 +540,7 @@ static double sched_bytes(int logn, int r3, int tile_log)
     const bool cacheable = logn >= 0 && logn <= NTTM_LOGN_MAX && (tile_log == 12 || tile_log == 13);
     double *m = cacheable ? &memo[logn][r3 ? 1 : 0][tile_log - 12] : nullptr;
     if (m && *m > 0) return *m;
-    Plan pl; plan_make2(&pl, logn, 1, product_tile_log(logn, tile_log), NTTM_FUSED_LOG);
+    Plan pl; product_plan(&pl, logn, 1, product_tile_log(logn, tile_log));
     const int P = pl.npass;
     const double v = r3 ? 3.0 * (2 * (8 + 16 * (P - 1)) + 24 + 16 * (P - 1) + 16)
                         : (P >= 2 ? 2 * (8 + 16 * (P - 2)) + 24 + 16 * (P - 1) : 8);
@@ -762,7 +778,7 @@ static void group_build(GroupBuild &gb, const nttm_prod *P, const std::vector<si
      * E/rns_mul.c:476-495) transform it once, where B has planes (a transform of more than one pass, or a radix-3
      * length) and the double-fused tiles lie inside one transform */
     std::vector<uint32_t> slot(cnt);
-    Plan pq; plan_make2(&pq, logn, r3 ? 3 * cnt : cnt, product_tile_log(logn, tile), NTTM_FUSED_LOG);
+    Plan pq; product_plan(&pq, logn, r3 ? 3 * cnt : cnt, product_tile_log(logn, tile));
     const bool can = logn >= NTTM_FUSED_LOG && (pq.npass > 1 || r3);
     std::map<std::pair<const uint64_t *, uint64_t>, uint32_t> seen;
     for (uint64_t j = 0; j < cnt && can; j++) {
@@ -1869,6 +1885,16 @@ extern "C" int nttm_debug_plan(int logn, uint64_t batch, int tile_log, int last_
     if (tile_log <= 0) tile_log = NTTM_TILE_LOG_DEFAULT;
     if (tile_log > NTTM_TILE_LOG || last_log > NTTM_TILE_LOG) return nttm_fail(NTTM_EINVAL, "tile logs above %d", NTTM_TILE_LOG);
     Plan pl; plan_make2(&pl, logn, batch, tile_log, last_log > 0 ? last_log : tile_log);
+    int k = 0;
+    for (int i = 0; i < pl.npass && k + 4 <= cap; i++) { out[k++] = pl.p[i].lo; out[k++] = pl.p[i].m; out[k++] = pl.p[i].logw; out[k++] = pl.p[i].logh; }
+    return pl.npass;
+}
+extern "C" int nttm_debug_product_plan(int logn, uint64_t batch, int tile_log, int *out, int cap)
+{
+    if (tile_log <= 0) tile_log = NTTM_TILE_LOG_DEFAULT;
+    if (tile_log != 12 && tile_log != 13) return nttm_fail(NTTM_EINVAL, "tile_log %d", tile_log);
+    if (logn < 1 || logn > NTTM_LOGN_MAX || !batch) return nttm_fail(NTTM_EINVAL, "nttm_debug_product_plan arguments");
+    Plan pl; product_plan(&pl, logn, batch, product_tile_log(logn, tile_log));
     int k = 0;
     for (int i = 0; i < pl.npass && k + 4 <= cap; i++) { out[k++] = pl.p[i].lo; out[k++] = pl.p[i].m; out[k++] = pl.p[i].logw; out[k++] = pl.p[i].logh; }
     return pl.npass;

@@ -405,7 +405,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 /* ---- the streaming strided pass ---------------------------------------------------------------------------------------
- * Strided passes (lo >= GW) of the fixed tile shape 2^GM rows x 2^GW columns (H = 1, 2^13 points, 16 per thread): one
+ * Strided passes (lo >= GW) of the fixed tile shapes 2^GM rows x 2^GW columns, 8 x 5 or 9 x 4 (H = 1, 2^13 points, 16
+ * per thread; the 9-bit rows are 128-byte runs, in three rounds of 3 bits instead of two of 4): one
  * block per CU walks the tiles blockIdx.x, + gridDim.x, ...  The next tile is fetched straight into the other of two
  * LDS buffers (global_load_lds, 16 bytes per lane, the tile in natural row-major order) while the current one is
  * transformed, so HBM streams during the butterflies.  Vector-memory operations of a wave complete in issue order, so
@@ -418,7 +419,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
  *   - at the top of an iteration at most the previous tile's 16 stores may remain outstanding (vmcnt 16), which
  *     leaves exactly the fetch of the tile about to be transformed complete.
  * The boundary twiddle is applied as a Barrett product (both factors variable, |x|, |w| <= p), the forward on the
- * store (after the last round, bits [0, 4)), the inverse on the load (before the first round, the same bits). */
+ * store (after the last round, bits [0, 4)), the inverse on the load (before the first round, the same bits).  The
+ * 9-bit form has two groups of 8 points per thread in that round and takes its chains from bound_chain (the same
+ * base * step^i construction per group, from the LDS copy of the boundary table). */
 #define STREAM_T 512
 /* a workgroup barrier for LDS only: the wave's LDS operations complete, then s_barrier.  The compiler places a wait
  * for every outstanding vector-memory operation (an LDS fetch included) before a barrier it sees; written as one asm
@@ -435,7 +438,8 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
     constexpr bool INV = MODE == PM_INV, LIMB = MODE == PM_FWD_LIMB;
     constexpr int PTS = 16, T = 1 << (GM + GW), W = 1 << GW, NBQ = 4 << NTTM_BQ_LOG;
     static_assert(T == PTS * STREAM_T, "2^13-point tiles, 16 points per thread");
-    static_assert(GM == 8, "two rounds of 4 levels");
+    static_assert(GM == 8 || GM == 9, "two rounds of 4 levels, or three of 3");
+    constexpr int NR = GM == 8 ? 2 : 3, LRL = GM == 8 ? 4 : 3;      /* rounds, and bits per round */
     __shared__ double buf0[T], buf1[T];
     __shared__ double2 stw[1 << GM], bqs[NBQ];
     const int t = threadIdx.x;
@@ -498,7 +502,7 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
         double *xt = a.x + gbase;
 
         double v[PTS], w[PTS];
-        {   /* boundary twiddles of the bits-[0, 4) round: w[i] = base * step^i (a limb pass's n^-1 folded into base) */
+        if (GM == 8) {   /* boundary twiddles of the bits-[0, 4) round: w[i] = base * step^i (a limb pass's n^-1 folded into base) */
             const uint32_t col = colbase + c_in, sh = 32 - lom;
             double base = bq_twiddle(bqs, (col * kb) << sh, p, pinv);
             if (LIMB && a.do_in_scale) base = mul_shoup(base, a.in_scale.x, a.in_scale.y, p);
@@ -510,42 +514,46 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
         }
         int s_lo, logR;
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
+        for (int i = 0; i < NR; i++) {
             round_split(GM, i, &s_lo, &logR, INV, 4);
             if (i > 0) lds_barrier();
             if (LIMB && i == 0) {                   /* limbs to residues, zero beyond the operand */
                 const uint32_t lim = limb_lim(gbase);
                 const uint64_t *cl = (const uint64_t *)cur;
-                /* slot j of the first round holds row bits 4..7 = j, so rows >= 2^(GM-1) are slots j >= 8; tiles
-                 * whose rows split into an all-inside and an all-outside half (the zero-padded upper half of a
-                 * product's operand) convert only the inside slots */
+                /* slot j of a group of the first round holds the top LRL row bits = j, so rows >= 2^(GM-1) are the
+                 * group's slots j >= 2^(LRL-1); tiles whose rows split into an all-inside and an all-outside half (the
+                 * zero-padded upper half of a product's operand) convert only the inside slots */
                 const uint32_t half = 1u << (GM - 1 + lo);
                 if (lim <= half && lim > half - (1u << lo) + W) {
-                    for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
-                        v[sl] = sl < PTS / 2 ? limb_to_res(cl[e], p, pinv) : 0.0; });
+                    for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
+                        v[sl] = (sl & ((1 << LRL) - 1)) < (1 << (LRL - 1)) ? limb_to_res(cl[e], p, pinv) : 0.0; });
                 } else {
-                    for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
+                    for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
                         v[sl] = o < lim ? limb_to_res(o + 1 == lim ? last : cl[e], p, pinv) : 0.0; });
                 }
             } else
-                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { v[sl] = cur[e]; });
+                for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { v[sl] = cur[e]; });
             if (INV && i == 0) {
+                if (GM == 8) {
 #pragma unroll
-                for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
+                    for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
+                } else bound_chain<LRL, PTS>(v, pp, lom, colbase, bqs, p, pinv);
             }
-            round_compute<4, INV, PTS>(v, pp, s_lo, stw, p, pinv);
-            if (i == 0)
-                for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { cur[e] = v[sl]; });
+            round_compute<LRL, INV, PTS>(v, pp, s_lo, stw, p, pinv);
+            if (i + 1 < NR)
+                for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { cur[e] = v[sl]; });
         }
         if (!INV) {
+            if (GM == 8) {
 #pragma unroll
-            for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
+                for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
+            } else bound_chain<LRL, PTS>(v, pp, lom, colbase, bqs, p, pinv, LIMB && a.do_in_scale, a.in_scale);
         }
         if (INV && a.do_scale) {                                       /* the inverse's final n^-1 */
 #pragma unroll
             for (int j = 0; j < PTS; j++) v[j] = mul_shoup(v[j], a.scale.x, a.scale.y, p);
         }
-        for_slots<4, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t, uint32_t) { xt[o] = v[sl]; });
+        for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t, uint32_t) { xt[o] = v[sl]; });
     }
     __builtin_amdgcn_s_waitcnt(0);                                     /* no LDS fetch in flight past the block's end */
 }
@@ -695,9 +703,12 @@ template <int PTS, int GM = -1, int GH = -1, bool LIMBS = true>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused2(PassArgs a, PassArgs bq)
 {
     __shared__ double xs_raw[LDS_WORDS(1 << FUSED_LOG)];
-    /* the fixed shape keeps the stage table's low entries in LDS (the inverse mirrors them) */
+    /* the fixed shapes (4096-point tiles of 2^GM-point blocks, GM = 10, 11 or 12) keep the stage table's low entries in
+     * LDS (the inverse mirrors them): the forward rounds after the first and the first two inverse rounds cover stages
+     * below 2^STW_LOG for each of them (rounds of 4, 3, 3 / 4, 4, 3 / 4, 4, 4 bits from bit 0) */
     constexpr bool LT = GM >= 0 && PTS >= 16;
-    static_assert(!LT || GM == 12, "the LDS stage table serves the rounds of 4 bits of a 4096-point tile");
+    static_assert(!LT || (GM >= 10 && GM <= 12 && GM + GH == FUSED_LOG),
+                  "the LDS stage table serves the 4096-point tiles of 2^10, 2^11 and 2^12-point blocks");
     __shared__ double2 stw_lds[LT ? 1 << STW_LOG : 1];
     const SwzLds xs{xs_raw};
     constexpr int MAXR = PTS >= 16 ? 4 : 3;
@@ -964,15 +975,22 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P, M, W, H>), grid, blk, lds, s, b); break; }
     /* the strided tile shapes of large transforms, compiled with their geometry fixed */
     const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
-    const bool stream = fixed && g_stream && g.m == 8 && g.logw == 5 && g.lo >= 5 && mode != PM_INV_PW &&
+    const bool shape8 = g.m == 8 && g.logw == 5, shape9 = g.m == 9 && g.logw == 4;
+    const bool stream = fixed && g_stream && (shape8 || shape9) && g.lo >= g.logw && mode != PM_INV_PW &&
                         (mode == PM_INV || !b.do_scale) &&
                         (mode != PM_FWD_LIMB || (!b.ldesc && b.batch == 1 && g.lo + g.m < 32));
     if (stream) {
         const unsigned nb = (unsigned)(tiles < (uint64_t)g_ncu ? tiles : (uint64_t)g_ncu);
         const uint32_t nt = (uint32_t)tiles;
-        if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-        else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-        else hipLaunchKernelGGL((k_pass_stream<PM_INV, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+        if (shape8) {
+            if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+            else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+            else hipLaunchKernelGGL((k_pass_stream<PM_INV, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+        } else {
+            if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, 9, 4>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+            else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, 9, 4>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+            else hipLaunchKernelGGL((k_pass_stream<PM_INV, 9, 4>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+        }
     }
     else if (fixed && ppt == PASS_PTS && g.m == 8 && g.logw == 5) { NTTM_PASS_LAUNCH_G(PASS_PTS, 8, 5, 0) }
     else if (T >= PASS_PTS && ppt == 8) { NTTM_PASS_LAUNCH(8) }
@@ -1034,12 +1052,20 @@ int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hi
     const int ppt = (T >= 64 * g_pts && T <= 512 * g_pts) ? g_pts : PASS_PTS;
     const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
     if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
-    /* the fixed shape (a 4096-point last pass of a transform of more than one pass) reads no limbs */
-    const bool fixed = g_fixed_shapes && g.m == FUSED_LOG && g.logh == 0 && !alimb && !blimb;
-    if (ppt == 8 && fixed) hipLaunchKernelGGL((k_fused2<8, FUSED_LOG, 0, false>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
-    else if (ppt == 8) hipLaunchKernelGGL((k_fused2<8>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
-    else if (fixed) hipLaunchKernelGGL((k_fused2<PASS_PTS, FUSED_LOG, 0, false>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
-    else hipLaunchKernelGGL((k_fused2<PASS_PTS>), dim3((unsigned)tiles), dim3(nthr), 0, s, A, Bq);
+    /* the fixed shapes (4096-point tiles of the last pass of a transform of more than one pass: 2^12-point blocks,
+     * or 4 x 2^10 and 2 x 2^11 where the strided passes took more bits) read no limbs */
+    const bool fixed = g_fixed_shapes && g.m + g.logh == FUSED_LOG && g.m >= 10 && !alimb && !blimb;
+    const dim3 grid((unsigned)tiles), blk(nthr);
+    if (fixed && ppt == 8) {
+        if (g.m == 12) hipLaunchKernelGGL((k_fused2<8, 12, 0, false>), grid, blk, 0, s, A, Bq);
+        else if (g.m == 11) hipLaunchKernelGGL((k_fused2<8, 11, 1, false>), grid, blk, 0, s, A, Bq);
+        else hipLaunchKernelGGL((k_fused2<8, 10, 2, false>), grid, blk, 0, s, A, Bq);
+    } else if (fixed) {
+        if (g.m == 12) hipLaunchKernelGGL((k_fused2<PASS_PTS, 12, 0, false>), grid, blk, 0, s, A, Bq);
+        else if (g.m == 11) hipLaunchKernelGGL((k_fused2<PASS_PTS, 11, 1, false>), grid, blk, 0, s, A, Bq);
+        else hipLaunchKernelGGL((k_fused2<PASS_PTS, 10, 2, false>), grid, blk, 0, s, A, Bq);
+    } else if (ppt == 8) hipLaunchKernelGGL((k_fused2<8>), grid, blk, 0, s, A, Bq);
+    else hipLaunchKernelGGL((k_fused2<PASS_PTS>), grid, blk, 0, s, A, Bq);
     prof_end(s);
     return nttm_hip_check(hipGetLastError(), "k_fused2 launch");
 }

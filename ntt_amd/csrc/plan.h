@@ -26,12 +26,13 @@ struct Plan { int logn, npass; PassGeom p[NTTM_MAX_PASSES]; };
 NTTM_PHD int plan_ctz64(uint64_t x) { int c = 0; if (!x) return 63; while (!(x & 1)) { x >>= 1; c++; } return c; }
 
 /* passes in forward order (highest bits first); tile_log = log2 of points per tile of the strided passes, last_log
- * that of the last (contiguous) pass */
-NTTM_PHD void plan_make2(Plan *pl, int logn, uint64_t batch, int tile_log, int last_log)
+ * that of the last (contiguous) pass, last_bits the bits that pass takes (at most last_log; its tiles then hold
+ * 2^(last_log - last_bits) consecutive blocks) */
+NTTM_PHD void plan_make3(Plan *pl, int logn, uint64_t batch, int tile_log, int last_bits, int last_log)
 {
     int bz = plan_ctz64(batch);
     if (bz > 40) bz = 40;
-    int m0 = logn < last_log ? logn : last_log;
+    int m0 = logn < last_bits ? logn : last_bits;
     int R = logn - m0;
     const int smax = tile_log - 4;
     int k = (R + smax - 1) / smax;
@@ -51,6 +52,10 @@ NTTM_PHD void plan_make2(Plan *pl, int logn, uint64_t batch, int tile_log, int l
     int room = last_log - m0, blocks = (logn - m0) + bz;
     g.logh = room < blocks ? room : blocks;
     pl->p[k] = g;
+}
+NTTM_PHD void plan_make2(Plan *pl, int logn, uint64_t batch, int tile_log, int last_log)
+{
+    plan_make3(pl, logn, batch, tile_log, last_log, last_log);
 }
 NTTM_PHD void plan_make(Plan *pl, int logn, uint64_t batch, int tile_log) { plan_make2(pl, logn, batch, tile_log, tile_log); }
 

@@ -147,6 +147,44 @@ def test_product_plan_shape():
     assert ntt_amd.debug_plan(28, 1, 12, 12) == [(20, 8, 4, 0), (12, 8, 4, 0), (0, 12, 0, 0)]
 
 
+def test_product_plans():
+    """the product transforms: 2^26 .. 2^28 points put 8 bits into each strided pass (the 8 x 32 streamed shape) and
+    10 to 12 into the last, in 4096-point tiles; 2^29 and 2^30 take 9-bit strided passes (the 9 x 16 streamed shape);
+    other lengths keep a 12-bit last pass; every plan covers each element of the batch once (the kernels' own index
+    map, sampled tiles at the large sizes)"""
+    P = ntt_amd.debug_product_plan
+    assert P(26) == [(18, 8, 5, 0), (10, 8, 5, 0), (0, 10, 0, 2)]
+    assert P(27) == [(19, 8, 5, 0), (11, 8, 5, 0), (0, 11, 0, 1)]
+    assert P(28) == [(20, 8, 5, 0), (12, 8, 5, 0), (0, 12, 0, 0)]
+    assert P(29) == [(20, 9, 4, 0), (12, 8, 5, 0), (0, 12, 0, 0)]
+    assert P(30) == [(21, 9, 4, 0), (12, 9, 4, 0), (0, 12, 0, 0)]
+    assert P(21)[-1] == (0, 12, 0, 0) and P(32)[-1] == (0, 12, 0, 0) and P(17, 64)[-1][:2] == (0, 12)
+    for logn in range(20, 33):
+        for batch in (1, 3):
+            passes = P(logn, batch)
+            assert sum(m for _, m, _, _ in passes) == logn
+            top, total = logn, batch << logn
+            for lo, m, logw, logh in passes:
+                assert lo + m == top and m + logw + logh <= 13 and (lo == 0 or logw >= 4)
+                if lo == 0:
+                    assert m + logh <= 12 and m >= 10 or logn <= 12
+                top = lo
+                T = 1 << (m + logw + logh)
+                assert total % T == 0
+                tiles = total // T
+                for t in (0, 1, tiles // 3, tiles - 1):
+                    idx = ntt_amd.debug_tile_indices(lo, m, logw, logh, t)
+                    assert max(idx) < total and len(set(idx)) == T
+    full = P(14, 8)                                               # small enough to check every element
+    seen = np.zeros(8 << 14, dtype=np.int8)
+    for lo, m, logw, logh in full:
+        seen[:] = 0
+        T = 1 << (m + logw + logh)
+        for t in range((8 << 14) // T):
+            seen[np.array(ntt_amd.debug_tile_indices(lo, m, logw, logh, t), dtype=np.int64)] += 1
+        assert (seen == 1).all()
+
+
 def test_limbs_to_ascii_formats_18_digits_per_limb():
     """the digit formatter of the wrapper (the text form of e's digits in the tests and the bench): top limb first, 18
     zero-padded digits per limb, against Python's own formatting"""
