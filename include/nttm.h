@@ -58,7 +58,7 @@ enum {
     NTTM_EHIP = -3,        /* a HIP runtime call failed */
     NTTM_EBOUND = -4,      /* the product exceeds what the prime set reconstructs, or the length exceeds 2^32 */
     NTTM_ESTATE = -5,      /* not initialised */
-    NTTM_ECHECK = -6,      /* a device-side post-condition failed (non-zero carry beyond the top limb) */
+    NTTM_ECHECK = -6,      /* a post-condition failed (a carry beyond the top limb, or the residue check) */
     NTTM_EIO = -7          /* a file could not be written */
 };
 
@@ -78,6 +78,10 @@ typedef struct nttm_config {
     int tile_log;               /* points per tile of the strided transform passes, log2: 12 or 13 (0: default) */
     int pts;                    /* transform points per thread: 8 or 16 (0: default) */
     int radix3;                 /* NTTM_R3_*: when products use 3 2^k lengths (0: NTTM_R3_AUTO) */
+    int check;                  /* 1: every whole product (single, batch, each) is checked afterwards: c = a b + x modulo
+                                 * the prime 4611686018427388039, NTTM_ECHECK on a mismatch (the reference's residue
+                                 * identity of "detect and retry", E/dr.h:1-8); costs one read of a, b, x and c.  Not
+                                 * applied by cut products, plan replays or nttm_mul_noalloc.  0: off (default) */
 } nttm_config;
 
 /* one product of a batch: c = a b (+ x).  c has na + nb limbs, or na + nb + 1 when x is given (nx <= na + nb),
@@ -122,6 +126,11 @@ int nttm_mul_add(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, s
  * No carry beyond limb w-1 is checked. */
 int nttm_mul_cut(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t lowcut, size_t w,
                  void *stream);
+/* the residue post-condition on demand: NTTM_ECHECK unless c (nc limbs) = a b + x modulo 4611686018427388039 (the
+ * first prime above 2^62; x may be null); device memory; synchronizes.  No counterpart in the reference's API (its
+ * checks run inside the product, E/dr.h:1-8). */
+int nttm_check_product(const uint64_t *c, size_t nc, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                       const uint64_t *x, size_t nx, void *stream);
 /* N independent products; P is HOST memory; its a, b, x, c may each be device or host memory (host arrays staged, see
  * the top; a batch with more than 1 GiB of host arrays runs as consecutive sub-batches).  Products are grouped by
  * transform length and run batched; ncn is filled on return (the call synchronizes the stream). */

@@ -45,7 +45,7 @@ R3_AUTO, R3_NEVER, R3_FIT = 0, 1, 2
 
 class Config(C.Structure):
     _fields_ = [("nprimes", C.c_int), ("workspace_bytes", C.c_size_t), ("tile_log", C.c_int), ("pts", C.c_int),
-                ("radix3", C.c_int)]
+                ("radix3", C.c_int), ("check", C.c_int)]
 
 
 _lib = None
@@ -85,6 +85,7 @@ def lib():
         L.nttm_limbs_mod.restype = i32
         L.nttm_e_write_packed.argtypes = [C.c_char_p, C.POINTER(C.c_uint64), sz, u64, vp]
         L.nttm_e_write_packed.restype = i32
+        L.nttm_check_product.argtypes = [vp, sz, vp, sz, vp, sz, vp, sz, vp]; L.nttm_check_product.restype = i32
         L.nttm_mul_each.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each.restype = i32
         L.nttm_mul_each_async.argtypes = [C.POINTER(Prod), sz, vp]; L.nttm_mul_each_async.restype = i32
         L.nttm_batch_plan_create.argtypes = [C.POINTER(Prod), sz, i32, C.POINTER(vp), vp]
@@ -225,8 +226,8 @@ def debug_tile_indices(lo, m, logw, logh, tile):
 
 
 # ---- device entry points ------------------------------------------------------------------------------------------------
-def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0, pts=0, radix3=R3_AUTO):
-    cfg = Config(nprimes, workspace_bytes, tile_log, pts, radix3)
+def init(device=0, nprimes=0, workspace_bytes=0, tile_log=0, pts=0, radix3=R3_AUTO, check=0):
+    cfg = Config(nprimes, workspace_bytes, tile_log, pts, radix3, check)
     _check(lib().nttm_init(device, C.byref(cfg)))
 
 
@@ -399,6 +400,18 @@ def make_prods(items):
         arr[i] = Prod(pa, na, pb, nb, pc, px, nx, 0)
     arr._keep = list(items)
     return arr
+
+
+def check_product(c, a, b, x=None, stream=None):
+    """True when c = a b + x modulo 4611686018427388039 (device tensors; the residue post-condition), else False"""
+    rc = lib().nttm_check_product(_ptr(c) if c.numel() else None, c.numel(), _ptr(a) if a.numel() else None, a.numel(),
+                                  _ptr(b) if b.numel() else None, b.numel(),
+                                  _ptr(x) if x is not None and x.numel() else None, 0 if x is None else x.numel(),
+                                  _stream(stream))
+    if rc == -6:
+        return False
+    _check(rc)
+    return True
 
 
 def mul_batch(prods, stream=None):

@@ -102,6 +102,7 @@ struct Ctx {
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     char *stage; size_t stage_bytes;         /* device copies of host-resident operands and results */
     size_t stage_chunk;                      /* staged bytes per sub-batch of a batch */
+    char *chk; size_t chk_bytes;             /* device scratch of the residue post-condition */
     char *bounce; size_t bounce_half;        /* pinned bounce buffer of the host copies: two halves */
     hipEvent_t bev[2]; int bev_live[2];      /* the last copy through each half */
     uint64_t gen;                            /* which nttm_init this context is (plans record it) */
@@ -327,6 +328,7 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     if (g.cfg.pts && g.cfg.pts != 8 && g.cfg.pts != 16) return nttm_fail(NTTM_EINVAL, "nttm_init: pts must be 0, 8 or 16");
     if (g.cfg.radix3 < NTTM_R3_AUTO || g.cfg.radix3 > NTTM_R3_FIT)
         return nttm_fail(NTTM_EINVAL, "nttm_init: radix3 must be 0, 1 or 2");
+    if (g.cfg.check != 0 && g.cfg.check != 1) return nttm_fail(NTTM_EINVAL, "nttm_init: check must be 0 or 1");
     nttm_set_pts(g.cfg.pts ? g.cfg.pts : 16);
     {   /* NTTM_GENERIC_PASSES=1: runtime-geometry pass kernels only (for comparison) */
         const char *e = getenv("NTTM_GENERIC_PASSES");
@@ -376,6 +378,7 @@ extern "C" void nttm_shutdown(void)
     if (g.tables) (void)hipFree(g.tables);
     if (g.hpin) (void)hipHostFree(g.hpin);
     if (g.stage) (void)hipFree(g.stage);
+    if (g.chk) (void)hipFree(g.chk);
     if (g.bounce) (void)hipHostFree(g.bounce);
     for (int k = 0; k < 2; k++) if (g.bev[k]) (void)hipEventDestroy(g.bev[k]);
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
@@ -663,6 +666,77 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     return 0;
 }
 
+/* ---- the residue post-condition ------------------------------------------------------------------------------------
+ * c = a b + x modulo the prime NTTM_CHECK_Q = 4611686018427388039 (the first prime above 2^62).  The reference's
+ * "detect and retry" checks the same identity on residues it tracks through the run (level 2, E/dr.h:1-8,
+ * ALGORITHM.md:441-458, modulo the same prime DR_Q); here the residues are computed from the arrays themselves on the
+ * device after the product, every array of a call in 64Ki-limb chunks by one kernel (k_res_chunks), the chunks of an
+ * array weighted by powers of B^65536 on the host.  Costs one read of a, b, x and c. */
+#define NTTM_CHECK_Q 4611686018427388039ULL
+struct ResChunkH { const uint64_t *x; uint64_t n; };          /* must match digits.hip's ResChunk */
+int nttm_run_res_chunks(const void *chunks_dev, uint64_t nchunks, uint64_t q, uint64_t *out_dev, hipStream_t s);
+uint64_t nttm_res_chunk_limbs(void);
+static int chk_need(size_t bytes)
+{
+    if (bytes <= g.chk_bytes) return 0;
+    if (g.chk) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.chk)); g.chk = nullptr; g.chk_bytes = 0; }
+    if (hipMalloc(&g.chk, bytes) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "residue check: %zu bytes", bytes); }
+    g.chk_bytes = bytes;
+    return 0;
+}
+/* res[i] = (sum_k arr[i][k] B^k) mod NTTM_CHECK_Q for device arrays; synchronizes */
+static int residues_q(uint64_t *res, const uint64_t *const *arr, const size_t *len, size_t count, hipStream_t s)
+{
+    const uint64_t CH = nttm_res_chunk_limbs(), q = NTTM_CHECK_Q;
+    std::vector<ResChunkH> ch;
+    std::vector<size_t> owner;
+    for (size_t i = 0; i < count; i++) {
+        res[i] = 0;
+        for (size_t off = 0; off < len[i]; off += CH) {
+            ch.push_back({arr[i] + off, std::min<uint64_t>(CH, len[i] - off)});
+            owner.push_back(i);
+        }
+    }
+    if (ch.empty()) return 0;
+    const size_t off_out = align_up(ch.size() * sizeof(ResChunkH), 256);
+    RCK(chk_need(off_out + ch.size() * 8));
+    std::vector<uint64_t> out(ch.size());
+    HIPCK(hipMemcpyAsync(g.chk, ch.data(), ch.size() * sizeof(ResChunkH), hipMemcpyHostToDevice, s));
+    RCK(nttm_run_res_chunks(g.chk, ch.size(), q, (uint64_t *)(g.chk + off_out), s));
+    HIPCK(hipMemcpyAsync(out.data(), g.chk + off_out, ch.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    const uint64_t bch = h_powmod(NTTM_B % q, CH, q);
+    uint64_t w = 1;
+    for (size_t k = 0; k < ch.size(); k++) {
+        if (k == 0 || owner[k] != owner[k - 1]) w = 1;
+        res[owner[k]] = (uint64_t)(((u128)out[k] * w + res[owner[k]]) % q);
+        w = h_mulmod(w, bch, q);
+    }
+    return 0;
+}
+/* the post-condition over N products on device memory: NTTM_ECHECK naming the first that fails */
+static int check_products(const nttm_prod *P, size_t N, hipStream_t s)
+{
+    std::vector<const uint64_t *> arr(4 * N);
+    std::vector<size_t> len(4 * N);
+    for (size_t i = 0; i < N; i++) {
+        const nttm_prod &q = P[i];
+        arr[4 * i] = q.a; len[4 * i] = q.na;
+        arr[4 * i + 1] = q.b; len[4 * i + 1] = q.nb;
+        arr[4 * i + 2] = q.x; len[4 * i + 2] = q.x ? q.nx : 0;
+        arr[4 * i + 3] = q.c; len[4 * i + 3] = q.na + q.nb + (q.x ? 1 : 0);
+    }
+    std::vector<uint64_t> r(4 * N);
+    RCK(residues_q(r.data(), arr.data(), len.data(), 4 * N, s));
+    for (size_t i = 0; i < N; i++) {
+        const uint64_t want = (uint64_t)(((u128)r[4 * i] * r[4 * i + 1] + r[4 * i + 2]) % NTTM_CHECK_Q);
+        if (want != r[4 * i + 3])
+            return nttm_fail(NTTM_ECHECK, "product %zu: c differs from a b + x modulo %llu (the residue post-condition)", i,
+                             (unsigned long long)NTTM_CHECK_Q);
+    }
+    return 0;
+}
+
 /* ---- products above one transform: the grid (E/rns_dist.c:610-700, our own form) ---------------------------------------
  * A product that no single transform holds (more than 2^32 points, the prime set's term bound, or the configured
  * workspace cap) is formed from ka x kb piece products a_i b_j (pieces of ceil(na/ka) and ceil(nb/kb) limbs).  The grid
@@ -731,6 +805,16 @@ static int mul_any(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b,
     if (x && nx > na + nb) return nttm_fail(NTTM_EINVAL, "nx = %zu exceeds na + nb = %zu", nx, na + nb);
     if (!c || !a || !b || (nx && !x)) return nttm_fail(NTTM_EINVAL, "null pointer");
     return mul_grid(c, a, na, b, nb, x, nx, s, lowcut, w);
+}
+/* mul_any and, with nttm_config.check, the residue post-condition on a whole product (w = SIZE_MAX) */
+static int mul_full(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, const uint64_t *x, size_t nx,
+                    hipStream_t s, size_t lowcut = 0, size_t w = SIZE_MAX)
+{
+    RCK(mul_any(c, a, na, b, nb, x, nx, s, lowcut, w));
+    if (!g.cfg.check || w != SIZE_MAX) return 0;
+    nttm_prod q; memset(&q, 0, sizeof q);
+    q.a = a; q.na = na; q.b = b; q.nb = nb; q.c = c; q.x = x; q.nx = x ? nx : 0;
+    return check_products(&q, 1, s);
 }
 
 extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
@@ -896,7 +980,7 @@ static int mul_batch_dev(nttm_prod *P, size_t N, hipStream_t s)
         while (n) { uint64_t v; HIPCK(hipMemcpy(&v, q.x + n - 1, 8, hipMemcpyDeviceToHost)); if (v) break; n--; }
         q.ncn = n;
     }
-    return 0;
+    return g.cfg.check ? check_products(P, N, s) : 0;
 }
 
 /* ---- host-resident operands: staging through HBM -----------------------------------------------------------------------
@@ -1067,8 +1151,8 @@ static int mul_staged(uint64_t *c, const uint64_t *a, size_t na, const uint64_t 
     st.layout();
     RCK(stage_need(st.bytes));
     RCK(stage_upload(st, s));
-    RCK(mul_any(ic == SIZE_MAX ? c : st.dout(ic), ia == SIZE_MAX ? a : st.din(ia), na, ib == SIZE_MAX ? b : st.din(ib), nb,
-                ix == SIZE_MAX ? x : st.din(ix), nx, s, lowcut, w));
+    RCK(mul_full(ic == SIZE_MAX ? c : st.dout(ic), ia == SIZE_MAX ? a : st.din(ia), na, ib == SIZE_MAX ? b : st.din(ib), nb,
+                 ix == SIZE_MAX ? x : st.din(ix), nx, s, lowcut, w));
     return stage_download(st, s);
 }
 
@@ -1082,7 +1166,7 @@ static int mul_entry(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *
     MemMap mm;
     int k[4];
     RCK(prod_kinds(mm, a, na, b, nb, x, nx, c, nout, k, 0));
-    if (!any_host(k)) return mul_any(c, a, na, b, nb, x, nx, s, lowcut, w);
+    if (!any_host(k)) return mul_full(c, a, na, b, nb, x, nx, s, lowcut, w);
     return mul_staged(c, a, na, b, nb, x, nx, s, lowcut, w, k);
 }
 
@@ -1147,6 +1231,22 @@ extern "C" int nttm_mul_noalloc(uint64_t *c, const uint64_t *a, size_t na, const
     const int rc = mul_core(c, a, na, b, nb, nullptr, 0, s, SIZE_MAX, g_scratch_dev[g.dev], NTTM_NOALLOC_BYTES);
     g_prof_on = prof0;
     return rc;
+}
+
+/* the residue post-condition on demand: c (nc limbs) = a b + x modulo NTTM_CHECK_Q; device memory; synchronizes */
+extern "C" int nttm_check_product(const uint64_t *c, size_t nc, const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                                  const uint64_t *x, size_t nx, void *stream)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if ((nc && !c) || (na && !a) || (nb && !b) || (nx && !x)) return nttm_fail(NTTM_EINVAL, "nttm_check_product: null pointer");
+    DEVCK(c, nc * 8, "c"); DEVCK(a, na * 8, "a"); DEVCK(b, nb * 8, "b"); DEVCK(x, nx * 8, "x");
+    const uint64_t *arr[4] = {a, b, x, c};
+    const size_t len[4] = {na, nb, x ? nx : 0, nc};
+    uint64_t r[4];
+    RCK(residues_q(r, arr, len, 4, (hipStream_t)stream));
+    if ((uint64_t)(((u128)r[0] * r[1] + r[2]) % NTTM_CHECK_Q) != r[3])
+        return nttm_fail(NTTM_ECHECK, "c differs from a b + x modulo %llu (the residue post-condition)", (unsigned long long)NTTM_CHECK_Q);
+    return 0;
 }
 
 extern "C" int nttm_mul_batch(nttm_prod *P, size_t N, void *stream)
@@ -1226,7 +1326,7 @@ extern "C" int nttm_mul_each(nttm_prod *P, size_t N, void *stream)
         int *ki = &k[4 * i];
         RCK(prod_kinds(mm, q.a, q.na, q.b, q.nb, q.x, q.nx, q.c, nc, ki, i));
         if (any_host(ki)) RCK(mul_staged(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s, 0, SIZE_MAX, ki));
-        else RCK(mul_any(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s));
+        else RCK(mul_full(q.c, q.a, q.na, q.b, q.nb, q.x, q.nx, s));
         P[i].ncn = nc;
     }
     /* normalized lengths: the top limbs read back */
@@ -1286,7 +1386,7 @@ extern "C" int nttm_mul_each_async(nttm_prod *P, size_t N, void *stream)
             return nttm_fail(NTTM_ECHECK, "product %zu: crt: non-zero carry beyond the top limb (wrong residues)", i);
         P[i].ncn = hn[i];
     }
-    return 0;
+    return g.cfg.check ? check_products(P, N, s) : 0;
 }
 
 /* ---- the binary-splitting product tree ----------------------------------------------------------------------------------- */

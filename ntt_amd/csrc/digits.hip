@@ -157,6 +157,49 @@ static int lmod_prepare(LmodArgs *A, const uint64_t *q, int nq, uint64_t T)
     return 0;
 }
 
+/* ---- residues of many limb arrays, for the products' post-condition ------------------------------------------------
+ * Each chunk (at most RES_CHUNK limbs of one array) is one block: thread t sums x_k B^k for k = t, t + 256, ... of the
+ * chunk (B^k relative to the chunk's start) mod q in Montgomery arithmetic, a block sum gives the chunk's residue.  The
+ * caller weights chunk j of an array by B^(j RES_CHUNK) and adds. */
+#define RES_CHUNK 65536
+struct ResChunk { const uint64_t *x; uint64_t n; };
+__global__ __launch_bounds__(LMOD_T) void k_res_chunks(const ResChunk *ch, LmodQ Q, uint64_t *out)
+{
+    __shared__ uint64_t s_acc[LMOD_T / 64];
+    const ResChunk c = ch[blockIdx.x];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint64_t q = Q.q, qn = Q.qneg;
+    uint64_t pw = Q.one, base = Q.bm;                               /* B^t R by square and multiply */
+    for (uint32_t e = (uint32_t)t; e; e >>= 1) {
+        if (e & 1) pw = mont_mul(pw, base, q, qn);
+        base = mont_mul(base, base, q, qn);
+    }
+    uint64_t acc = 0;
+    for (uint64_t k = (uint64_t)t; k < c.n; k += LMOD_T) {
+        acc += mont_mul(c.x[k], pw, q, qn); if (acc >= q) acc -= q;
+        pw = mont_mul(pw, Q.bTm, q, qn);
+    }
+    for (int d = 32; d; d >>= 1) { const uint64_t o = __shfl_down(acc, d, 64); acc += o; if (acc >= q) acc -= q; }
+    if (lane == 0) s_acc[wave] = acc;
+    __syncthreads();
+    if (t == 0) {
+        uint64_t v = 0;
+        for (int w = 0; w < LMOD_T / 64; w++) { v += s_acc[w]; if (v >= q) v -= q; }
+        out[blockIdx.x] = v;
+    }
+}
+uint64_t nttm_res_chunk_limbs(void) { return RES_CHUNK; }
+/* out_dev[j] = the residue mod q of chunk j (chunks_dev: nchunks ResChunk records, each n <= RES_CHUNK) */
+int nttm_run_res_chunks(const void *chunks_dev, uint64_t nchunks, uint64_t q, uint64_t *out_dev, hipStream_t s)
+{
+    if (!nchunks) return 0;
+    if (nchunks > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "residues: %llu chunks", (unsigned long long)nchunks);
+    LmodArgs A;
+    RCK(lmod_prepare(&A, &q, 1, LMOD_T));
+    hipLaunchKernelGGL(k_res_chunks, dim3((unsigned)nchunks), dim3(LMOD_T), 0, s, (const ResChunk *)chunks_dev, A.q[0], out_dev);
+    return nttm_hip_check(hipGetLastError(), "k_res_chunks launch");
+}
+
 extern "C" int nttm_limbs_mod(uint64_t *res, const uint64_t *x, size_t n, const uint64_t *q, int nq, void *stream)
 {
     if (!res || !q || (n && !x)) return nttm_fail(NTTM_EINVAL, "nttm_limbs_mod arguments");

@@ -22,9 +22,13 @@ _lib = None
 _ref = None
 
 
+SANITIZE = os.environ.get("ORC_SANITIZE") == "1"       # load the AddressSanitizer / UBSan build of the oracle
+LIBNAME = "liborc_san.so" if SANITIZE else "liborc.so"
+
+
 def build(force=False):
     """Compile the oracle (and the reference build when the reference tree exists) with make."""
-    args = ["make", "-s", "-C", HERE]
+    args = ["make", "-s", "-C", HERE] + (["liborc_san.so"] if SANITIZE else [])
     if force:
         args.append("-B")
     subprocess.run(args, check=True)
@@ -37,7 +41,7 @@ def _p(a):
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liborc.so")
+        path = os.path.join(HERE, LIBNAME)
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)

@@ -37,7 +37,7 @@ static Big load(const uint64_t *v, size_t n) { Big r(v, v + n); big_norm(r); ret
 static size_t store(const Big &b, uint64_t *out, size_t cap)
 {
     if (b.size() > cap) return (size_t)-1;
-    memcpy(out, b.data(), b.size() * 8);
+    if (!b.empty()) memcpy(out, b.data(), b.size() * 8);    /* an empty Big has no data pointer to copy from */
     return b.size();
 }
 

@@ -10,7 +10,9 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "native", "arith_host.c")
-SO = os.path.join(HERE, "native", "arith_host.so")
+SAN = os.environ.get("NTTM_NATIVE_SANITIZE") == "1"        # the AddressSanitizer / UBSan build (tests/test_sanitize.py)
+SO = os.path.join(HERE, "native", "arith_host_san.so" if SAN else "arith_host.so")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
 B = 10 ** 18
 
 
@@ -18,7 +20,8 @@ B = 10 ** 18
 def A():
     if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(SRC), os.path.getmtime(
             os.path.join(HERE, "..", "ntt_amd", "csrc", "arith.h"))):
-        subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", SO, SRC, "-lm"], check=True)
+        subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared"] + (SAN_FLAGS if SAN else []) +
+                       ["-o", SO, SRC, "-lm"], check=True)
     L = C.CDLL(SO)
     d = C.c_double
     L.t_mul_shoup.argtypes = [d, d, d, d]; L.t_mul_shoup.restype = d

@@ -13,7 +13,9 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "native", "ebig_host.cpp")
 HDR = os.path.join(HERE, "..", "ntt_amd", "csrc", "ebig.h")
-SO = os.path.join(HERE, "native", "ebig_host.so")
+SAN = os.environ.get("NTTM_NATIVE_SANITIZE") == "1"        # the AddressSanitizer / UBSan build (tests/test_sanitize.py)
+SO = os.path.join(HERE, "native", "ebig_host_san.so" if SAN else "ebig_host.so")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
 B = 10 ** 18
 U64P = C.POINTER(C.c_uint64)
 
@@ -21,7 +23,8 @@ U64P = C.POINTER(C.c_uint64)
 @pytest.fixture(scope="module")
 def E():
     if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(SRC), os.path.getmtime(HDR)):
-        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", SO, SRC], check=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared"] + (SAN_FLAGS if SAN else []) + ["-o", SO, SRC],
+                       check=True)
     L = C.CDLL(SO)
     sz = C.c_size_t
     L.eb_t_recip.argtypes = [U64P, sz, sz, U64P, sz]; L.eb_t_recip.restype = C.c_long
