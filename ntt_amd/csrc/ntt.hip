@@ -405,8 +405,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 /* ---- the streaming strided pass ---------------------------------------------------------------------------------------
- * Strided passes (lo >= GW) of the fixed tile shapes 2^GM rows x 2^GW columns, 8 x 5 or 9 x 4 (H = 1, 2^13 points, 16
- * per thread; the 9-bit rows are 128-byte runs, in three rounds of 3 bits instead of two of 4): one
+ * Strided passes (lo >= GW) of the fixed tile shapes 2^GM rows x 2^GW columns, GM + GW = 13 with GM = 5 .. 9 (H = 1,
+ * 2^13 points, 16 per thread; 9-bit rows are 128-byte runs in three rounds of 3 bits, the others two rounds): one
  * block per CU walks the tiles blockIdx.x, + gridDim.x, ...  The next tile is fetched straight into the other of two
  * LDS buffers (global_load_lds, 16 bytes per lane, the tile in natural row-major order) while the current one is
  * transformed, so HBM streams during the butterflies.  Vector-memory operations of a wave complete in issue order, so
@@ -420,8 +420,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
  *     leaves exactly the fetch of the tile about to be transformed complete.
  * The boundary twiddle is applied as a Barrett product (both factors variable, |x|, |w| <= p), the forward on the
  * store (after the last round, bits [0, 4)), the inverse on the load (before the first round, the same bits).  The
- * 9-bit form has two groups of 8 points per thread in that round and takes its chains from bound_chain (the same
- * base * step^i construction per group, from the LDS copy of the boundary table). */
+ * other shapes have one or two groups per thread in that round and take their chains from bound_chain (the same
+ * base * step^i construction per group, from the LDS copy of the boundary table).
+ * A limb pass reads one operand (a batch of one) or, per tile, the operand of the tile's transform from the batch's
+ * descriptors; descriptors and the operand's last limb are read through the scalar unit, whose loads the hand-counted
+ * vector-memory waits do not see. */
 #define STREAM_T 512
 /* a workgroup barrier for LDS only: the wave's LDS operations complete, then s_barrier.  The compiler places a wait
  * for every outstanding vector-memory operation (an LDS fetch included) before a barrier it sees; written as one asm
@@ -438,8 +441,10 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
     constexpr bool INV = MODE == PM_INV, LIMB = MODE == PM_FWD_LIMB;
     constexpr int PTS = 16, T = 1 << (GM + GW), W = 1 << GW, NBQ = 4 << NTTM_BQ_LOG;
     static_assert(T == PTS * STREAM_T, "2^13-point tiles, 16 points per thread");
-    static_assert(GM == 8 || GM == 9, "two rounds of 4 levels, or three of 3");
-    constexpr int NR = GM == 8 ? 2 : 3, LRL = GM == 8 ? 4 : 3;      /* rounds, and bits per round */
+    static_assert(GM >= 5 && GM <= 9, "rows of 5 to 9 bits");
+    /* rounds of at most 4 bits (round_split's balanced split): NR of them; LR0 bits in the round over bits [0, LR0)
+     * (the boundary twiddles' round), LRT in the top round (the first forward round) */
+    constexpr int NR = (GM + 3) / 4, LR0 = GM / NR + (GM % NR > 0 ? 1 : 0), LRT = GM / NR + (GM % NR > NR - 1 ? 1 : 0);
     __shared__ double buf0[T], buf1[T];
     __shared__ double2 stw[1 << GM], bqs[NBQ];
     const int t = threadIdx.x;
@@ -448,8 +453,12 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
     const int lo = a.lo, lom = a.lo + GM;
     const uint32_t ncb_mask = (uint32_t)((((uint64_t)1 << lo) >> GW) - 1);
     PassPos pp; pp.m = GM; pp.logw = GW; pp.lo = lo; pp.nthr = STREAM_T; pp.t = t;
+    /* descriptors and limbs read through the scalar unit (constant address space, uniform per tile): they leave the
+     * vector-memory counter to the tile fetches and stores alone */
+    typedef const __attribute__((address_space(4))) LoadDesc *CDesc;
+    typedef const __attribute__((address_space(4))) uint64_t *CLimb;
 
-    /* the thread's group in the round over bits [0, 4): column c, bit-reversed row kb of slot 0 */
+    /* the thread's group in the round over bits [0, 4) of the 8-bit shape: column c, bit-reversed row kb of slot 0 */
     const uint32_t c_in = (uint32_t)t & (W - 1);
     const uint32_t r0 = (((uint32_t)t >> GW) << 4) & ((1u << GM) - 1);
     const uint32_t kb = __brev(r0) >> (32 - GM);
@@ -460,23 +469,33 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
         *colbase = cb << GW;
         return (hb << lom) + ((uint64_t)cb << GW);
     };
-    /* a limb-input tile (one operand a.limbs, batch 1): the number of its elements, counted from the tile's first
-     * index, that lie inside the operand, saturated (element offsets within a tile are below 2^lom <= 2^31) */
-    auto limb_lim = [&](uint64_t gbase) -> uint32_t {
-        const uint64_t nl = a.nlimbs;
-        return nl <= gbase ? 0u : nl - gbase >= 0xffffffffULL ? 0xffffffffu : (uint32_t)(nl - gbase);
+    /* a limb-input tile: its operand (one a.limbs of a batch of one, or the descriptor of the tile's transform -- a
+     * strided tile lies inside one transform), the operand's limb at the tile's first element, and how many of the
+     * tile's elements (counted from the first, saturated; offsets within a tile are below 2^lom <= 2^31) lie inside it */
+    struct LimbTile { const uint64_t *base; uint64_t nl, gloc; uint32_t lim; };
+    auto limb_tile = [&](uint64_t gb) -> LimbTile {
+        LimbTile r;
+        if (a.ldesc) {
+            const CDesc d = (CDesc)a.ldesc + (gb >> a.logn);
+            r.base = a.which ? d->b : d->a; r.nl = a.which ? d->nb : d->na;
+            r.gloc = gb & (((uint64_t)1 << a.logn) - 1);
+        } else { r.base = a.limbs; r.nl = a.nlimbs; r.gloc = gb; }
+        r.lim = r.nl <= r.gloc ? 0u : r.nl - r.gloc >= 0xffffffffULL ? 0xffffffffu : (uint32_t)(r.nl - r.gloc);
+        return r;
     };
     auto fetch = [&](uint32_t tile, double *dst) {
         uint32_t cbase;
         const uint64_t gb = tile_base(tile, &cbase);
-        const uint32_t lim = LIMB ? limb_lim(gb) : 0u;
-        const double *src = LIMB ? (const double *)(a.limbs + gb) : a.x + gb;
+        LimbTile lt;
+        if (LIMB) lt = limb_tile(gb);
+        const uint32_t lim = LIMB ? lt.lim : 0u;
+        const double *src = LIMB ? (const double *)(lt.base + lt.gloc) : a.x + gb;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const uint32_t q = (uint32_t)(i * STREAM_T + t);           /* 16-byte chunk: elements 2q, 2q + 1 */
             const uint32_t off = ((q >> (GW - 1)) << lo) + ((q & (W / 2 - 1)) << 1);
             /* a limb chunk is fetched when both limbs lie inside the operand; the one that straddles its end (odd
-             * length) is read from the preloaded last limb */
+             * length) is read from the operand's last limb */
             if (!LIMB || off + 1 < lim)
                 __builtin_amdgcn_global_load_lds((const void *)(src + off),
                                                  (__attribute__((address_space(3))) void *)(dst + (i * STREAM_T + wv * 64) * 2),
@@ -486,7 +505,6 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
 
     if (t < (1 << GM)) stw[t] = a.tw.stage[t];
     for (int i = t; i < NBQ; i += STREAM_T) bqs[i] = a.tw.bq[i];
-    const uint64_t last = LIMB && a.nlimbs ? a.limbs[a.nlimbs - 1] : 0;   /* waited for below, before the loop */
     uint32_t tile = blockIdx.x;
     if (tile < ntiles) fetch(tile, buf0);
     __builtin_amdgcn_s_waitcnt(0);                                     /* vmcnt(0) expcnt(0) lgkmcnt(0) */
@@ -518,42 +536,47 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
             round_split(GM, i, &s_lo, &logR, INV, 4);
             if (i > 0) lds_barrier();
             if (LIMB && i == 0) {                   /* limbs to residues, zero beyond the operand */
-                const uint32_t lim = limb_lim(gbase);
+                const LimbTile lt = limb_tile(gbase);
+                const uint32_t lim = lt.lim;
+                const uint64_t last = lt.nl ? ((CLimb)lt.base)[lt.nl - 1] : 0;
                 const uint64_t *cl = (const uint64_t *)cur;
-                /* slot j of a group of the first round holds the top LRL row bits = j, so rows >= 2^(GM-1) are the
-                 * group's slots j >= 2^(LRL-1); tiles whose rows split into an all-inside and an all-outside half (the
+                /* slot j of a group of the first round holds the top LRT row bits = j, so rows >= 2^(GM-1) are the
+                 * group's slots j >= 2^(LRT-1); tiles whose rows split into an all-inside and an all-outside half (the
                  * zero-padded upper half of a product's operand) convert only the inside slots */
                 const uint32_t half = 1u << (GM - 1 + lo);
                 if (lim <= half && lim > half - (1u << lo) + W) {
-                    for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
-                        v[sl] = (sl & ((1 << LRL) - 1)) < (1 << (LRL - 1)) ? limb_to_res(cl[e], p, pinv) : 0.0; });
+                    for_slots<LRT, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
+                        v[sl] = (sl & ((1 << LRT) - 1)) < (1 << (LRT - 1)) ? limb_to_res(cl[e], p, pinv) : 0.0; });
                 } else {
-                    for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
+                    for_slots<LRT, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t o, uint32_t, uint32_t) {
                         v[sl] = o < lim ? limb_to_res(o + 1 == lim ? last : cl[e], p, pinv) : 0.0; });
                 }
             } else
-                for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { v[sl] = cur[e]; });
+                ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
+                    v[sl] = cur[e]; });)
             if (INV && i == 0) {
                 if (GM == 8) {
 #pragma unroll
                     for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
-                } else bound_chain<LRL, PTS>(v, pp, lom, colbase, bqs, p, pinv);
+                } else bound_chain<LR0, PTS>(v, pp, lom, colbase, bqs, p, pinv);
             }
-            round_compute<LRL, INV, PTS>(v, pp, s_lo, stw, p, pinv);
+            ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, stw, p, pinv);)
             if (i + 1 < NR)
-                for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) { cur[e] = v[sl]; });
+                ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
+                    cur[e] = v[sl]; });)
         }
         if (!INV) {
             if (GM == 8) {
 #pragma unroll
                 for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
-            } else bound_chain<LRL, PTS>(v, pp, lom, colbase, bqs, p, pinv, LIMB && a.do_in_scale, a.in_scale);
+            } else bound_chain<LR0, PTS>(v, pp, lom, colbase, bqs, p, pinv, LIMB && a.do_in_scale, a.in_scale);
         }
         if (INV && a.do_scale) {                                       /* the inverse's final n^-1 */
 #pragma unroll
             for (int j = 0; j < PTS; j++) v[j] = mul_shoup(v[j], a.scale.x, a.scale.y, p);
         }
-        for_slots<LRL, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t, uint32_t) { xt[o] = v[sl]; });
+        ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t, uint32_t) {
+            xt[o] = v[sl]; });)
     }
     __builtin_amdgcn_s_waitcnt(0);                                     /* no LDS fetch in flight past the block's end */
 }
@@ -975,22 +998,27 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P, M, W, H>), grid, blk, lds, s, b); break; }
     /* the strided tile shapes of large transforms, compiled with their geometry fixed */
     const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
-    const bool shape8 = g.m == 8 && g.logw == 5, shape9 = g.m == 9 && g.logw == 4;
-    const bool stream = fixed && g_stream && (shape8 || shape9) && g.lo >= g.logw && mode != PM_INV_PW &&
+    /* the streamed shapes: strided 2^13-point tiles of 5 to 9-bit rows (2^(13 - m) columns), limbs from one operand
+     * (a batch of one) or per transform from the descriptors */
+    const bool shape = g.m >= 5 && g.m <= 9 && g.m + g.logw == 13;
+    const bool stream = fixed && g_stream && shape && g.lo >= g.logw && mode != PM_INV_PW &&
                         (mode == PM_INV || !b.do_scale) &&
-                        (mode != PM_FWD_LIMB || (!b.ldesc && b.batch == 1 && g.lo + g.m < 32));
+                        (mode != PM_FWD_LIMB || ((b.ldesc || b.batch == 1) && g.lo + g.m < 32));
     if (stream) {
         const unsigned nb = (unsigned)(tiles < (uint64_t)g_ncu ? tiles : (uint64_t)g_ncu);
         const uint32_t nt = (uint32_t)tiles;
-        if (shape8) {
-            if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-            else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-            else hipLaunchKernelGGL((k_pass_stream<PM_INV, 8, 5>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-        } else {
-            if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, 9, 4>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-            else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, 9, 4>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
-            else hipLaunchKernelGGL((k_pass_stream<PM_INV, 9, 4>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+#define NTTM_STREAM_LAUNCH(M, W) \
+        if (mode == PM_FWD) hipLaunchKernelGGL((k_pass_stream<PM_FWD, M, W>), dim3(nb), dim3(STREAM_T), 0, s, b, nt); \
+        else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, M, W>), dim3(nb), dim3(STREAM_T), 0, s, b, nt); \
+        else hipLaunchKernelGGL((k_pass_stream<PM_INV, M, W>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
+        switch (g.m) {
+        case 5: NTTM_STREAM_LAUNCH(5, 8) break;
+        case 6: NTTM_STREAM_LAUNCH(6, 7) break;
+        case 7: NTTM_STREAM_LAUNCH(7, 6) break;
+        case 8: NTTM_STREAM_LAUNCH(8, 5) break;
+        default: NTTM_STREAM_LAUNCH(9, 4) break;
         }
+#undef NTTM_STREAM_LAUNCH
     }
     else if (fixed && ppt == PASS_PTS && g.m == 8 && g.logw == 5) { NTTM_PASS_LAUNCH_G(PASS_PTS, 8, 5, 0) }
     else if (T >= PASS_PTS && ppt == 8) { NTTM_PASS_LAUNCH(8) }
