@@ -113,6 +113,8 @@ struct Ctx {
 static Ctx g;
 static uint64_t g_init_count;
 static int g_prof_on;                        /* live kernel timing on (nttm_profile) */
+static int g_fused_bits_env;                 /* NTTM_FUSED_BITS=10|11|12: that last-pass width for 2^26 .. 2^30 points
+                                              * (for comparison measurements) */
 
 /* the workspace of nttm_mul_noalloc: a file-scope device array, one copy per GPU, held until the process ends.  Host
  * code never uses its name as an address (on the host it is a stand-in, not the device address): nttm_init looks up
@@ -357,6 +359,10 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     for (int i = 0; i < 4; i++) HIPCK(hipEventCreate(&g.ev[i]));
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) HIPCK(hipEventCreate(&g.pev[i]));
     RCK(hpin_need(1 << 16));
+    {
+        const char *e = getenv("NTTM_FUSED_BITS");
+        g_fused_bits_env = e ? atoi(e) : 0;
+    }
     {   /* NTTM_STAGE_CHUNK=<bytes>: staged bytes per sub-batch (default 1 GiB; smaller values exercise the split) */
         const char *e = getenv("NTTM_STAGE_CHUNK");
         g.stage_chunk = e && atoll(e) > 0 ? (size_t)atoll(e) : NTTM_STAGE_CHUNK;
@@ -470,8 +476,9 @@ static int product_tile_log(int logn, int tl = 0)
  * at 10, so 9+9+10 loses to 8+8+12 there, and 2^29 / 2^30 points keep 9+8+12 / 9+9+12. */
 static int fused_bits(int logn, int tile_log)
 {
-    if (tile_log != 13 || logn < 26 || logn > 28) return NTTM_FUSED_LOG;
-    return logn - 16;
+    if (tile_log != 13 || logn < 26 || logn > 30) return NTTM_FUSED_LOG;
+    if (g_fused_bits_env >= 10 && g_fused_bits_env <= 12 && logn - g_fused_bits_env <= 18) return g_fused_bits_env;
+    return logn <= 28 ? logn - 16 : NTTM_FUSED_LOG;
 }
 static void product_plan(Plan *pl, int logn, uint64_t batch, int tile_log)
 {
