@@ -441,7 +441,7 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
     constexpr bool INV = MODE == PM_INV, LIMB = MODE == PM_FWD_LIMB;
     constexpr int PTS = 16, T = 1 << (GM + GW), W = 1 << GW, NBQ = 4 << NTTM_BQ_LOG;
     static_assert(T == PTS * STREAM_T, "2^13-point tiles, 16 points per thread");
-    static_assert(GM >= 5 && GM <= 9, "rows of 5 to 9 bits");
+    static_assert(GM >= 1 && GM <= 9, "rows of 1 to 9 bits");
     /* rounds of at most 4 bits (round_split's balanced split): NR of them; LR0 bits in the round over bits [0, LR0)
      * (the boundary twiddles' round), LRT in the top round (the first forward round) */
     constexpr int NR = (GM + 3) / 4, LR0 = GM / NR + (GM % NR > 0 ? 1 : 0), LRT = GM / NR + (GM % NR > NR - 1 ? 1 : 0);
@@ -998,9 +998,9 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     default: hipLaunchKernelGGL((k_pass<PM_INV_PW, P, M, W, H>), grid, blk, lds, s, b); break; }
     /* the strided tile shapes of large transforms, compiled with their geometry fixed */
     const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
-    /* the streamed shapes: strided 2^13-point tiles of 5 to 9-bit rows (2^(13 - m) columns), limbs from one operand
+    /* the streamed shapes: strided 2^13-point tiles of 1 to 9-bit rows (2^(13 - m) columns), limbs from one operand
      * (a batch of one) or per transform from the descriptors */
-    const bool shape = g.m >= 5 && g.m <= 9 && g.m + g.logw == 13;
+    const bool shape = g.m >= 1 && g.m <= 9 && g.m + g.logw == 13;
     const bool stream = fixed && g_stream && shape && g.lo >= g.logw && mode != PM_INV_PW &&
                         (mode == PM_INV || !b.do_scale) &&
                         (mode != PM_FWD_LIMB || ((b.ldesc || b.batch == 1) && g.lo + g.m < 32));
@@ -1012,6 +1012,10 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
         else if (mode == PM_FWD_LIMB) hipLaunchKernelGGL((k_pass_stream<PM_FWD_LIMB, M, W>), dim3(nb), dim3(STREAM_T), 0, s, b, nt); \
         else hipLaunchKernelGGL((k_pass_stream<PM_INV, M, W>), dim3(nb), dim3(STREAM_T), 0, s, b, nt);
         switch (g.m) {
+        case 1: NTTM_STREAM_LAUNCH(1, 12) break;
+        case 2: NTTM_STREAM_LAUNCH(2, 11) break;
+        case 3: NTTM_STREAM_LAUNCH(3, 10) break;
+        case 4: NTTM_STREAM_LAUNCH(4, 9) break;
         case 5: NTTM_STREAM_LAUNCH(5, 8) break;
         case 6: NTTM_STREAM_LAUNCH(6, 7) break;
         case 7: NTTM_STREAM_LAUNCH(7, 6) break;
