@@ -99,6 +99,8 @@ struct Ctx {
     char *ws; size_t ws_bytes;               /* device workspace */
     uint64_t *tmp; size_t tmp_limbs;         /* a piece product of a grid (above one transform) */
     uint64_t *bsp[2]; size_t bsp_limbs[2];   /* the inner levels of a binary-splitting tree (even / odd levels) */
+    char *tdesc; size_t tdesc_bytes;         /* the planned tree's descriptors, every group of every level */
+    char *tpin; size_t tpin_bytes;           /* their pinned host image, and the groups' flags */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     char *stage; size_t stage_bytes;         /* device copies of host-resident operands and results */
     size_t stage_chunk;                      /* staged bytes per sub-batch of a batch */
@@ -113,6 +115,7 @@ struct Ctx {
 static Ctx g;
 static uint64_t g_init_count;
 static int g_prof_on;                        /* live kernel timing on (nttm_profile) */
+static int g_tree_levelwise;                 /* NTTM_TREE_LEVELWISE=1: the tree level by level (for comparison) */
 static int g_fused_bits_env;                 /* NTTM_FUSED_BITS=10|11|12: that last-pass width for 2^26 .. 2^30 points
                                               * (for comparison measurements) */
 
@@ -360,6 +363,10 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) HIPCK(hipEventCreate(&g.pev[i]));
     RCK(hpin_need(1 << 16));
     {
+        const char *e = getenv("NTTM_TREE_LEVELWISE");
+        g_tree_levelwise = e && e[0] == '1';
+    }
+    {
         const char *e = getenv("NTTM_FUSED_BITS");
         g_fused_bits_env = e ? atoi(e) : 0;
     }
@@ -381,6 +388,8 @@ extern "C" void nttm_shutdown(void)
     if (g.ws) (void)hipFree(g.ws);
     if (g.tmp) (void)hipFree(g.tmp);
     for (int k = 0; k < 2; k++) if (g.bsp[k]) (void)hipFree(g.bsp[k]);
+    if (g.tdesc) (void)hipFree(g.tdesc);
+    if (g.tpin) (void)hipHostFree(g.tpin);
     if (g.tables) (void)hipFree(g.tables);
     if (g.hpin) (void)hipHostFree(g.hpin);
     if (g.stage) (void)hipFree(g.stage);
@@ -914,7 +923,7 @@ static void group_image(GroupBuild &gb, double *A, char *img)
 }
 /* the group's launches: transforms, CRT with the carry scan, normalized lengths; under the group's tile and points
  * per thread (the library's settings are restored afterwards) */
-static int group_launch(const GroupBuild &gb, int np, double *A, char *dbase, hipStream_t s)
+static int group_launch(const GroupBuild &gb, int np, double *A, char *dbase, hipStream_t s, bool norm = true)
 {
     const int tile0 = g.cfg.tile_log, pts0 = nttm_get_pts();
     g.cfg.tile_log = gb.tile; nttm_set_pts(gb.pts);
@@ -924,7 +933,7 @@ static int group_launch(const GroupBuild &gb, int np, double *A, char *dbase, hi
                                 gb.share ? (const uint32_t *)(dbase + gb.off_bmap) : nullptr);
     if (!rc) rc = nttm_crt_launch_batch(dbase + gb.off_cd, gb.cnt, gb.nblk, np, g.cc, (uint32_t *)(dbase + gb.off_crt), s,
                                         gb.crt_bytes, gb.crt_per);
-    if (!rc) rc = nttm_run_norm(dbase + gb.off_cd, gb.cnt, (uint64_t *)(dbase + gb.off_ncn), s);
+    if (!rc && norm) rc = nttm_run_norm(dbase + gb.off_cd, gb.cnt, (uint64_t *)(dbase + gb.off_ncn), s);
     g.cfg.tile_log = tile0; nttm_set_pts(pts0);
     return rc;
 }
@@ -1405,6 +1414,182 @@ static int bsp_need(int k, size_t limbs)
     g.bsp_limbs[k] = limbs;
     return 0;
 }
+/* ---- the binary-splitting tree planned on the host ---------------------------------------------------------------------
+ * Every level's products take their operands' unnormalized lengths: a parent's P has |P_L| + |Q_R| + 1 limbs and its Q
+ * |Q_L| + |Q_R| (top limbs possibly 0, which changes no value).  So every level's shape, grouping and pool layout is
+ * known before any product runs, and the levels follow each other on the stream with no synchronization: the host
+ * builds level k's descriptors while the device runs level k - 1 (a level's large groups in chunks of TREE_CHUNK
+ * products), and copies them from a pinned image that no later group reuses.  One synchronization at the end reads every group's CRT flags and the root's top limbs (normalized
+ * there).  The reference's level loop reads each level's lengths back before the next (E/binsplit.c:1583-1651).
+ * Returns 1, having launched nothing, when a pair's |P_R| exceeds |P_L| + |Q_R| on the planned lengths or the residue
+ * post-condition is configured (the level-by-level path handles both). */
+int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
+static int tdesc_need(size_t bytes)
+{
+    if (bytes <= g.tdesc_bytes) return 0;
+    if (g.tdesc) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.tdesc)); g.tdesc = nullptr; g.tdesc_bytes = 0; }
+    if (hipMalloc(&g.tdesc, bytes) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "tree descriptors: %zu bytes", bytes); }
+    g.tdesc_bytes = bytes;
+    return 0;
+}
+static int tpin_need(size_t bytes)
+{
+    if (bytes <= g.tpin_bytes) return 0;
+    if (g.tpin) { HIPCK(hipDeviceSynchronize()); HIPCK(hipHostFree(g.tpin)); g.tpin = nullptr; g.tpin_bytes = 0; }
+    HIPCK(hipHostMalloc((void **)&g.tpin, bytes, hipHostMallocDefault));
+    g.tpin_bytes = bytes;
+    return 0;
+}
+/* products per launched group: a level's large length group is launched in chunks, so that the device starts on the
+ * first chunk while the host builds the next (the lowest levels have 10^5 products) */
+#define TREE_CHUNK 16384
+static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q,
+                           size_t q_cap, size_t *qn, hipStream_t s)
+{
+    if (g.cfg.check || n < 2) return 1;
+    struct Len { size_t pn, qn; };
+    std::vector<std::vector<Len>> len(1, std::vector<Len>(n));
+    for (size_t i = 0; i < n; i++) len[0][i] = {leaves[i].pn, leaves[i].qn};
+    while (len.back().size() > 1) {
+        const std::vector<Len> &c = len.back();
+        const size_t npairs = c.size() / 2;
+        std::vector<Len> nx(npairs + (c.size() & 1));
+        for (size_t i = 0; i < npairs; i++) {
+            const Len &l = c[2 * i], &r = c[2 * i + 1];
+            if (r.pn > l.pn + r.qn) return 1;
+            nx[i] = {l.pn + r.qn + 1, l.qn + r.qn};
+        }
+        if (c.size() & 1) nx.back() = c.back();
+        len.push_back(std::move(nx));
+    }
+    const int nlev = (int)len.size() - 1;                        /* levels of products */
+    /* resources for every level, sized before the first launch: pools by level parity, planes, descriptors */
+    size_t pool_max[2] = {1, 1}, planes_max = 0, desc_total = 0, img_total = 0, ngroups = 0;
+    std::vector<int> lev_np(nlev + 1, 3);
+    auto gkey = [](size_t na, size_t nb) { int logn, r3; choose_len(na + nb - 1, &logn, &r3); return r3 * (NTTM_LOGN_MAX + 1) + logn; };
+    for (int k = 1; k <= nlev; k++) {
+        size_t tot = 0;
+        for (const Len &x : len[k]) tot += x.pn + x.qn;
+        pool_max[k & 1] = std::max(pool_max[k & 1], tot);
+        const std::vector<Len> &c = len[k - 1];
+        size_t mn_max = 1;
+        std::map<int, std::pair<uint64_t, uint64_t>> grp;           /* key -> (products, CRT blocks) */
+        for (size_t i = 0; i + 1 < c.size(); i += 2) {
+            const Len &l = c[i], &r = c[i + 1];
+            const size_t pr[2][2] = {{l.pn, r.qn}, {l.qn, r.qn}};
+            for (int t = 0; t < 2; t++) {
+                const size_t na = pr[t][0], nb = pr[t][1];
+                if (!na || !nb) continue;
+                mn_max = std::max(mn_max, std::min(na, nb));
+                auto &e = grp[gkey(na, nb)];
+                const int key = gkey(na, nb), logn = key % (NTTM_LOGN_MAX + 1), r3 = key > NTTM_LOGN_MAX;
+                const uint64_t npts = (uint64_t)(r3 ? 3 : 1) << logn;
+                e.first++;
+                e.second += nttm_crt_blocks(na + nb + (t == 0 ? 1 : 0), npts <= 1024 ? 1 : 8);
+            }
+        }
+        RCK(choose_np(mn_max, &lev_np[k]));
+        for (auto &kv : grp) {
+            const int logn = kv.first % (NTTM_LOGN_MAX + 1), r3 = kv.first > NTTM_LOGN_MAX;
+            const uint64_t npts = (uint64_t)(r3 ? 3 : 1) << logn, cnt_all = kv.second.first;
+            if (logn > NTTM_LOGN_MAX) return 1;
+            for (uint64_t c0 = 0; c0 < cnt_all; c0 += TREE_CHUNK) {   /* the group's chunks (below) */
+                const uint64_t cnt = std::min<uint64_t>(TREE_CHUNK, cnt_all - c0), tb = r3 ? 3 * cnt : cnt;
+                planes_max = std::max(planes_max, (size_t)(lev_np[k] + 1) * cnt * npts * 8);
+                const size_t img = align_up(cnt * sizeof(CrtDesc), 256) + 2 * align_up(cnt * sizeof(LoadDesc), 256) +
+                                   align_up(tb * 4, 256) + 256;
+                img_total += img;
+                desc_total += img + align_up(cnt * 8, 256) + align_up(nttm_crt_scratch_words(kv.second.second) * 4, 256) + 256;
+                ngroups++;
+            }
+        }
+    }
+    RCK(bsp_need(0, pool_max[0])); RCK(bsp_need(1, pool_max[1]));
+    RCK(ws_need(planes_max ? planes_max : 256));
+    RCK(tdesc_need(desc_total + 512));
+    RCK(tpin_need(img_total + 8 * ngroups + 2048));
+    /* the levels */
+    std::vector<nttm_bs_node> cur(leaves, leaves + n), nxt;
+    std::vector<nttm_prod> prods;
+    std::vector<const char *> flag_at;                          /* each group's CRT flag word in device memory */
+    size_t dpos = 0, ipos = 0;
+    for (int k = 1; k <= nlev; k++) {
+        const size_t npairs = cur.size() / 2, odd = cur.size() & 1;
+        uint64_t *pool = g.bsp[k & 1];
+        nxt.assign(npairs + odd, nttm_bs_node());
+        prods.assign(2 * npairs, nttm_prod());
+        size_t off = 0;
+        for (size_t i = 0; i < npairs; i++) {                    /* P = P_L Q_R + P_R, Q = Q_L Q_R: Q_R shared */
+            const nttm_bs_node &L = cur[2 * i], &R = cur[2 * i + 1];
+            uint64_t *pc = pool + off; off += L.pn + R.qn + 1;
+            uint64_t *qc = pool + off; off += L.qn + R.qn;
+            nttm_prod &a = prods[2 * i], &b = prods[2 * i + 1];
+            a.a = L.p; a.na = L.pn; a.b = R.q; a.nb = R.qn; a.c = pc; a.x = R.pn ? R.p : pc; a.nx = R.pn;
+            b.a = L.q; b.na = L.qn; b.b = R.q; b.nb = R.qn; b.c = qc; b.x = nullptr; b.nx = 0;
+            nxt[i].p = pc; nxt[i].pn = L.pn + R.qn + 1; nxt[i].q = qc; nxt[i].qn = L.qn + R.qn;
+        }
+        if (odd) {                                               /* the last node moves up (into this level's pool) */
+            const nttm_bs_node &L = cur.back();
+            uint64_t *pc = pool + off; off += L.pn;
+            uint64_t *qc = pool + off; off += L.qn;
+            if (L.pn) HIPCK(hipMemcpyAsync(pc, L.p, L.pn * 8, hipMemcpyDeviceToDevice, s));
+            if (L.qn) HIPCK(hipMemcpyAsync(qc, L.q, L.qn * 8, hipMemcpyDeviceToDevice, s));
+            nxt[npairs] = L; nxt[npairs].p = pc; nxt[npairs].q = qc;
+        }
+        /* zero-operand products: c = x, zero-extended */
+        std::vector<std::vector<size_t>> groups(2 * (NTTM_LOGN_MAX + 1));
+        for (size_t i = 0; i < prods.size(); i++) {
+            const nttm_prod &pq = prods[i];
+            const size_t nc = pq.na + pq.nb + (pq.x ? 1 : 0);
+            if (pq.na && pq.nb) { groups[gkey(pq.na, pq.nb)].push_back(i); continue; }
+            if (nc) HIPCK(hipMemsetAsync(pq.c, 0, nc * 8, s));
+            if (pq.x && pq.nx) HIPCK(hipMemcpyAsync(pq.c, pq.x, pq.nx * 8, hipMemcpyDeviceToDevice, s));
+        }
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            const std::vector<size_t> &Gall = groups[gi];
+            for (size_t c0 = 0; c0 < Gall.size(); c0 += TREE_CHUNK) {
+                const std::vector<size_t> G(Gall.begin() + c0, Gall.begin() + std::min(Gall.size(), c0 + TREE_CHUNK));
+                const int r3 = gi > NTTM_LOGN_MAX, logn = (int)(gi % (NTTM_LOGN_MAX + 1));
+                GroupBuild gb;
+                group_build(gb, prods.data(), G, lev_np[k], r3, logn, g.cfg.tile_log, batch_pts(logn));
+                if (dpos + gb.desc_bytes > g.tdesc_bytes || ipos + gb.off_ncn > img_total || gb.planes_bytes > g.ws_bytes)
+                    return nttm_fail(NTTM_EINVAL, "internal: tree plan exceeds its sizing (level %d)", k);
+                char *img = g.tpin + ipos, *dbase = g.tdesc + dpos;
+                group_image(gb, (double *)g.ws, img);
+                HIPCK(hipMemcpyAsync(dbase, img, gb.off_ncn, hipMemcpyHostToDevice, s));
+                RCK(group_launch(gb, lev_np[k], (double *)g.ws, dbase, s, false));
+                flag_at.push_back(dbase + gb.off_crt + 4);
+                ipos = align_up(ipos + gb.off_ncn, 256);
+                dpos = align_up(dpos + gb.desc_bytes, 256);
+            }
+        }
+        cur.swap(nxt);
+    }
+    /* every group's flags and the root's normalized lengths (its leading zero limbs found on the device), then one
+     * synchronization */
+    const nttm_bs_node root = cur[0];
+    char *hf = g.tpin + align_up(ipos, 256);
+    uint64_t *hlen = (uint64_t *)(hf + align_up(flag_at.size() * 4, 64));
+    if ((size_t)((char *)(hlen + 2) - g.tpin) > g.tpin_bytes || dpos + 16 > g.tdesc_bytes)
+        return nttm_fail(NTTM_EINVAL, "internal: tree flags exceed their staging");
+    uint64_t *dlen = (uint64_t *)(g.tdesc + dpos);
+    RCK(nttm_run_top_nonzero(root.p, root.pn, dlen, s));
+    RCK(nttm_run_top_nonzero(root.q, root.qn, dlen + 1, s));
+    HIPCK(hipMemcpyAsync(hlen, dlen, 16, hipMemcpyDeviceToHost, s));
+    for (size_t j = 0; j < flag_at.size(); j++) HIPCK(hipMemcpyAsync(hf + 4 * j, flag_at[j], 4, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (size_t j = 0; j < flag_at.size(); j++)
+        if (((const uint32_t *)hf)[j] & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
+    const size_t rp = hlen[0], rq = hlen[1];
+    if (rp > p_cap || rq > q_cap)
+        return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: the root needs %zu and %zu limbs (capacities %zu, %zu)", rp, rq,
+                         p_cap, q_cap);
+    if (rp) HIPCK(hipMemcpyAsync(p, root.p, rp * 8, hipMemcpyDeviceToDevice, s));
+    if (rq) HIPCK(hipMemcpyAsync(q, root.q, rq * 8, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
+    *pn = rp; *qn = rq;
+    return 0;
+}
 extern "C" int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q,
                             size_t q_cap, size_t *qn, void *stream)
 {
@@ -1419,6 +1604,10 @@ extern "C" int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, s
     }
     if (mm.kind(p, p_cap * 8) != MEM_DEV || mm.kind(q, q_cap * 8) != MEM_DEV)
         return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: p or q is not device memory of this device");
+    if (!g_tree_levelwise) {
+        const int rc = bs_tree_planned(leaves, n, p, p_cap, pn, q, q_cap, qn, s);
+        if (rc <= 0) return rc;
+    }
     std::vector<nttm_bs_node> cur(leaves, leaves + n), nxt;
     std::vector<nttm_prod> prods;
     for (int level = 0; cur.size() > 1; level++) {

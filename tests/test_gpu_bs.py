@@ -66,3 +66,37 @@ def test_bs_tree_random_nodes(nttm):
     P, Q = nttm.bs_tree(leaves)
     p_want, q_want = combine(ints)
     assert np.array_equal(host(Q), from_int(q_want)) and np.array_equal(host(P), from_int(p_want))
+
+
+def test_bs_tree_planned_equals_levelwise(nttm):
+    """the tree planned on the host from unnormalized lengths (no synchronization between levels) gives the same root,
+    limb for limb, as the level-by-level tree that normalizes every level's lengths (NTTM_TREE_LEVELWISE=1); leaves of
+    up to 600 limbs so that the upper levels run two-pass transforms and shared-B groups"""
+    import os
+    import torch
+    rng = np.random.default_rng(23)
+    leaves, s = [], 5
+    for i in range(301):
+        lq = int(rng.integers(1, 600))
+        lp = int(rng.integers(0, lq))
+        p, s = O.gen_limbs(lp, s) if lp else (np.zeros(0, dtype=np.uint64), s)
+        q, s = O.gen_limbs(lq, s)
+        leaves.append((p, q))
+    say("bs tree planned")
+    P1, Q1 = nttm.bs_tree([(dev(p), dev(q)) for p, q in leaves])
+    p1, q1 = host(P1), host(Q1)
+    try:
+        torch.cuda.synchronize(); nttm.shutdown()
+        os.environ["NTTM_TREE_LEVELWISE"] = "1"
+        nttm.init(0)
+        say("bs tree levelwise")
+        P2, Q2 = nttm.bs_tree([(dev(p), dev(q)) for p, q in leaves])
+        p2, q2 = host(P2), host(Q2)
+    finally:
+        os.environ.pop("NTTM_TREE_LEVELWISE", None)
+        torch.cuda.synchronize(); nttm.shutdown(); nttm.init(0)
+    assert np.array_equal(p1, p2) and np.array_equal(q1, q2)
+    assert p1[-1] != 0 and q1[-1] != 0
+    if O.have_gmp() and len(q1) < 200000:
+        pw, qw = combine([(to_int(p), to_int(q)) for p, q in leaves])
+        assert np.array_equal(q1, from_int(qw)) and np.array_equal(p1, from_int(pw))
