@@ -232,6 +232,23 @@ __device__ __forceinline__ double input_limb(const PassArgs &a, uint64_t gi, dou
     if (a.do_in_scale) x = mul_shoup(x, a.in_scale.x, a.in_scale.y, p);
     return x;
 }
+/* input_limb with the operand of the last transform looked up kept: consecutive elements of one transform (the slots
+ * of a group) read its descriptor once */
+struct LimbCache {
+    uint64_t b = ~(uint64_t)0, nl = 0;
+    const uint64_t *src = nullptr;
+    __device__ __forceinline__ double at(const PassArgs &a, uint64_t gi, double p, double pinv)
+    {
+        const uint64_t k = gi & (((uint64_t)1 << a.logn) - 1);
+        if (a.ldesc) {
+            const uint64_t bb = gi >> a.logn;
+            if (bb != b) { const LoadDesc d = a.ldesc[bb]; src = a.which ? d.b : d.a; nl = a.which ? d.nb : d.na; b = bb; }
+        } else { src = a.limbs; nl = a.nlimbs; }
+        double x = k < nl ? limb_to_res(src[k], p, pinv) : 0.0;
+        if (a.do_in_scale) x = mul_shoup(x, a.in_scale.x, a.in_scale.y, p);
+        return x;
+    }
+};
 __device__ __forceinline__ uint64_t y_index(const PassArgs &a, uint64_t gi)
 {
     const uint64_t nmask = ((uint64_t)1 << a.logn) - 1;
@@ -689,16 +706,17 @@ __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &p
     const int nr = round_count(q.m, MAXR);
     int s_lo, logR;
     round_split(q.m, 0, &s_lo, &logR, false, MAXR);
+    LimbCache lc;
     if (s_lo >= 4) {
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t, uint32_t o, uint32_t,
                                                                        uint32_t) {
-            v[sl] = limb ? input_limb(q, gbase + o, p, pinv) : xt[o]; });)
+            v[sl] = limb ? lc.at(q, gbase + o, p, pinv) : xt[o]; });)
     } else {
         if (active) {
 #pragma unroll
             for (int sl = 0; sl < PTS; sl++) {
                 const uint32_t e = sl * pp.nthr + pp.t;
-                xs[e] = limb ? input_limb(q, gbase + e, p, pinv) : xt[e];
+                xs[e] = limb ? lc.at(q, gbase + e, p, pinv) : xt[e];
             }
         }
         __syncthreads();
@@ -1084,11 +1102,22 @@ int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hi
     const int ppt = (T >= 64 * g_pts && T <= 512 * g_pts) ? g_pts : PASS_PTS;
     const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
     if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
-    /* the fixed shapes (4096-point tiles of the last pass of a transform of more than one pass: 2^12-point blocks,
-     * or 4 x 2^10 and 2 x 2^11 where the strided passes took more bits) read no limbs */
-    const bool fixed = g_fixed_shapes && g.m + g.logh == FUSED_LOG && g.m >= 10 && !alimb && !blimb;
+    /* the fixed shapes: 4096-point tiles of 2^12-point blocks, or of 4 x 2^10 and 2 x 2^11 (the last pass of a
+     * transform of more than one pass, or a whole transform of 2^10 to 2^12 points, which reads the limbs) */
+    const bool fixed_shape = g_fixed_shapes && g.m + g.logh == FUSED_LOG && g.m >= 10;
+    const bool fixed = fixed_shape && !alimb && !blimb;
     const dim3 grid((unsigned)tiles), blk(nthr);
-    if (fixed && ppt == 8) {
+    if (fixed_shape && !fixed) {                                /* a single-pass transform: both operands from limbs */
+        if (ppt == 8) {
+            if (g.m == 12) hipLaunchKernelGGL((k_fused2<8, 12, 0, true>), grid, blk, 0, s, A, Bq);
+            else if (g.m == 11) hipLaunchKernelGGL((k_fused2<8, 11, 1, true>), grid, blk, 0, s, A, Bq);
+            else hipLaunchKernelGGL((k_fused2<8, 10, 2, true>), grid, blk, 0, s, A, Bq);
+        } else {
+            if (g.m == 12) hipLaunchKernelGGL((k_fused2<PASS_PTS, 12, 0, true>), grid, blk, 0, s, A, Bq);
+            else if (g.m == 11) hipLaunchKernelGGL((k_fused2<PASS_PTS, 11, 1, true>), grid, blk, 0, s, A, Bq);
+            else hipLaunchKernelGGL((k_fused2<PASS_PTS, 10, 2, true>), grid, blk, 0, s, A, Bq);
+        }
+    } else if (fixed && ppt == 8) {
         if (g.m == 12) hipLaunchKernelGGL((k_fused2<8, 12, 0, false>), grid, blk, 0, s, A, Bq);
         else if (g.m == 11) hipLaunchKernelGGL((k_fused2<8, 11, 1, false>), grid, blk, 0, s, A, Bq);
         else hipLaunchKernelGGL((k_fused2<8, 10, 2, false>), grid, blk, 0, s, A, Bq);
