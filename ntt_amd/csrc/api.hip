@@ -15,6 +15,7 @@
 int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s);
 int nttm_run_fused(const PassArgs &a, const PassGeom &g, hipStream_t s);
 int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hipStream_t s);
+bool nttm_fwd_pass_streams(const PassArgs &a, const PassGeom &g);
 void nttm_set_pts(int pts);
 int nttm_get_pts(void);
 void nttm_set_fixed_shapes(int on);
@@ -115,9 +116,10 @@ struct Ctx {
 static Ctx g;
 static uint64_t g_init_count;
 static int g_prof_on;                        /* live kernel timing on (nttm_profile) */
+static int g_no_defer;                       /* NTTM_NO_DEFER=1: every pass applies its own boundary twiddles */
 static int g_tree_levelwise;                 /* NTTM_TREE_LEVELWISE=1: the tree level by level (for comparison) */
-static int g_fused_bits_env;                 /* NTTM_FUSED_BITS=10|11|12: that last-pass width for 2^26 .. 2^30 points
-                                              * (for comparison measurements) */
+static int g_fused_bits_env;                 /* NTTM_FUSED_BITS=10|11|12: that last-pass width for every product
+                                              * transform above it (for comparison measurements) */
 
 /* the workspace of nttm_mul_noalloc: a file-scope device array, one copy per GPU, held until the process ends.  Host
  * code never uses its name as an address (on the host it is a stand-in, not the device address): nttm_init looks up
@@ -363,6 +365,10 @@ extern "C" int nttm_init(int device, const nttm_config *cfg)
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) HIPCK(hipEventCreate(&g.pev[i]));
     RCK(hpin_need(1 << 16));
     {
+        const char *e = getenv("NTTM_NO_DEFER");
+        g_no_defer = e && e[0] == '1';
+    }
+    {
         const char *e = getenv("NTTM_TREE_LEVELWISE");
         g_tree_levelwise = e && e[0] == '1';
     }
@@ -485,8 +491,10 @@ static int product_tile_log(int logn, int tl = 0)
  * at 10, so 9+9+10 loses to 8+8+12 there, and 2^29 / 2^30 points keep 9+8+12 / 9+9+12. */
 static int fused_bits(int logn, int tile_log)
 {
+    if (tile_log == 13 && g_fused_bits_env >= 10 && g_fused_bits_env <= 12 && logn > g_fused_bits_env &&
+        logn - g_fused_bits_env <= 18)
+        return g_fused_bits_env;
     if (tile_log != 13 || logn < 26 || logn > 30) return NTTM_FUSED_LOG;
-    if (g_fused_bits_env >= 10 && g_fused_bits_env <= 12 && logn - g_fused_bits_env <= 18) return g_fused_bits_env;
     return logn <= 28 ? logn - 16 : NTTM_FUSED_LOG;
 }
 static void product_plan(Plan *pl, int logn, uint64_t batch, int tile_log)
@@ -501,6 +509,25 @@ static void product_plan(Plan *pl, int logn, uint64_t batch, int tile_log)
  * Shared B operands (bmap non-null): B's plane holds the cnt_b distinct operands of ldesc_b; A's engine transform t
  * pairs with B's engine transform bmap[t] in the double-fused pass.
  * pev (or null): 2 np events recorded around B's transform of each prime. */
+/* the first forward pass (from limbs) leaves its boundary twiddles to the second when both are streamed strided
+ * passes (a transform of three or more passes): the limb pass has no VALU to spare, the second pass, bound by HBM, has.
+ * q: the first pass's arguments (limbs set, in_scale as the caller wants it applied) */
+static bool defer_ok(const PassArgs &q, const Plan &pl, int last)
+{
+    if (last < 2 || g_no_defer || !(q.limbs || q.ldesc)) return false;
+    PassArgs nx = q; nx.limbs = nullptr; nx.ldesc = nullptr; nx.do_in_scale = 0;
+    return nttm_fwd_pass_streams(q, pl.p[0]) && nttm_fwd_pass_streams(nx, pl.p[1]) && pl.p[1].m >= 4;
+}
+/* the deferral's flags for pass j of the plan (q carries pass 0's in_scale until pass 0 has run) */
+static void defer_set(PassArgs &q, const Plan &pl, int j, bool on)
+{
+    q.dfr_out = on && j == 0;
+    if (on && j == 1) {
+        q.dfr_in = 1; q.dfr_lom = pl.p[0].lo + pl.p[0].m; q.dfr_gm = pl.p[0].m;
+    } else q.dfr_in = 0;
+    if (on && j == 0) { q.dfr_do_scale = q.do_in_scale; q.dfr_scale = q.in_scale; }
+    if (j > 1) q.dfr_do_scale = 0;
+}
 static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A, double *Bp, const uint64_t *a,
                               uint64_t na, const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s,
                               hipEvent_t *pev, uint64_t cnt_b = 0, const LoadDesc *ldesc_b = nullptr,
@@ -523,7 +550,9 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
         PassArgs fb = base_args(i, Bp, logn, tbb, false);
         if (r3) RCK(nttm_run_r3(false, Bp, cnt_b, logn, g.pk[i], g.twf[i].b3, ldesc_b ? nullptr : b, nb, ldesc_b, 1, &ninv, s));
         else { fb.limbs = ldesc_b ? nullptr : b; fb.nlimbs = nb; fb.ldesc = ldesc_b; fb.which = 1; fb.in_scale = ninv; fb.do_in_scale = 1; }
+        const bool dfb = !r3 && defer_ok(fb, plb, last);
         for (int j = 0; j < last; j++) {
+            defer_set(fb, plb, j, dfb);
             RCK(nttm_run_pass(false, fb, plb.p[j], s));
             fb.limbs = nullptr; fb.ldesc = nullptr; fb.do_in_scale = 0;
         }
@@ -532,10 +561,13 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
         PassArgs fa = base_args(i, Ai, logn, tb, false);
         if (r3) RCK(nttm_run_r3(false, Ai, cnt, logn, g.pk[i], g.twf[i].b3, ldesc ? nullptr : a, na, ldesc, 0, nullptr, s));
         else { fa.limbs = ldesc ? nullptr : a; fa.nlimbs = na; fa.ldesc = ldesc; fa.which = 0; }
+        const bool dfa = !r3 && defer_ok(fa, pl, last);
         for (int j = 0; j < last; j++) {
+            defer_set(fa, pl, j, dfa);
             RCK(nttm_run_pass(false, fa, pl.p[j], s));
             fa.limbs = nullptr; fa.ldesc = nullptr;
         }
+        fa.dfr_in = 0;
         /* B's last forward pass, A's last forward pass, the pointwise product and A's first inverse pass */
         PassArgs fbl = base_args(i, Bp, logn, tb, false);
         fbl.bmap = bmap;

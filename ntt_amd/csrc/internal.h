@@ -51,6 +51,12 @@ struct PassArgs {
     /* B of the double-fused pass shared between products: transform t reads plane transform bmap[t] (null: t); only
      * for tiles inside one transform (m + logh <= logn) */
     const uint32_t *bmap;
+    /* Deferred boundary twiddles between a transform's first two strided forward passes (both streamed):
+     * dfr_out = 1 on the first pass stores its outputs without its boundary twiddles (and without in_scale);
+     * dfr_in = 1 on the second multiplies its loads by them: element (h, c1) of the first pass (h its stored row,
+     * c1 < 2^(dfr_lom - dfr_gm) its column) by w_(2^dfr_lom)^(brev_dfr_gm(h) c1), times dfr_scale when dfr_do_scale */
+    int dfr_out, dfr_in, dfr_lom, dfr_gm, dfr_do_scale;
+    double2 dfr_scale;
 };
 
 struct Ctx;

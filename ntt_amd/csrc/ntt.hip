@@ -537,7 +537,8 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
         double *xt = a.x + gbase;
 
         double v[PTS], w[PTS];
-        if (GM == 8) {   /* boundary twiddles of the bits-[0, 4) round: w[i] = base * step^i (a limb pass's n^-1 folded into base) */
+        const bool bound_out = !INV && !(LIMB && a.dfr_out);           /* the output's boundary twiddles here */
+        if (GM == 8 && (INV || bound_out)) {   /* boundary twiddles of the bits-[0, 4) round: w[i] = base * step^i (a limb pass's n^-1 folded into base) */
             const uint32_t col = colbase + c_in, sh = 32 - lom;
             double base = bq_twiddle(bqs, (col * kb) << sh, p, pinv);
             if (LIMB && a.do_in_scale) base = mul_shoup(base, a.in_scale.x, a.in_scale.y, p);
@@ -577,12 +578,37 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
                     for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
                 } else bound_chain<LR0, PTS>(v, pp, lom, colbase, bqs, p, pinv);
             }
+            if (MODE == PM_FWD && GM >= 4 && i == 0 && a.dfr_in) {
+                /* the first pass's boundary twiddles: slot (g, j) holds row r = (g << (9 - GW)) + (t >> GW) +
+                 * (j << s_lo) of this tile (rows of at least 4 bits, so that 2^GW divides the 512 threads), i.e.
+                 * column c1 = (r << lo) + col of the first pass, whose stored row h is
+                 * this tile's block index (mod 2^dfr_gm): w^(K c1) = base_g * step^j with K = brev(h),
+                 * base_g = w^(K ((rb_g << lo) + col)), step = w^(K 2^(s_lo + lo)) (one per tile) */
+                const uint32_t sh1 = 32 - a.dfr_lom;
+                const uint32_t hb = (uint32_t)(gbase >> lom) & ((1u << a.dfr_gm) - 1);
+                const uint32_t K = __brev(hb) >> (32 - a.dfr_gm);
+                const uint32_t col = colbase + c_in;
+                const double step = bq_twiddle(bqs, (K << (s_lo + lo)) << sh1, p, pinv);
+                const double stepq = step * pinv;
+                constexpr int RT = 1 << LRT, GT = PTS / RT;
+#pragma unroll
+                for (int g = 0; g < GT; g++) {
+                    const uint32_t rb = ((uint32_t)g << (GW < 9 ? 9 - GW : 0)) + ((uint32_t)t >> GW);
+                    double wj = bq_twiddle(bqs, (K * ((rb << lo) + col)) << sh1, p, pinv);
+                    if (a.dfr_do_scale) wj = mul_shoup(wj, a.dfr_scale.x, a.dfr_scale.y, p);
+#pragma unroll
+                    for (int j = 0; j < RT; j++) {
+                        if (j > 0) wj = mul_shoup(wj, step, stepq, p);
+                        v[g * RT + j] = mul_barrett(v[g * RT + j], wj, p, pinv);
+                    }
+                }
+            }
             ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, stw, p, pinv);)
             if (i + 1 < NR)
                 ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
                     cur[e] = v[sl]; });)
         }
-        if (!INV) {
+        if (bound_out) {
             if (GM == 8) {
 #pragma unroll
                 for (int j = 0; j < PTS; j++) v[j] = mul_barrett(v[j], w[brev_c<4>(j)], p, pinv);
@@ -980,6 +1006,17 @@ void nttm_set_fixed_shapes(int on) { g_fixed_shapes = on != 0; }
 static int g_stream = 1, g_ncu = 256;                           /* streaming strided passes; CUs of the device */
 void nttm_set_stream(int on, int ncu) { g_stream = on != 0; g_ncu = ncu > 0 ? ncu : 256; }
 
+/* whether nttm_run_pass runs this forward pass as the streamed kernel (the condition in nttm_run_pass below) */
+bool nttm_fwd_pass_streams(const PassArgs &a, const PassGeom &g)
+{
+    if (!pass_geom_ok(g, a.logn, a.batch) || a.y) return false;
+    const bool limb = a.limbs || a.ldesc;
+    const int T = 1 << (g.m + g.logw + g.logh);
+    const bool fixed = g_fixed_shapes && g.logh == 0 && T >= PASS_PTS;
+    const bool shape = g.m >= 1 && g.m <= 9 && g.m + g.logw == 13;
+    return fixed && g_stream && shape && g.lo >= g.logw && !a.do_scale &&
+           (!limb || ((a.ldesc || a.batch == 1) && g.lo + g.m < 32));
+}
 int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
 {
     if (!pass_geom_ok(g, a.logn, a.batch)) return nttm_fail(NTTM_EINVAL, "internal: bad pass geometry");
@@ -1022,6 +1059,8 @@ int nttm_run_pass(bool inv, const PassArgs &a, const PassGeom &g, hipStream_t s)
     const bool stream = fixed && g_stream && shape && g.lo >= g.logw && mode != PM_INV_PW &&
                         (mode == PM_INV || !b.do_scale) &&
                         (mode != PM_FWD_LIMB || ((b.ldesc || b.batch == 1) && g.lo + g.m < 32));
+    if ((b.dfr_out && (!stream || mode != PM_FWD_LIMB)) || (b.dfr_in && (!stream || mode != PM_FWD || g.m < 4)))
+        return nttm_fail(NTTM_EINVAL, "internal: deferred boundary twiddles on a pass that does not stream");
     if (stream) {
         const unsigned nb = (unsigned)(tiles < (uint64_t)g_ncu ? tiles : (uint64_t)g_ncu);
         const uint32_t nt = (uint32_t)tiles;
