@@ -16,6 +16,7 @@
  * CRT's block carry maps (crt.hip), normalized lengths read back as one word; only the few-limb base of the Newton
  * iteration and the final fraction cross to the host. */
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 #include <chrono>
 #include <memory>
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(64) void k_bs_school(const uint64_t *in, uint32_t w
 /* ---- the device backend of ebig.h ------------------------------------------------------------------------------------ */
 struct DBuf {
     uint64_t *p = nullptr;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    bool own = true;                                              /* false: a slice of the pipeline's arena */
+    ~DBuf() { if (p && own) (void)hipFree(p); }
 };
 /* a normalized number: n limbs at buf + off (views share the buffer) */
 struct DNum {
@@ -181,19 +183,27 @@ struct DNum {
 struct DevOps {
     typedef DNum Num;
     hipStream_t s = nullptr;
+    /* the numbers' memory: slices of one arena, never reused within the pipeline (hipMalloc and hipFree per number
+     * would each stall the stream); hipMalloc only when the arena is exhausted */
+    char *arena = nullptr; size_t arena_bytes = 0, arena_used = 0;
     uint32_t *scratch = nullptr; size_t scratch_words = 0;        /* carry status words of k_addsub */
     uint64_t *dword = nullptr;                                    /* one device word (lengths) */
     uint64_t *hword = nullptr;                                    /* pinned host words for read-back */
     int err = 0;
     ~DevOps()
     {
+        if (arena) (void)hipFree(arena);
         if (scratch) (void)hipFree(scratch);
         if (dword) (void)hipFree(dword);
         if (hword) (void)hipHostFree(hword);
     }
-    int init(hipStream_t st)
+    int init(hipStream_t st, size_t arena_limbs = 0)
     {
         s = st;
+        if (arena_limbs) {
+            if (hipMalloc(&arena, arena_limbs * 8) == hipSuccess) arena_bytes = arena_limbs * 8;
+            else { (void)hipGetLastError(); arena = nullptr; }    /* without: hipMalloc per number */
+        }
         if (hipMalloc(&dword, 64) != hipSuccess || hipHostMalloc((void **)&hword, 64, 0) != hipSuccess) {
             (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "e pipeline: device words");
         }
@@ -203,6 +213,12 @@ struct DevOps {
     {
         r = DNum();
         std::shared_ptr<DBuf> b = std::make_shared<DBuf>();
+        const size_t bytes = ((n ? n : 1) * 8 + 255) & ~(size_t)255;
+        if (arena && arena_used + bytes <= arena_bytes) {
+            b->p = (uint64_t *)(arena + arena_used); b->own = false; arena_used += bytes;
+            r.buf = b; r.n = n;
+            return 0;
+        }
         if (hipMalloc(&b->p, (n ? n : 1) * 8) != hipSuccess) {
             (void)hipGetLastError(); b->p = nullptr;
             return nttm_fail(NTTM_ENOMEM, "e pipeline: %zu limbs", n);
@@ -414,7 +430,11 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     while (lgamma((double)N + 1.0) < need) N += N / 2 + 1;
     { size_t lo = N / 2, hi = N; while (lo + 1 < hi) { size_t md = (lo + hi) / 2; if (lgamma((double)md + 1.0) < need) lo = md; else hi = md; } N = hi; }
     DevOps o;
-    RCK(o.init((hipStream_t)stream));
+    /* the arena: the series' leaves (about 2 m limbs per schoolbook level) and root, the Newton levels' numbers
+     * (about 9 m limbs over all levels) and the quotient's (about 5 m), with room to spare (14 GB at 10^9 digits);
+     * NTTM_E_ARENA=0 allocates per number */
+    const char *ea = getenv("NTTM_E_ARENA");
+    RCK(o.init((hipStream_t)stream, ea && ea[0] == '0' ? 0 : 32 * (m + 1024)));
     DNum P, Q, A, x, Y;
     RCK(e_series(o, N, P, Q));
     bool ge;
