@@ -9,6 +9,7 @@
 #include <map>
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include "internal.h"
 #include "plan.h"
 
@@ -107,6 +108,7 @@ struct Ctx {
     size_t stage_chunk;                      /* staged bytes per sub-batch of a batch */
     char *chk; size_t chk_bytes;             /* device scratch of the residue post-condition */
     char *bounce; size_t bounce_half;        /* pinned bounce buffer of the host copies: two halves */
+    char *dl; hipEvent_t dlev[2];            /* pinned halves of the large downloads (nttm_copy_to_host) */
     hipEvent_t bev[2]; int bev_live[2];      /* the last copy through each half */
     uint64_t gen;                            /* which nttm_init this context is (plans record it) */
     hipEvent_t ev[4];
@@ -401,6 +403,8 @@ extern "C" void nttm_shutdown(void)
     if (g.stage) (void)hipFree(g.stage);
     if (g.chk) (void)hipFree(g.chk);
     if (g.bounce) (void)hipHostFree(g.bounce);
+    if (g.dl) (void)hipHostFree(g.dl);
+    for (int k = 0; k < 2; k++) if (g.dlev[k]) (void)hipEventDestroy(g.dlev[k]);
     for (int k = 0; k < 2; k++) if (g.bev[k]) (void)hipEventDestroy(g.bev[k]);
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) if (g.pev[i]) (void)hipEventDestroy(g.pev[i]);
@@ -1172,6 +1176,46 @@ static int stage_download(const StageSet &st, hipStream_t s)
     }
     if (!win.empty()) { RCK(bounce_wait((int)((win.size() - 1) & 1))); scatter(win.size() - 1); }
     return nttm_hip_check(hipStreamSynchronize(s), "staging");
+}
+
+/* one large device array to host memory.  Pinned destinations take one DMA.  Pageable ones go through two pinned
+ * 64 MiB halves: the DMA of chunk k overlaps the host copy of chunk k - 1, which is split over up to 8 threads (the
+ * destination's pages are first touched there, so their faults are taken in parallel).  Synchronizes. */
+#define NTTM_DL_HALF ((size_t)64 << 20)
+int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (!bytes) return nttm_hip_check(hipStreamSynchronize(s), "copy to host");
+    MemMap mm;
+    if (mm.kind(dst, bytes) == MEM_PINNED || bytes <= NTTM_DL_HALF) {
+        HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        return nttm_hip_check(hipStreamSynchronize(s), "copy to host");
+    }
+    if (!g.dl) {
+        HIPCK(hipHostMalloc((void **)&g.dl, 2 * NTTM_DL_HALF, hipHostMallocDefault));
+        for (int k = 0; k < 2; k++) HIPCK(hipEventCreateWithFlags(&g.dlev[k], hipEventDisableTiming));
+    }
+    const size_t H = NTTM_DL_HALF, nch = (bytes + H - 1) / H;
+    const unsigned hw = std::thread::hardware_concurrency(), T = hw >= 8 ? 8 : hw ? hw : 1;
+    auto host_copy = [&](size_t c) {
+        const size_t off = c * H, len = std::min(H, bytes - off), part = (len + T - 1) / T;
+        const char *from = g.dl + (c & 1) * H;
+        char *to = (char *)dst + off;
+        std::vector<std::thread> th;
+        for (unsigned i = 1; i < T; i++)
+            if ((size_t)i * part < len)
+                th.emplace_back([=]() { memcpy(to + (size_t)i * part, from + (size_t)i * part, std::min(part, len - (size_t)i * part)); });
+        memcpy(to, from, std::min(part, len));
+        for (std::thread &t : th) t.join();
+    };
+    for (size_t c = 0; c < nch; c++) {
+        const size_t off = c * H, len = std::min(H, bytes - off);
+        HIPCK(hipMemcpyAsync(g.dl + (c & 1) * H, (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+        HIPCK(hipEventRecord(g.dlev[c & 1], s));
+        if (c) { HIPCK(hipEventSynchronize(g.dlev[(c - 1) & 1])); host_copy(c - 1); }
+    }
+    HIPCK(hipEventSynchronize(g.dlev[(nch - 1) & 1]));
+    host_copy(nch - 1);
+    return nttm_hip_check(hipStreamSynchronize(s), "copy to host");
 }
 
 /* the four pointers of a product classed (a, b, x, c with their byte counts); NTTM_EINVAL for memory no entry can use */

@@ -31,6 +31,7 @@
 int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, uint64_t sa, int64_t apow, const uint64_t *b,
                     uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s);
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
+int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
 uint64_t nttm_crt_blocks(uint64_t nc, int per = 8);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 
@@ -449,11 +450,8 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     rc = eb_quotient(o, A, Q, m, x, Y);                           /* floor(A B^m / Q) */
     if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: quotient correction did not converge");
     if (Y.n > m) return nttm_fail(NTTM_ECHECK, "e pipeline: fraction of %zu limbs above %zu", Y.n, m);
-    memset(out, 0, m * 8);
-    if (Y.n) {
-        RCK(nttm_hip_check(hipMemcpyAsync(out, Y.ptr(), Y.n * 8, hipMemcpyDeviceToHost, o.s), "copy fraction"));
-        RCK(nttm_hip_check(hipStreamSynchronize(o.s), "copy fraction"));
-    }
+    if (Y.n) RCK(nttm_copy_to_host(out, Y.ptr(), Y.n * 8, o.s));
+    if (Y.n < m) memset(out + Y.n, 0, (m - Y.n) * 8);
     g_e_ms[3] = ms_since(t0);
     return 0;
 }
