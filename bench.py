@@ -500,32 +500,40 @@ def trace_summary(db):
 
 
 def replay_summary(db, replays=3):
-    """the trailing repeated sequence of library kernels (k_*) in a kernel trace -- the last `replays` identical runs of
-    one schedule, e.g. graph replays -- summarized per run: kernel time per family, and the span from the first start
-    to the last end (the difference is the idle time between launches)"""
+    """the graph replays of a kernel trace -- the last burst of library kernels (k_*, no gap above 200 us) made of at
+    least `replays` identical runs of one schedule -- summarized per run: kernel time per family, and the span from the
+    first start to the last end (the difference is the idle time between launches)"""
     import sqlite3
     con = sqlite3.connect(db)
     rows = con.execute("select name, start, end from kernels order by start").fetchall()
     ks = [r for r in rows if r[0].startswith("k_") or r[0].startswith("void k_")]
-    names = [r[0] for r in ks]
-    period = None
-    # the longest period whose last `replays` runs repeat: a shorter one can be a run of identical length groups inside
-    # one replay, while twice the true period needs twice as many replays in the trace
-    for R in range(len(ks) // replays, 0, -1):
-        tail = names[len(names) - replays * R:]
-        if all(tail[i * R:(i + 1) * R] == tail[:R] for i in range(replays)):
-            period = R
+    bursts, cur = [], ks[:1]
+    for a, b in zip(ks, ks[1:]):
+        if b[1] - a[2] > 200000:
+            bursts.append(cur)
+            cur = [b]
+        else:
+            cur.append(b)
+    bursts.append(cur)
+    runs = None
+    for bt in reversed(bursts):
+        names = [r[0] for r in bt]
+        for P in range(1, len(bt) // replays + 1):       # the shortest period that tiles the whole burst
+            if len(bt) % P == 0 and all(names[i] == names[i % P] for i in range(len(bt))):
+                runs = [bt[len(bt) - (replays - i) * P:len(bt) - (replays - i - 1) * P] for i in range(replays)]
+                break
+        if runs:
             break
-    if period is None:
-        print("no repeated tail found")
+    if not runs:
+        print("no burst of %d repeated runs found" % replays)
         return
-    runs = [ks[len(ks) - (replays - i) * period:len(ks) - (replays - i - 1) * period] for i in range(replays)]
+    period = len(runs[0])
     fam = {}
     for run in runs:
-        for n, s, e in run:
+        for n, s0, e in run:
             f = fam.setdefault(n, [0, 0.0])
             f[0] += 1
-            f[1] += (e - s) / 1e3
+            f[1] += (e - s0) / 1e3
     busy = sum(v[1] for v in fam.values()) / replays
     span = sum((run[-1][2] - run[0][1]) / 1e3 for run in runs) / replays
     print("%d kernels per run; per run: kernel time %.1f us, span %.1f us (%.1f%% idle between launches)"
