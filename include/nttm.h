@@ -271,6 +271,9 @@ int nttm_debug_addsub(uint64_t *r, size_t nout, const uint64_t *a, size_t na, si
 int nttm_debug_gen_limbs(uint64_t *x, size_t n, uint64_t seed, void *stream);
 /* the global element indices of tile `tile` of a pass of that geometry; returns the tile's element count */
 int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, uint64_t *out);
+/* the normalized length of x[0 .. n) (one more than the index of its top non-zero limb, 0 if none) into the device word
+ * *len_dev, by the scan the e pipeline and the planned tree normalize with.  Asynchronous. */
+int nttm_debug_top_nonzero(const uint64_t *x, size_t n, uint64_t *len_dev, void *stream);
 
 #ifdef __cplusplus
 }

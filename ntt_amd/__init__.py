@@ -118,6 +118,7 @@ def lib():
         L.nttm_debug_gen_limbs.argtypes = [vp, sz, u64, vp]; L.nttm_debug_gen_limbs.restype = i32
         L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
         L.nttm_debug_tile_indices.restype = i32
+        L.nttm_debug_top_nonzero.argtypes = [vp, sz, vp, vp]; L.nttm_debug_top_nonzero.restype = i32
         _lib = L
     return _lib
 
@@ -282,8 +283,9 @@ def mul_cut(a, b, lowcut=0, w=WHOLE, out=None, stream=None):
     return c[:n]
 
 
-def bs_tree(leaves, stream=None):
-    """binary-splitting product tree over leaves [(P_i, Q_i)] of int64 device tensors: the root (P, Q), normalized."""
+def bs_tree(leaves, stream=None, caps=None):
+    """binary-splitting product tree over leaves [(P_i, Q_i)] of int64 device tensors: the root (P, Q), normalized.
+    caps: (p_cap, q_cap) limbs of the root's buffers (default: room for the root's unnormalized lengths)."""
     import torch
     arr = (BsNode * len(leaves))()
     for i, (p, q) in enumerate(leaves):
@@ -291,11 +293,21 @@ def bs_tree(leaves, stream=None):
     dev = leaves[0][0].device if leaves[0][0].numel() else leaves[0][1].device
     pcap = sum(p.numel() for p, _ in leaves) + sum(q.numel() for _, q in leaves) + len(leaves)
     qcap = sum(q.numel() for _, q in leaves)
+    if caps is not None:
+        pcap, qcap = caps
     P = torch.empty(max(pcap, 1), dtype=torch.int64, device=dev)
     Q = torch.empty(max(qcap, 1), dtype=torch.int64, device=dev)
     pn, qn = C.c_size_t(), C.c_size_t()
     _check(lib().nttm_bs_tree(arr, len(leaves), _ptr(P), pcap, C.byref(pn), _ptr(Q), qcap, C.byref(qn), _stream(stream)))
     return P[:pn.value], Q[:qn.value]
+
+
+def top_nonzero(x, stream=None):
+    """normalized length of the int64 device tensor x (the device scan behind the e pipeline's normalization)"""
+    import torch
+    out = torch.full((1,), -1, dtype=torch.int64, device=x.device)
+    _check(lib().nttm_debug_top_nonzero(_ptr(x) if x.numel() else None, x.numel(), _ptr(out), _stream(stream)))
+    return int(out.item())
 
 
 def e_fraction(m, stream=None):

@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <vector>
 #include <map>
+#include <array>
 #include <algorithm>
 #include <chrono>
 #include <thread>
@@ -103,6 +104,8 @@ struct Ctx {
     uint64_t *bsp[2]; size_t bsp_limbs[2];   /* the inner levels of a binary-splitting tree (even / odd levels) */
     char *tdesc; size_t tdesc_bytes;         /* the planned tree's descriptors, every group of every level */
     char *tpin; size_t tpin_bytes;           /* their pinned host image, and the groups' flags */
+    hipStream_t tcs;                         /* the tree's descriptor copies (ahead of the launches on the caller's stream) */
+    hipEvent_t tev[8];                       /* the last copies on tcs, in turn */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     char *stage; size_t stage_bytes;         /* device copies of host-resident operands and results */
     size_t stage_chunk;                      /* staged bytes per sub-batch of a batch */
@@ -406,6 +409,8 @@ extern "C" void nttm_shutdown(void)
     if (g.dl) (void)hipHostFree(g.dl);
     for (int k = 0; k < 2; k++) if (g.dlev[k]) (void)hipEventDestroy(g.dlev[k]);
     for (int k = 0; k < 2; k++) if (g.bev[k]) (void)hipEventDestroy(g.bev[k]);
+    for (int k = 0; k < 8; k++) if (g.tev[k]) (void)hipEventDestroy(g.tev[k]);
+    if (g.tcs) (void)hipStreamDestroy(g.tcs);
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) if (g.pev[i]) (void)hipEventDestroy(g.pev[i]);
     memset(&g, 0, sizeof g);
@@ -1519,6 +1524,27 @@ static int tpin_need(size_t bytes)
 /* products per launched group: a level's large length group is launched in chunks, so that the device starts on the
  * first chunk while the host builds the next (the lowest levels have 10^5 products) */
 #define TREE_CHUNK 16384
+#define TREE_NKEY (2 * (NTTM_LOGN_MAX + 1))       /* transform-length classes: 2^logn at logn, 3 2^logn at 33 + logn */
+/* the transform-length class of a product of ncoef coefficients.  choose_len depends on ncoef only through
+ * lg = max(ceil_log2(ncoef), NTTM_PRODUCT_LOGN_MIN) and whether ncoef <= 3 2^(lg - 2), so each of those pairs is
+ * resolved once, on a representative ncoef of the pair */
+struct LenClass {
+    int8_t key[65][2];
+    LenClass() { memset(key, -1, sizeof key); }
+    int operator()(uint64_t ncoef)
+    {
+        int lg = 63 - __builtin_clzll(ncoef | 1); if (((uint64_t)1 << lg) < ncoef) lg++;
+        if (lg < NTTM_PRODUCT_LOGN_MIN) lg = NTTM_PRODUCT_LOGN_MIN;
+        const int f = ncoef <= ((uint64_t)3 << (lg - 2));
+        if (key[lg][f] < 0) {
+            int logn, r3;
+            choose_len(f ? (uint64_t)3 << (lg - 2) : (uint64_t)1 << lg, &logn, &r3);
+            key[lg][f] = (int8_t)(r3 * (NTTM_LOGN_MAX + 1) + logn);
+        }
+        return key[lg][f];
+    }
+};
+static_assert(TREE_NKEY <= 127, "length classes fit int8_t");
 static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q,
                            size_t q_cap, size_t *qn, hipStream_t s)
 {
@@ -1539,69 +1565,101 @@ static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, si
         len.push_back(std::move(nx));
     }
     const int nlev = (int)len.size() - 1;                        /* levels of products */
-    /* resources for every level, sized before the first launch: pools by level parity, planes, descriptors */
+    /* resources for every level, sized before the first launch: pools by level parity, planes, descriptors; each
+     * product's length class kept for its level's grouping (-1: an empty operand) */
     size_t pool_max[2] = {1, 1}, planes_max = 0, desc_total = 0, img_total = 0, ngroups = 0;
     std::vector<int> lev_np(nlev + 1, 3);
-    auto gkey = [](size_t na, size_t nb) { int logn, r3; choose_len(na + nb - 1, &logn, &r3); return r3 * (NTTM_LOGN_MAX + 1) + logn; };
+    std::vector<std::vector<int8_t>> keys(nlev + 1);
+    std::vector<std::array<uint64_t, TREE_NKEY>> lev_cnt(nlev + 1);
+    LenClass lclass;
     for (int k = 1; k <= nlev; k++) {
         size_t tot = 0;
         for (const Len &x : len[k]) tot += x.pn + x.qn;
         pool_max[k & 1] = std::max(pool_max[k & 1], tot);
         const std::vector<Len> &c = len[k - 1];
+        const size_t npairs = c.size() / 2;
         size_t mn_max = 1;
-        std::map<int, std::pair<uint64_t, uint64_t>> grp;           /* key -> (products, CRT blocks) */
-        for (size_t i = 0; i + 1 < c.size(); i += 2) {
-            const Len &l = c[i], &r = c[i + 1];
+        uint64_t *cnt = lev_cnt[k].data(), blk[TREE_NKEY];     /* per class: products, CRT blocks */
+        for (int j = 0; j < TREE_NKEY; j++) cnt[j] = blk[j] = 0;
+        std::vector<int8_t> &kk = keys[k];
+        kk.resize(2 * npairs);
+        for (size_t i = 0; i < npairs; i++) {
+            const Len &l = c[2 * i], &r = c[2 * i + 1];
             const size_t pr[2][2] = {{l.pn, r.qn}, {l.qn, r.qn}};
             for (int t = 0; t < 2; t++) {
                 const size_t na = pr[t][0], nb = pr[t][1];
-                if (!na || !nb) continue;
+                if (!na || !nb) { kk[2 * i + t] = -1; continue; }
                 mn_max = std::max(mn_max, std::min(na, nb));
-                auto &e = grp[gkey(na, nb)];
-                const int key = gkey(na, nb), logn = key % (NTTM_LOGN_MAX + 1), r3 = key > NTTM_LOGN_MAX;
+                const int key = lclass(na + nb - 1);
+                const int logn = key % (NTTM_LOGN_MAX + 1), r3 = key > NTTM_LOGN_MAX;
                 const uint64_t npts = (uint64_t)(r3 ? 3 : 1) << logn;
-                e.first++;
-                e.second += nttm_crt_blocks(na + nb + (t == 0 ? 1 : 0), npts <= 1024 ? 1 : 8);
+                kk[2 * i + t] = (int8_t)key;
+                cnt[key]++;
+                blk[key] += nttm_crt_blocks(na + nb + (t == 0 ? 1 : 0), npts <= 1024 ? 1 : 8);
             }
         }
         RCK(choose_np(mn_max, &lev_np[k]));
-        for (auto &kv : grp) {
-            const int logn = kv.first % (NTTM_LOGN_MAX + 1), r3 = kv.first > NTTM_LOGN_MAX;
-            const uint64_t npts = (uint64_t)(r3 ? 3 : 1) << logn, cnt_all = kv.second.first;
+        for (int key = 0; key < TREE_NKEY; key++) {
+            if (!cnt[key]) continue;
+            const int logn = key % (NTTM_LOGN_MAX + 1), r3 = key > NTTM_LOGN_MAX;
+            const uint64_t npts = (uint64_t)(r3 ? 3 : 1) << logn, cnt_all = cnt[key];
             if (logn > NTTM_LOGN_MAX) return 1;
             for (uint64_t c0 = 0; c0 < cnt_all; c0 += TREE_CHUNK) {   /* the group's chunks (below) */
-                const uint64_t cnt = std::min<uint64_t>(TREE_CHUNK, cnt_all - c0), tb = r3 ? 3 * cnt : cnt;
-                planes_max = std::max(planes_max, (size_t)(lev_np[k] + 1) * cnt * npts * 8);
-                const size_t img = align_up(cnt * sizeof(CrtDesc), 256) + 2 * align_up(cnt * sizeof(LoadDesc), 256) +
+                const uint64_t cn = std::min<uint64_t>(TREE_CHUNK, cnt_all - c0), tb = r3 ? 3 * cn : cn;
+                planes_max = std::max(planes_max, (size_t)(lev_np[k] + 1) * cn * npts * 8);
+                const size_t img = align_up(cn * sizeof(CrtDesc), 256) + 2 * align_up(cn * sizeof(LoadDesc), 256) +
                                    align_up(tb * 4, 256) + 256;
                 img_total += img;
-                desc_total += img + align_up(cnt * 8, 256) + align_up(nttm_crt_scratch_words(kv.second.second) * 4, 256) + 256;
+                desc_total += img + align_up(cn * 8, 256) + align_up(nttm_crt_scratch_words(blk[key]) * 4, 256) + 256;
                 ngroups++;
             }
+        }
+    }
+    /* the root's P and Q go straight to p and q when their planned lengths fit there and no operand of the last level
+     * lies in them (its operands are pool slots above the leaves) */
+    const Len &rl = len[nlev][0];
+    bool root_direct = rl.pn <= p_cap && rl.qn <= q_cap;
+    if (root_direct && nlev == 1) {
+        for (size_t i = 0; i < 2 && root_direct; i++) {
+            const nttm_bs_node &L = leaves[i];
+            const uint64_t *arr[2] = {L.p, L.q};
+            const size_t nn[2] = {L.pn, L.qn};
+            for (int t = 0; t < 2; t++)
+                for (int o = 0; o < 2; o++) {
+                    const uint64_t *d = o ? q : p;
+                    const size_t dn = o ? rl.qn : rl.pn;
+                    if (nn[t] && dn && arr[t] < d + dn && d < arr[t] + nn[t]) root_direct = false;
+                }
         }
     }
     RCK(bsp_need(0, pool_max[0])); RCK(bsp_need(1, pool_max[1]));
     RCK(ws_need(planes_max ? planes_max : 256));
     RCK(tdesc_need(desc_total + 512));
     RCK(tpin_need(img_total + 8 * ngroups + 2048));
+    if (!g.tcs) {
+        HIPCK(hipStreamCreateWithFlags(&g.tcs, hipStreamNonBlocking));
+        for (int j = 0; j < 8; j++) HIPCK(hipEventCreateWithFlags(&g.tev[j], hipEventDisableTiming));
+    }
+    HIPCK(hipStreamSynchronize(g.tcs));                           /* (a failed call's copies) before the image is reused */
     /* the levels */
     std::vector<nttm_bs_node> cur(leaves, leaves + n), nxt;
     std::vector<nttm_prod> prods;
     std::vector<const char *> flag_at;                          /* each group's CRT flag word in device memory */
-    size_t dpos = 0, ipos = 0;
+    size_t dpos = 0, ipos = 0, ncopy = 0;
     for (int k = 1; k <= nlev; k++) {
         const size_t npairs = cur.size() / 2, odd = cur.size() & 1;
+        const bool top = k == nlev && root_direct;
         uint64_t *pool = g.bsp[k & 1];
         nxt.assign(npairs + odd, nttm_bs_node());
-        prods.assign(2 * npairs, nttm_prod());
+        prods.resize(2 * npairs);
         size_t off = 0;
         for (size_t i = 0; i < npairs; i++) {                    /* P = P_L Q_R + P_R, Q = Q_L Q_R: Q_R shared */
             const nttm_bs_node &L = cur[2 * i], &R = cur[2 * i + 1];
-            uint64_t *pc = pool + off; off += L.pn + R.qn + 1;
-            uint64_t *qc = pool + off; off += L.qn + R.qn;
+            uint64_t *pc = top ? p : pool + off; off += L.pn + R.qn + 1;
+            uint64_t *qc = top ? q : pool + off; off += L.qn + R.qn;
             nttm_prod &a = prods[2 * i], &b = prods[2 * i + 1];
-            a.a = L.p; a.na = L.pn; a.b = R.q; a.nb = R.qn; a.c = pc; a.x = R.pn ? R.p : pc; a.nx = R.pn;
-            b.a = L.q; b.na = L.qn; b.b = R.q; b.nb = R.qn; b.c = qc; b.x = nullptr; b.nx = 0;
+            a.a = L.p; a.na = L.pn; a.b = R.q; a.nb = R.qn; a.c = pc; a.x = R.pn ? R.p : pc; a.nx = R.pn; a.ncn = 0;
+            b.a = L.q; b.na = L.qn; b.b = R.q; b.nb = R.qn; b.c = qc; b.x = nullptr; b.nx = 0; b.ncn = 0;
             nxt[i].p = pc; nxt[i].pn = L.pn + R.qn + 1; nxt[i].q = qc; nxt[i].qn = L.qn + R.qn;
         }
         if (odd) {                                               /* the last node moves up (into this level's pool) */
@@ -1612,27 +1670,34 @@ static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, si
             if (L.qn) HIPCK(hipMemcpyAsync(qc, L.q, L.qn * 8, hipMemcpyDeviceToDevice, s));
             nxt[npairs] = L; nxt[npairs].p = pc; nxt[npairs].q = qc;
         }
-        /* zero-operand products: c = x, zero-extended */
-        std::vector<std::vector<size_t>> groups(2 * (NTTM_LOGN_MAX + 1));
+        /* zero-operand products (c = x, zero-extended); the others by length class */
+        const std::vector<int8_t> &kk = keys[k];
+        std::vector<std::vector<size_t>> groups(TREE_NKEY);
+        for (int key = 0; key < TREE_NKEY; key++) groups[key].reserve(lev_cnt[k][key]);
         for (size_t i = 0; i < prods.size(); i++) {
+            if (kk[i] >= 0) { groups[kk[i]].push_back(i); continue; }
             const nttm_prod &pq = prods[i];
             const size_t nc = pq.na + pq.nb + (pq.x ? 1 : 0);
-            if (pq.na && pq.nb) { groups[gkey(pq.na, pq.nb)].push_back(i); continue; }
             if (nc) HIPCK(hipMemsetAsync(pq.c, 0, nc * 8, s));
             if (pq.x && pq.nx) HIPCK(hipMemcpyAsync(pq.c, pq.x, pq.nx * 8, hipMemcpyDeviceToDevice, s));
         }
-        for (size_t gi = 0; gi < groups.size(); gi++) {
+        for (int gi = 0; gi < TREE_NKEY; gi++) {
             const std::vector<size_t> &Gall = groups[gi];
             for (size_t c0 = 0; c0 < Gall.size(); c0 += TREE_CHUNK) {
                 const std::vector<size_t> G(Gall.begin() + c0, Gall.begin() + std::min(Gall.size(), c0 + TREE_CHUNK));
-                const int r3 = gi > NTTM_LOGN_MAX, logn = (int)(gi % (NTTM_LOGN_MAX + 1));
+                const int r3 = gi > NTTM_LOGN_MAX, logn = gi % (NTTM_LOGN_MAX + 1);
                 GroupBuild gb;
                 group_build(gb, prods.data(), G, lev_np[k], r3, logn, g.cfg.tile_log, batch_pts(logn));
                 if (dpos + gb.desc_bytes > g.tdesc_bytes || ipos + gb.off_ncn > img_total || gb.planes_bytes > g.ws_bytes)
                     return nttm_fail(NTTM_EINVAL, "internal: tree plan exceeds its sizing (level %d)", k);
                 char *img = g.tpin + ipos, *dbase = g.tdesc + dpos;
                 group_image(gb, (double *)g.ws, img);
-                HIPCK(hipMemcpyAsync(dbase, img, gb.off_ncn, hipMemcpyHostToDevice, s));
+                /* the descriptors copied on the side stream, which runs ahead of the launches (their region of the
+                 * image and of tdesc is this group's alone): the caller's stream waits on the copy, not behind it */
+                hipEvent_t ev = g.tev[ncopy++ % 8];
+                HIPCK(hipMemcpyAsync(dbase, img, gb.off_ncn, hipMemcpyHostToDevice, g.tcs));
+                HIPCK(hipEventRecord(ev, g.tcs));
+                HIPCK(hipStreamWaitEvent(s, ev, 0));
                 RCK(group_launch(gb, lev_np[k], (double *)g.ws, dbase, s, false));
                 flag_at.push_back(dbase + gb.off_crt + 4);
                 ipos = align_up(ipos + gb.off_ncn, 256);
@@ -1660,9 +1725,11 @@ static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, si
     if (rp > p_cap || rq > q_cap)
         return nttm_fail(NTTM_EINVAL, "nttm_bs_tree: the root needs %zu and %zu limbs (capacities %zu, %zu)", rp, rq,
                          p_cap, q_cap);
-    if (rp) HIPCK(hipMemcpyAsync(p, root.p, rp * 8, hipMemcpyDeviceToDevice, s));
-    if (rq) HIPCK(hipMemcpyAsync(q, root.q, rq * 8, hipMemcpyDeviceToDevice, s));
-    HIPCK(hipStreamSynchronize(s));
+    if (!root_direct) {
+        if (rp) HIPCK(hipMemcpyAsync(p, root.p, rp * 8, hipMemcpyDeviceToDevice, s));
+        if (rq) HIPCK(hipMemcpyAsync(q, root.q, rq * 8, hipMemcpyDeviceToDevice, s));
+        HIPCK(hipStreamSynchronize(s));
+    }
     *pn = rp; *qn = rq;
     return 0;
 }

@@ -412,10 +412,13 @@ static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
         cap += hl[2 * i + 1];                                      /* |Q| <= sum of the leaves' |Q|; P < 2 Q */
     }
     const uint64_t nleaves = nodes;
-    RCK(o.alloc(cap, P));
+    /* P's buffer also holds the root's unnormalized length (|P_0| <= |Q_0| + 1, one limb per level), so the tree
+     * writes the root in place */
+    const size_t pcap = cap + 72;
+    RCK(o.alloc(pcap, P));
     RCK(o.alloc(cap, Q));
     size_t pn = 0, qn = 0;
-    RCK(nttm_bs_tree(leaves.data(), nleaves, P.wptr(), cap, &pn, Q.wptr(), cap, &qn, s));
+    RCK(nttm_bs_tree(leaves.data(), nleaves, P.wptr(), pcap, &pn, Q.wptr(), cap, &qn, s));
     P.n = pn; Q.n = qn;                                            /* normalized by the tree */
     g_e_ms[1] = ms_since(t0);
     return 0;

@@ -100,3 +100,45 @@ def test_bs_tree_planned_equals_levelwise(nttm):
     if O.have_gmp() and len(q1) < 200000:
         pw, qw = combine([(to_int(p), to_int(q)) for p, q in leaves])
         assert np.array_equal(q1, from_int(qw)) and np.array_equal(p1, from_int(pw))
+
+
+def test_bs_tree_root_caps(nttm):
+    """the root written straight into the caller's buffers (room for its unnormalized lengths) and through the pool
+    (room for the normalized lengths only): the same limbs; two leaves (one level, operands in the caller's leaves)
+    and 77 leaves"""
+    rng = np.random.default_rng(31)
+    for nleaf in (2, 77):
+        leaves, s = [], 3
+        for i in range(nleaf):
+            lq = int(rng.integers(1, 300))
+            lp = int(rng.integers(0, lq))
+            p, s = O.gen_limbs(lp, s) if lp else (np.zeros(0, dtype=np.uint64), s)
+            q, s = O.gen_limbs(lq, s)
+            leaves.append((dev(p), dev(q)))
+        say("bs tree caps", nleaf)
+        P1, Q1 = nttm.bs_tree(leaves)
+        p1, q1 = host(P1), host(Q1)
+        P2, Q2 = nttm.bs_tree(leaves, caps=(len(p1), len(q1)))
+        assert np.array_equal(p1, host(P2)) and np.array_equal(q1, host(Q2))
+        pw, qw = combine([(to_int(host(p)), to_int(host(q))) for p, q in leaves])
+        assert np.array_equal(q1, from_int(qw)) and np.array_equal(p1, from_int(pw))
+
+
+@pytest.mark.parametrize("n", [1, 5, 4095, 4096, 4097, 3 * 4096 + 17, 100003])
+def test_top_nonzero(nttm, n):
+    """the top-down scan with early exit: all zero, the top limb set, the top non-zero limb several chunks (4096 limbs)
+    below the top, only limb 0 set, non-zero limbs scattered below the top one"""
+    import torch
+    rng = np.random.default_rng(n)
+    cases = [np.zeros(n, dtype=np.uint64)]
+    x = np.zeros(n, dtype=np.uint64); x[n - 1] = 5; cases.append(x)
+    x = np.zeros(n, dtype=np.uint64); x[0] = 1; cases.append(x)
+    for top in {n // 2, max(0, n - 3 * 4096 - 5), max(0, n - 4097), max(0, n - 4096)}:
+        x = np.zeros(n, dtype=np.uint64)
+        x[: top + 1] = rng.integers(0, 3, size=top + 1).astype(np.uint64)
+        x[top] = 7
+        cases.append(x)
+    for x in cases:
+        want = int(np.flatnonzero(x)[-1]) + 1 if x.any() else 0
+        say("top nonzero", n, want)
+        assert nttm.top_nonzero(dev(x)) == want
