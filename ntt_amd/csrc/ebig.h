@@ -14,8 +14,12 @@
  * eps_h <= 3 B^-h and l <= 2h - 2 the squared term is below 9/B units, so the error stays within 3 units at every
  * level.  Each step costs two products of about 1.5 l limbs; the levels halve down to a base of at most 8 limbs, which
  * is exact long division.
- * Quotient: Y = floor(A x_m / B^n) lies within 4 of floor(A B^m / Q) for A < Q; one product Y Q and a few
- * additions and subtractions of Q to the remainder R = A B^m - Y Q make it exact. */
+ * Quotient, from x = x_(m+1) (one guard limb): for A < Q, A B^m / Q = (T + A d) / B^(n+1) with T = A x and |d| <= 3,
+ * so with Y = floor(T / B^(n+1)) and T_lo = T mod B^(n+1), A B^m / Q = Y + (T_lo + A d) / B^(n+1), where
+ * |A d| < 3 B^n.  When limb n of T (the top limb of T_lo) lies in [3, B - 4], 0 < T_lo + A d < B^(n+1) and Y is
+ * exact: one product and one limb read.  Otherwise (a fraction A B^m / Q within about 3 B^-1 of an integer, e.g. an
+ * exact quotient) Y lies within 1 of the quotient, and the remainder R = A B^m - Y Q (one more product) with a few
+ * additions or subtractions of Q makes it exact. */
 #ifndef NTTM_EBIG_H
 #define NTTM_EBIG_H
 #include <stdint.h>
@@ -168,16 +172,21 @@ template <class O> int eb_recip(O &o, const typename O::Num &Q, size_t m, typena
     return 0;
 }
 
-/* Y = floor(A B^m / Q) for 0 <= A < Q, from x ~ B^(n+m) / Q within 3 units */
+/* Y = floor(A B^m / Q) for 0 <= A < Q, from x ~ B^(n+m+1) / Q within 3 units; full_check: always form the remainder
+ * (the path the rare undecided case takes) */
 template <class O> int eb_quotient(O &o, const typename O::Num &A, const typename O::Num &Q, size_t m,
-                                   const typename O::Num &x, typename O::Num &Y)
+                                   const typename O::Num &x, typename O::Num &Y, bool full_check = false)
 {
     typedef typename O::Num N;
     const size_t n = o.size(Q);
     N T, R, R2;
     int rc = o.mul(A, x, T);
     if (rc) return rc;
-    Y = o.shr(T, n);
+    Y = o.shr(T, n + 1);
+    const uint64_t tn = o.limb(T, n);
+    if (tn == ~(uint64_t)0) return -1;                               /* (a failed read) */
+    if (!full_check && tn >= 3 && tn <= EB_BASE - 4) return 0;
+    T = N();
     if ((rc = o.mul(Y, Q, T))) return rc;
     bool ge, neg = false;
     if ((rc = o.addsub(R, A, m, T, 0, true, &ge))) return rc;      /* R = A B^m - Y Q */

@@ -446,11 +446,12 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     if (!ge || !A.n) return nttm_fail(NTTM_ECHECK, "e pipeline: P <= Q");
     P = DNum();
     auto t0 = eclk::now();
-    int rc = eb_recip(o, Q, m, x);                                /* x ~ B^(n+m) / Q */
+    int rc = eb_recip(o, Q, m + 1, x);                            /* x ~ B^(n+m+1) / Q (one guard limb) */
     if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: reciprocal invariant broken%s", o.err ? " (read-back failed)" : "");
     g_e_ms[2] = ms_since(t0);
     t0 = eclk::now();
-    rc = eb_quotient(o, A, Q, m, x, Y);                           /* floor(A B^m / Q) */
+    const char *fc = getenv("NTTM_E_FULL_CHECK");                 /* 1: always form the remainder (for tests) */
+    rc = eb_quotient(o, A, Q, m, x, Y, fc && fc[0] == '1');       /* floor(A B^m / Q) */
     if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: quotient correction did not converge");
     if (Y.n > m) return nttm_fail(NTTM_ECHECK, "e pipeline: fraction of %zu limbs above %zu", Y.n, m);
     if (Y.n) RCK(nttm_copy_to_host(out, Y.ptr(), Y.n * 8, o.s));

@@ -54,14 +54,15 @@ long eb_t_recip(const uint64_t *q, size_t n, size_t m, uint64_t *x, size_t cap)
     if (eb_recip(o, load(q, n), m, X)) return -1;
     return (long)store(X, x, cap);
 }
-/* floor(A B^m / Q) through eb_recip and eb_quotient; returns the limbs written, or -1 */
-long eb_t_quotient(const uint64_t *a, size_t na, const uint64_t *q, size_t n, size_t m, uint64_t *y, size_t cap)
+/* floor(A B^m / Q) through eb_recip (one guard limb) and eb_quotient (full: always through the remainder); returns
+ * the limbs written, or -1 */
+long eb_t_quotient(const uint64_t *a, size_t na, const uint64_t *q, size_t n, size_t m, uint64_t *y, size_t cap, int full)
 {
     SchoolMul mul;
     HostOps<SchoolMul> o(mul);
     const Big Q = load(q, n);
     Big X, Y;
-    if (eb_recip(o, Q, m, X) || eb_quotient(o, load(a, na), Q, m, X, Y)) return -1;
+    if (eb_recip(o, Q, m + 1, X) || eb_quotient(o, load(a, na), Q, m, X, Y, full != 0)) return -1;
     return (long)store(Y, y, cap);
 }
 /* floor(B^e / D) by the host long division */

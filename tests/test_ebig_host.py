@@ -28,7 +28,7 @@ def E():
     L = C.CDLL(SO)
     sz = C.c_size_t
     L.eb_t_recip.argtypes = [U64P, sz, sz, U64P, sz]; L.eb_t_recip.restype = C.c_long
-    L.eb_t_quotient.argtypes = [U64P, sz, U64P, sz, sz, U64P, sz]; L.eb_t_quotient.restype = C.c_long
+    L.eb_t_quotient.argtypes = [U64P, sz, U64P, sz, sz, U64P, sz, C.c_int]; L.eb_t_quotient.restype = C.c_long
     L.eb_t_div_pow.argtypes = [sz, U64P, sz, U64P, sz]; L.eb_t_div_pow.restype = C.c_long
     L.eb_t_products.restype = C.c_long
     L.eb_t_max_operand.restype = sz
@@ -61,12 +61,12 @@ def recip(E, q, m):
     return value(out, k)
 
 
-def quotient(E, a, q, m):
+def quotient(E, a, q, m, full=0):
     aa, na = arr(a)
     qa, n = arr(q)
     cap = m + 2
     out = (C.c_uint64 * cap)()
-    k = E.eb_t_quotient(aa, na, qa, n, m, out, cap)
+    k = E.eb_t_quotient(aa, na, qa, n, m, out, cap, full)
     assert k >= 0
     return value(out, k)
 
@@ -98,12 +98,44 @@ def test_recip_within_three_units(E, n, m):
         assert abs(x - B ** (n + m) // q) <= 3, (n, m)
 
 
+@pytest.mark.parametrize("full", [0, 1])
 @pytest.mark.parametrize("n,m", [(1, 1), (2, 3), (4, 9), (10, 10), (25, 60), (60, 25), (3, 120)])
-def test_quotient_exact(E, n, m):
+def test_quotient_exact(E, n, m, full):
+    """the one-product quotient (the guard limb decides) and the path through the remainder"""
     rnd = random.Random(7 * n + m)
     for q in q_kinds(rnd, n):
         for a in (0, 1, q - 1, rnd.randrange(q), q // 2):
-            assert quotient(E, a, q, m) == a * B ** m // q, (n, m)
+            assert quotient(E, a, q, m, full) == a * B ** m // q, (n, m)
+
+
+def quotient_products(E, a, q, m):
+    """products that eb_quotient made for floor(a B^m / q) (1: the guard limb decided; 2: through the remainder)"""
+    E.eb_t_reset()
+    recip(E, q, m + 1)
+    r = E.eb_t_products()
+    E.eb_t_reset()
+    assert quotient(E, a, q, m) == a * B ** m // q
+    return E.eb_t_products() - r
+
+
+@pytest.mark.parametrize("m", [4, 9, 30])
+def test_quotient_exact_divisions(E, m):
+    """Q a product of powers of 2 and 5 that divides B^m, so A B^m / Q is an integer for every A: the fraction sits
+    on an integer, the guard limb cannot decide, and the quotient goes through the remainder (two products)"""
+    for q in (2 ** 60, 5 ** 26, 2 ** 40 * 5 ** 30, 2 ** 100 * 5 ** 3, 1 << 59):
+        if B ** m % q:
+            continue
+        for a in (1, 3, q - 1, q // 2, q // 3, (q * 2) // 3 + 1):
+            assert quotient_products(E, a, q, m) == 2, (q, a, m)
+
+
+def test_quotient_one_product(E):
+    """random A < Q: the guard limb decides, and the quotient takes the one product A x"""
+    rnd = random.Random(17)
+    for n, m in ((3, 7), (10, 20), (40, 25)):
+        q = rnd.randrange(B ** (n - 1), B ** n)
+        for _ in range(8):
+            assert quotient_products(E, rnd.randrange(q), q, m) == 1, (n, m)
 
 
 def test_reciprocal_products_on_truncated_q(E):

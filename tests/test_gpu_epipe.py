@@ -37,6 +37,23 @@ def test_e_fraction_exact(nttm, m):
     assert sum(int(v) * B ** i for i, v in enumerate(got)) == want
 
 
+@pytest.mark.parametrize("m", [7, 300, 55600])
+def test_e_fraction_full_check(nttm, m):
+    """the quotient through the remainder A B^m - Y Q (NTTM_E_FULL_CHECK=1, the path of the rare case the guard limb
+    leaves undecided) gives the same limbs as the one-product quotient, and both the exact value at small sizes"""
+    import numpy as np
+    say("e fraction full check", m)
+    fast = nttm.e_fraction(m)
+    os.environ["NTTM_E_FULL_CHECK"] = "1"
+    try:
+        full = nttm.e_fraction(m)
+    finally:
+        os.environ.pop("NTTM_E_FULL_CHECK", None)
+    assert np.array_equal(fast, full)
+    if m <= 300:
+        assert sum(int(v) * B ** i for i, v in enumerate(full)) == e_fraction_python(m)
+
+
 def test_e_million_digits_windows(nttm):
     m = 55600                                                    # 1,000,800 digits
     say("e 1e6 digits")
