@@ -274,6 +274,11 @@ int nttm_debug_tile_indices(int lo, int m, int logw, int logh, uint64_t tile, ui
 /* the normalized length of x[0 .. n) (one more than the index of its top non-zero limb, 0 if none) into the device word
  * *len_dev, by the scan the e pipeline and the planned tree normalize with.  Asynchronous. */
 int nttm_debug_top_nonzero(const uint64_t *x, size_t n, uint64_t *len_dev, void *stream);
+/* the cyclic product behind the e pipeline's wrapped Newton product: v[0 .. L + 3) = sum_k c_k 10^(18 k) for
+ * c_k = sum over i + j = k mod L of a_i b_j (a b mod (x^L - 1) at x = 10^18, carries not wrapped), L = 2^k or 3 2^k
+ * (k >= 4), na, nb <= L, device pointers.  Synchronous. */
+int nttm_debug_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L,
+                          void *stream);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,7 @@ def lib():
         L.nttm_debug_tile_indices.argtypes = [i32, i32, i32, i32, u64, C.POINTER(C.c_uint64)]
         L.nttm_debug_tile_indices.restype = i32
         L.nttm_debug_top_nonzero.argtypes = [vp, sz, vp, vp]; L.nttm_debug_top_nonzero.restype = i32
+        L.nttm_debug_mul_cyclic.argtypes = [vp, vp, sz, vp, sz, sz, vp]; L.nttm_debug_mul_cyclic.restype = i32
         _lib = L
     return _lib
 
@@ -308,6 +309,16 @@ def top_nonzero(x, stream=None):
     out = torch.full((1,), -1, dtype=torch.int64, device=x.device)
     _check(lib().nttm_debug_top_nonzero(_ptr(x) if x.numel() else None, x.numel(), _ptr(out), _stream(stream)))
     return int(out.item())
+
+
+def mul_cyclic(a, b, L, stream=None):
+    """the cyclic product of int64 device tensors a, b (at most L limbs each): L + 3 limbs of sum_k c_k B^k, c the
+    length-L cyclic convolution of their limbs"""
+    import torch
+    v = torch.empty(L + 3, dtype=torch.int64, device=a.device if a.numel() else b.device)
+    _check(lib().nttm_debug_mul_cyclic(_ptr(v), _ptr(a) if a.numel() else None, a.numel(),
+                                       _ptr(b) if b.numel() else None, b.numel(), L, _stream(stream)))
+    return v
 
 
 def e_fraction(m, stream=None):

@@ -731,6 +731,59 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
  * array weighted by powers of B^65536 on the host.  Costs one read of a, b, x and c. */
 #define NTTM_CHECK_Q 4611686018427388039ULL
 struct ResChunkH { const uint64_t *x; uint64_t n; };          /* must match digits.hip's ResChunk */
+/* the cyclic product of length L = 2^logn or 3 2^logn (at least 16 points): v[0 .. L + 3) = sum_k c_k B^k over the
+ * cyclic convolution c_k = sum_(i + j = k mod L) a_i b_j, i.e. a b mod (x^L - 1) at x = B before its carries wrap (the
+ * caller folds v modulo B^L - 1).  The transforms are DFTs of length L, so the pointwise product gives the cyclic
+ * convolution directly; each c_k has at most min(na, nb) terms, the bound of an ordinary product.  na, nb <= L, device
+ * pointers; synchronous.  The Newton reciprocal's wrapped product (GMP's mpn_mulmod_bnm1 plays this part there). */
+int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s)
+{
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    int logn = -1, r3 = 0;
+    if (L >= 16 && !(L & (L - 1))) logn = __builtin_ctzll(L);
+    else if (L % 3 == 0 && L / 3 >= 16 && !((L / 3) & (L / 3 - 1))) { logn = __builtin_ctzll(L / 3); r3 = 1; }
+    if (logn < NTTM_PRODUCT_LOGN_MIN || logn > NTTM_LOGN_MAX)
+        return nttm_fail(NTTM_EINVAL, "cyclic product: length %zu is not 2^k or 3 2^k (k in [%d, %d])", L,
+                         NTTM_PRODUCT_LOGN_MIN, NTTM_LOGN_MAX);
+    if (na > L || nb > L) return nttm_fail(NTTM_EINVAL, "cyclic product: operands of %zu and %zu limbs above %zu", na, nb, L);
+    if (!v || (na && !a) || (nb && !b)) return nttm_fail(NTTM_EINVAL, "cyclic product: null pointer");
+    const size_t nc = L + 3;
+    if (!na || !nb) { HIPCK(hipMemsetAsync(v, 0, nc * 8, s)); HIPCK(hipStreamSynchronize(s)); return 0; }
+    int np; RCK(choose_np(na < nb ? na : nb, &np));
+    const size_t planes_bytes = (size_t)(np + 1) * L * 8;
+    const size_t off_desc = align_up(planes_bytes, 256), off_crt = off_desc + 256;
+    RCK(ws_need(off_crt + nttm_crt_scratch_words(nttm_crt_blocks(nc)) * 4 + 256));
+    double *A = (double *)g.ws, *Bp = A + (size_t)np * L;
+    RCK(product_transforms(np, logn, r3, 1, A, Bp, a, na, b, nb, nullptr, s, nullptr));
+    CrtDesc hd; memset(&hd, 0, sizeof hd);
+    hd.pl = A; hd.plane = L; hd.ncoef = L; hd.c = v; hd.nc = nc;
+    char *hs = (char *)g.hpin;
+    memcpy(hs, &hd, sizeof hd);
+    HIPCK(hipMemcpyAsync(g.ws + off_desc, hs, sizeof hd, hipMemcpyHostToDevice, s));
+    uint32_t *crt_scr = (uint32_t *)(g.ws + off_crt);
+    RCK(nttm_crt_launch_batch(g.ws + off_desc, 1, nttm_crt_blocks(nc), np, g.cc, crt_scr, s, 8.0 * np * L + 8.0 * nc));
+    HIPCK(hipMemcpyAsync(hs + 96, crt_scr, 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    if (((const uint32_t *)(hs + 96))[1] & 1) return nttm_fail(NTTM_ECHECK, "cyclic product: crt: non-zero carry beyond the top limb");
+    return 0;
+}
+
+extern "C" int nttm_debug_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L,
+                                     void *stream)
+{
+    if ((na && nttm_require_device(a, na * 8, "a")) || (nb && nttm_require_device(b, nb * 8, "b")) ||
+        nttm_require_device(v, (L + 3) * 8, "v"))
+        return nttm_fail(NTTM_EINVAL, "nttm_debug_mul_cyclic: device pointers only");
+    return nttm_mul_cyclic(v, a, na, b, nb, L, (hipStream_t)stream);
+}
+/* the transform length (points) the product rule gives `need` coefficients: a supported length of the cyclic product */
+size_t nttm_wrap_len(size_t need)
+{
+    int logn, r3;
+    choose_len(need ? need : 1, &logn, &r3);
+    return (size_t)(r3 ? 3 : 1) << logn;
+}
+
 int nttm_run_res_chunks(const void *chunks_dev, uint64_t nchunks, uint64_t q, uint64_t *out_dev, hipStream_t s);
 uint64_t nttm_res_chunk_limbs(void);
 static int chk_need(size_t bytes)

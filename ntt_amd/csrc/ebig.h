@@ -3,9 +3,11 @@
  * Numbers are base-10^18 limb vectors (least significant first, normalized: no leading zero limb; 0 is empty).  The
  * algorithms below use only the backend's operations (the device backend in epipe.hip, HostOps here for the host):
  *   size(a), top(a, t) (the top t limbs), shr(a, k) (drop the low k limbs), limb(a, i), low(a, k, r) (a mod B^k),
- *   mul(a, b, c), pow_sub(K, a, r) (B^K - a, for a < B^K), addsub(r, a, sa, b, sb, sub, &ge) (a B^sa +- b B^sb; for a
- *   subtraction ge tells whether it was >= 0, and r is only meaningful then), add_one / sub_one, div_pow(e, d, r)
- *   (floor(B^e / d) for a few limbs).
+ *   mul(a, b, c), pow_sub(K, a, r) (B^K - a, for a < B^K), sub_pow(r, a, K, &ge) (a - B^K), addsub(r, a, sa, b, sb, sub,
+ *   &ge) (a B^sa +- b B^sb; for a subtraction ge tells whether it was >= 0, and r is only meaningful then), add_one /
+ *   sub_one, div_pow(e, d, r) (floor(B^e / d) for a few limbs), and the wrapped product mul_wrap(a, b, L, r)
+ *   (a b mod (B^L - 1)) with wrap_len(need) (a length L >= need it supports) and wrap_min() (the reciprocal size from
+ *   which eb_recip takes it).
  *
  * Reciprocal (the reference's Newton correction form, E/ALGORITHM.md:197-246, E/newton_db.c:218): x_l approximates
  * B^(n+l) / Q (Q of n limbs) to within 3 units.  From x_h to x_l with l <= 2h - 2, using only the top t = l + 2 limbs
@@ -13,7 +15,9 @@
  * The relative error after the step is eps_h^2 + B^(-l-1) (the truncation of Q) + one unit of rounding; with
  * eps_h <= 3 B^-h and l <= 2h - 2 the squared term is below 9/B units, so the error stays within 3 units at every
  * level.  Each step costs two products of about 1.5 l limbs; the levels halve down to a base of at most 8 limbs, which
- * is exact long division.
+ * is exact long division.  Q_t x_h is B^K - E with |E| <= B^(h+1) + 3 B^t <= 4 B^t, so above wrap_min() it is formed
+ * modulo B^L - 1 only (L >= t + 2, about 2/3 of the whole product's length), and E read off the residue: the
+ * wrap-around of the Newton product (as GMP's mpn_invertappr does with mpn_mulmod_bnm1).
  * Quotient, from x = x_(m+1) (one guard limb): for A < Q, A B^m / Q = (T + A d) / B^(n+1) with T = A x and |d| <= 3,
  * so with Y = floor(T / B^(n+1)) and T_lo = T mod B^(n+1), A B^m / Q = Y + (T_lo + A d) / B^(n+1), where
  * |A d| < 3 B^n.  When limb n of T (the top limb of T_lo) lies in [3, B - 4], 0 < T_lo + A d < B^(n+1) and Y is
@@ -130,6 +134,38 @@ template <class Mul> struct HostOps {
         return 0;
     }
     int add_one(Big &a) const { a = big_add(a, big_small(1)); return 0; }
+    int sub_pow(Big &r, const Big &a, size_t K, bool *ge) const
+    {
+        const Big p = big_pow(K);
+        *ge = big_cmp(a, p) >= 0;
+        r = *ge ? big_sub(a, p) : Big();
+        return 0;
+    }
+    /* the wrapped product a b mod (B^L - 1): the whole product folded in L-limb pieces (wrap_h: from which reciprocal
+     * size on eb_recip takes it; any length is a valid L here, the lengths below mirror the device's 2^k and 3 2^k) */
+    size_t wrap_h = 0;
+    long wraps = 0;                                               /* wrapped products made */
+    size_t wrap_min() const { return wrap_h; }
+    size_t wrap_len(size_t need) const
+    {
+        size_t p2 = 16, p3 = 48;
+        while (p2 < need) p2 *= 2;
+        while (p3 < need) p3 *= 2;
+        return std::min(p2, p3);
+    }
+    int mul_wrap(const Big &a, const Big &b, size_t L, Big &r)
+    {
+        Big c;
+        int rc = m(a, b, c);
+        if (rc) return rc;
+        wraps++;
+        r.clear();
+        for (size_t i = 0; i < c.size(); i += L)
+            r = big_add(r, Big(c.begin() + i, c.begin() + std::min(c.size(), i + L)));
+        while (r.size() > L) r = big_add(Big(r.begin(), r.begin() + L), Big(r.begin() + L, r.end()));
+        big_norm(r);
+        return 0;
+    }
     int sub_one(Big &a) const { if (a.empty()) return -1; a = big_sub(a, big_small(1)); return 0; }
     int div_pow(size_t e, const Big &d, Big &r) const { r = big_div_pow(e, d); return 0; }
 };
@@ -150,15 +186,39 @@ template <class O> int eb_recip(O &o, const typename O::Num &Q, size_t m, typena
     for (size_t i = lv.size() - 1; i-- > 0;) {
         const size_t l = lv[i], t = std::min(n, l + 2), K = t + h;
         N T, E, D, xn;
-        if ((rc = o.mul(o.top(Q, t), x, T))) return rc;
         bool neg;
-        if (o.size(T) > K) {                                        /* T >= B^K: E = T - B^K, T < 2 B^K */
-            if (o.size(T) != K + 1 || o.limb(T, K) != 1) return -1;
-            if ((rc = o.low(T, K, E))) return rc;
-            neg = true;
+        const N Qt = o.top(Q, t);
+        const size_t L = o.wrap_len(t + 2);
+        if (h >= o.wrap_min() && L < o.wrap_len(o.size(Qt) + o.size(x) - 1)) {
+            /* T modulo B^L - 1 only: E = B^K - T has |E| < 3 Q_t < 3 B^t <= B^(L-2), so its residue U = B^(K mod L) - T
+             * mod (B^L - 1) is E itself when small (size <= L - 1) and B^L - 1 - |E| when E < 0 (size L) */
+            const size_t Kw = K % L;
+            if ((rc = o.mul_wrap(Qt, x, L, T))) return rc;
+            if (o.size(T) <= Kw) {                                  /* T < B^Kw: U = B^Kw - T, small */
+                if ((rc = o.pow_sub(Kw, T, E))) return rc;
+                neg = false;
+            } else {                                                /* V = T - B^Kw = -U */
+                bool ge;
+                N V;
+                if ((rc = o.sub_pow(V, T, Kw, &ge))) return rc;
+                if (!ge) return -1;
+                if (o.size(V) < L) { E = V; neg = true; }           /* E = -V */
+                else {                                              /* E = B^L - 1 - V */
+                    if ((rc = o.pow_sub(L, V, E)) || (rc = o.sub_one(E))) return rc;
+                    neg = false;
+                }
+            }
+            if (o.size(E) > t + 1) return -1;
         } else {
-            if ((rc = o.pow_sub(K, T, E))) return rc;
-            neg = false;
+            if ((rc = o.mul(Qt, x, T))) return rc;
+            if (o.size(T) > K) {                                    /* T >= B^K: E = T - B^K, T < 2 B^K */
+                if (o.size(T) != K + 1 || o.limb(T, K) != 1) return -1;
+                if ((rc = o.low(T, K, E))) return rc;
+                neg = true;
+            } else {
+                if ((rc = o.pow_sub(K, T, E))) return rc;
+                neg = false;
+            }
         }
         T = N();
         if ((rc = o.mul(x, E, D))) return rc;

@@ -32,6 +32,8 @@ int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, 
                     uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s);
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
 int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
+int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s);
+size_t nttm_wrap_len(size_t need);
 uint64_t nttm_crt_blocks(uint64_t nc, int per = 8);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
 
@@ -311,6 +313,31 @@ struct DevOps {
         return ge ? 0 : nttm_fail(NTTM_ECHECK, "e pipeline: B^%zu below its subtrahend", K);
     }
     int add_one(DNum &a) { DNum r; RCK(addsub_raw(r, &a, 0, -1, nullptr, 0, 0, false, nullptr)); a = r; return 0; }
+    int sub_pow(DNum &r, const DNum &a, size_t K, bool *ge) { return addsub_raw(r, &a, 0, -1, nullptr, 0, (int64_t)K, true, ge); }
+    /* the wrapped product: r = a b mod (B^L - 1) (B^L - 1 itself may stand for 0), through the cyclic convolution of
+     * length L and a fold of its top limbs onto the bottom; used from h = wrap_h limbs of reciprocal up (below, its
+     * extra additions cost more than the shorter transform saves; NTTM_E_NO_WRAP=1: never) */
+    size_t wrap_h = (size_t)1 << 15;
+    size_t wrap_min() const { return wrap_h; }
+    size_t wrap_len(size_t need) const { return nttm_wrap_len(need); }
+    int mul_wrap(const DNum &a, const DNum &b, size_t L, DNum &r)
+    {
+        if (!a.n || !b.n) { r = DNum(); return 0; }
+        DNum v, lo, hi, t;
+        RCK(alloc(L + 3, v));
+        RCK(nttm_mul_cyclic(v.wptr(), a.ptr(), a.n, b.ptr(), b.n, L, s));
+        RCK(norm(v));
+        RCK(low(v, L, lo));
+        hi = shr(v, L);
+        RCK(addsub(t, lo, 0, hi, 0, false, nullptr));             /* v mod B^L + v div B^L < B^L + B^3 */
+        if (t.n > L) {                                            /* B^L = 1 (mod B^L - 1) */
+            RCK(low(t, L, lo));
+            RCK(add_one(lo));
+            t = lo;
+        }
+        r = t;
+        return 0;
+    }
     int sub_one(DNum &a)
     {
         bool ge; DNum r;
@@ -439,6 +466,8 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
      * NTTM_E_ARENA=0 allocates per number */
     const char *ea = getenv("NTTM_E_ARENA");
     RCK(o.init((hipStream_t)stream, ea && ea[0] == '0' ? 0 : 32 * (m + 1024)));
+    const char *nw = getenv("NTTM_E_NO_WRAP");
+    if (nw && nw[0] == '1') o.wrap_h = SIZE_MAX;
     DNum P, Q, A, x, Y;
     RCK(e_series(o, N, P, Q));
     bool ge;

@@ -45,13 +45,19 @@ extern "C" {
 long eb_t_products(void) { return g_products; }
 size_t eb_t_max_operand(void) { return g_max_operand; }
 void eb_t_reset(void) { g_products = 0; g_max_operand = 0; }
-/* x ~ B^(n+m) / Q; returns the limbs of x written, or -1 */
-long eb_t_recip(const uint64_t *q, size_t n, size_t m, uint64_t *x, size_t cap)
+static long g_wraps;
+long eb_t_wraps(void) { return g_wraps; }
+/* x ~ B^(n+m) / Q; returns the limbs of x written, or -1.  wrap: the Newton product modulo B^L - 1 wherever it is
+ * shorter (0: always the whole product) */
+long eb_t_recip(const uint64_t *q, size_t n, size_t m, uint64_t *x, size_t cap, int wrap)
 {
     SchoolMul mul;
     HostOps<SchoolMul> o(mul);
+    if (!wrap) o.wrap_h = (size_t)-1;
     Big X;
-    if (eb_recip(o, load(q, n), m, X)) return -1;
+    const int rc = eb_recip(o, load(q, n), m, X);
+    g_wraps = o.wraps;
+    if (rc) return -1;
     return (long)store(X, x, cap);
 }
 /* floor(A B^m / Q) through eb_recip (one guard limb) and eb_quotient (full: always through the remainder); returns

@@ -27,7 +27,8 @@ def E():
                        check=True)
     L = C.CDLL(SO)
     sz = C.c_size_t
-    L.eb_t_recip.argtypes = [U64P, sz, sz, U64P, sz]; L.eb_t_recip.restype = C.c_long
+    L.eb_t_recip.argtypes = [U64P, sz, sz, U64P, sz, C.c_int]; L.eb_t_recip.restype = C.c_long
+    L.eb_t_wraps.restype = C.c_long
     L.eb_t_quotient.argtypes = [U64P, sz, U64P, sz, sz, U64P, sz, C.c_int]; L.eb_t_quotient.restype = C.c_long
     L.eb_t_div_pow.argtypes = [sz, U64P, sz, U64P, sz]; L.eb_t_div_pow.restype = C.c_long
     L.eb_t_products.restype = C.c_long
@@ -52,11 +53,11 @@ def value(buf, n):
     return sum(int(buf[i]) * B ** i for i in range(n))
 
 
-def recip(E, q, m):
+def recip(E, q, m, wrap=1):
     qa, n = arr(q)
     cap = m + 4
     out = (C.c_uint64 * cap)()
-    k = E.eb_t_recip(qa, n, m, out, cap)
+    k = E.eb_t_recip(qa, n, m, out, cap, wrap)
     assert k >= 0
     return value(out, k)
 
@@ -136,6 +137,19 @@ def test_quotient_one_product(E):
         q = rnd.randrange(B ** (n - 1), B ** n)
         for _ in range(8):
             assert quotient_products(E, rnd.randrange(q), q, m) == 1, (n, m)
+
+
+@pytest.mark.parametrize("n,m", [(3, 40), (40, 70), (90, 33), (7, 200), (300, 250), (600, 1000)])
+def test_recip_wrapped_product(E, n, m):
+    """the Newton step's product formed modulo B^L - 1 (the wrap-around) gives the same reciprocal, limb for limb, as
+    the whole product, and the wrapped path is taken"""
+    rnd = random.Random(n * 7 + m)
+    for q in q_kinds(rnd, n):
+        x0 = recip(E, q, m, wrap=0)
+        x1 = recip(E, q, m, wrap=1)
+        assert E.eb_t_wraps() > 0
+        assert x0 == x1, (n, m)
+        assert abs(x1 - B ** (n + m) // q) <= 3
 
 
 def test_reciprocal_products_on_truncated_q(E):

@@ -54,6 +54,63 @@ def test_e_fraction_full_check(nttm, m):
         assert sum(int(v) * B ** i for i, v in enumerate(full)) == e_fraction_python(m)
 
 
+def limbs_int(v):
+    """the integer of base-10^18 limbs (least significant first), by halves"""
+    v = [int(t) for t in v]
+
+    def rec(lo, hi):
+        if hi - lo <= 64:
+            r = 0
+            for t in reversed(v[lo:hi]):
+                r = r * B + t
+            return r
+        mid = (lo + hi) // 2
+        return rec(lo, mid) + rec(mid, hi) * B ** (mid - lo)
+    return rec(0, len(v)) if v else 0
+
+
+@pytest.mark.parametrize("L,na,nb", [(16, 16, 16), (48, 48, 17), (1024, 1024, 1024), (1024, 700, 1000),
+                                      (3 * 4096, 3 * 4096, 9000), (1 << 16, 1 << 16, 40000), (3 << 14, 20000, 3 << 14)])
+def test_mul_cyclic(nttm, orc, L, na, nb):
+    """the cyclic product of length L (2^k and 3 2^k, operands up to L limbs, so the convolution wraps): congruent to
+    a b modulo B^L - 1, lowest limb c_0 = a_0 b_0 + sum a_i b_(L-i) mod B, at most L + 3 limbs; every coefficient
+    against the cyclic convolution in exact integers up to L = 1024"""
+    import numpy as np
+    from gpu_util import dev, host
+    a, s = orc.gen_limbs(na, 5 * L + na)
+    b, s = orc.gen_limbs(nb, s)
+    say("cyclic product", L, na, nb)
+    v = host(nttm.mul_cyclic(dev(a), dev(b), L))
+    ai = [int(t) for t in a] + [0] * (L - na)
+    bi = [int(t) for t in b] + [0] * (L - nb)
+    V, A, Bv = limbs_int(v), limbs_int(ai), limbs_int(bi)
+    M = B ** L - 1
+    assert V % M == (A * Bv) % M
+    c0 = ai[0] * bi[0] + sum(ai[i] * bi[L - i] for i in range(1, L))
+    assert int(v[0]) == c0 % B
+    assert all(int(t) < B for t in v)
+    if L <= 1024:
+        c = [sum(ai[i] * bi[(k - i) % L] for i in range(L)) for k in range(L)]
+        assert V == sum(ck * B ** k for k, ck in enumerate(c))
+
+
+def test_e_fraction_wrapped_newton(nttm):
+    """the Newton reciprocal with its product formed modulo B^L - 1 from 2^15 limbs up (the default) gives the same
+    limbs as with whole products (NTTM_E_NO_WRAP=1); 5.4 million digits, so three levels wrap"""
+    import numpy as np
+    m = 300000
+    say("e wrapped newton", m)
+    wrapped = nttm.e_fraction(m)
+    os.environ["NTTM_E_NO_WRAP"] = "1"
+    try:
+        whole = nttm.e_fraction(m)
+    finally:
+        os.environ.pop("NTTM_E_NO_WRAP", None)
+    assert np.array_equal(wrapped, whole)
+    d = nttm.limbs_to_ascii(wrapped)
+    assert d[49:99].decode() == WIN["50"] and d[999999:1000049].decode() == WIN["1000000"]
+
+
 def test_e_million_digits_windows(nttm):
     m = 55600                                                    # 1,000,800 digits
     say("e 1e6 digits")
