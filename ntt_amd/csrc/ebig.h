@@ -12,10 +12,10 @@
  * Reciprocal (the reference's Newton correction form, E/ALGORITHM.md:197-246, E/newton_db.c:218): x_l approximates
  * B^(n+l) / Q (Q of n limbs) to within 3 units.  From x_h to x_l with l <= 2h - 2, using only the top t = l + 2 limbs
  * Q_t of Q:  E = B^(t+h) - Q_t x_h,  x_l = x_h B^(l-h) + floor(x_h E / B^(t+2h-l)).
- * The relative error after the step is eps_h^2 + B^(-l-1) (the truncation of Q) + one unit of rounding; with
- * eps_h <= 3 B^-h and l <= 2h - 2 the squared term is below 9/B units, so the error stays within 3 units at every
- * level.  Each step costs two products of about 1.5 l limbs; the levels halve down to a base of at most 8 limbs, which
- * is exact long division.  Q_t x_h is B^K - E with |E| <= B^(h+1) + 3 B^t <= 4 B^t, so above wrap_min() it is formed
+ * The relative error after the step is eps_h^2 + B^(-l-1) (the truncation of Q) + one unit of rounding + 1/B of a
+ * unit (x_h E formed from E's top limbs only, below); with eps_h <= 3 B^-h and l <= 2h - 2 the squared term is below
+ * 9/B units, so the error stays within 3 units at every level.  Each step costs two products, of about 1.5 l and
+ * l limbs; the levels halve down to a base of at most 8 limbs, which is exact long division.  Q_t x_h is B^K - E with |E| <= B^(h+1) + 3 B^t <= 4 B^t, so above wrap_min() it is formed
  * modulo B^L - 1 only (L >= t + 2, about 2/3 of the whole product's length), and E read off the residue: the
  * wrap-around of the Newton product (as GMP's mpn_invertappr does with mpn_mulmod_bnm1).
  * Quotient, from x = x_(m+1) (one guard limb): for A < Q, A B^m / Q = (T + A d) / B^(n+1) with T = A x and |d| <= 3,
@@ -221,10 +221,13 @@ template <class O> int eb_recip(O &o, const typename O::Num &Q, size_t m, typena
             }
         }
         T = N();
-        if ((rc = o.mul(x, E, D))) return rc;
+        /* only floor(x E / B^sh) is used: dropping E's low k = sh - h - 2 limbs moves x E by less than
+         * x B^k < (B^(h+1) + 3) B^k, under 1/B of a unit after the shift, so the product takes E's top limbs */
+        const size_t sh = t + 2 * h - l, k = sh > h + 2 ? sh - h - 2 : 0;
+        if ((rc = o.mul(x, o.shr(E, k), D))) return rc;
         E = N();
         bool ge = true;
-        if ((rc = o.addsub(xn, x, l - h, o.shr(D, t + 2 * h - l), 0, neg, &ge))) return rc;   /* x B^(l-h) +- step */
+        if ((rc = o.addsub(xn, x, l - h, o.shr(D, sh - k), 0, neg, &ge))) return rc;   /* x B^(l-h) +- step */
         if (!ge) return -1;
         x = xn;
         h = l;
