@@ -927,6 +927,20 @@ static int mul_full(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     return check_products(&q, 1, s);
 }
 
+/* c = a b on device memory without a synchronization where it fits one transform (no residue check configured): the
+ * launches only, the CRT's flag word copied to the pinned word *flag (bit 0 set: a carry beyond the top limb), which
+ * the caller reads after synchronizing the stream; *flag = null when the product ran synchronously (a grid) */
+int nttm_mul_launch(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, hipStream_t s,
+                    const uint32_t **flag)
+{
+    *flag = nullptr;
+    if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
+    if (!na || !nb || g.cfg.check || !fits_one(na, nb, na + nb)) return mul_full(c, a, na, b, nb, nullptr, 0, s);
+    RCK(mul_core(c, a, na, b, nb, nullptr, 0, s, SIZE_MAX, nullptr, 0, 0, 0));
+    *flag = (const uint32_t *)((char *)g.hpin + 96) + 1;
+    return 0;
+}
+
 extern "C" int nttm_last_phase_ms(double *f, double *i, double *c)
 {
     if (f) *f = g.last_ms[0];

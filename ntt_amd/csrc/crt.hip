@@ -585,35 +585,39 @@ int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, 
 
 /* the normalized length of x[0 .. n): one more than the index of the highest nonzero limb */
 /* one more than the index of the top non-zero limb of x[0 .. n) (0 if none), by atomicMax into *out (cleared first).
- * Block b scans the chunk [n - (b + 1) C, n - b C), so the chunks at the top are dispatched first; a block returns at once
- * when *out already reaches its chunk's top (nothing it could find is larger), so a number with few leading zero limbs
- * costs about one wave of blocks instead of a pass over all of x */
+ * Chunk c covers [n - (c + 1) C, n - c C): the chunks are taken from the top down, block b taking b, b + G, b + 2G, ...
+ * (G blocks, at most one per CU), and a block stops as soon as *out reaches the top of its next chunk (nothing below
+ * can raise the maximum).  A number with a few leading zero limbs costs one chunk per block; one with many streams
+ * its zero top at full bandwidth until the first non-zero chunk. */
 #define TOPNZ_PER 16
 #define TOPNZ_CHUNK (256 * TOPNZ_PER)
+#define TOPNZ_GRID 256
 __global__ __launch_bounds__(256) void k_top_nonzero(const uint64_t *x, uint64_t n, unsigned long long *out)
 {
-    const uint64_t hi = n - (uint64_t)blockIdx.x * TOPNZ_CHUNK;    /* the grid keeps blockIdx.x * TOPNZ_CHUNK < n */
-    const uint64_t lo = hi > TOPNZ_CHUNK ? hi - TOPNZ_CHUNK : 0;
-    if (__hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi) return;
-    unsigned long long best = 0;
+    const uint64_t nchunk = (n + TOPNZ_CHUNK - 1) / TOPNZ_CHUNK;
+    for (uint64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const uint64_t hi = n - c * TOPNZ_CHUNK, lo = hi > TOPNZ_CHUNK ? hi - TOPNZ_CHUNK : 0;
+        if (__hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi) return;
+        unsigned long long best = 0;
 #pragma unroll
-    for (int j = 0; j < TOPNZ_PER; j++) {
-        const uint64_t k = lo + (uint64_t)j * 256 + threadIdx.x;
-        if (k < hi && x[k]) best = k + 1;
+        for (int j = 0; j < TOPNZ_PER; j++) {
+            const uint64_t k = lo + (uint64_t)j * 256 + threadIdx.x;
+            if (k < hi && x[k]) best = k + 1;
+        }
+        for (int d = 32; d; d >>= 1) {
+            const unsigned long long o = __shfl_down(best, d, 64);
+            best = o > best ? o : best;
+        }
+        if ((threadIdx.x & 63) == 0 && best) atomicMax(out, best);
     }
-    for (int d = 32; d; d >>= 1) {
-        const unsigned long long o = __shfl_down(best, d, 64);
-        best = o > best ? o : best;
-    }
-    if ((threadIdx.x & 63) == 0 && best) atomicMax(out, best);
 }
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s)
 {
     int rc = nttm_hip_check(hipMemsetAsync(out_dev, 0, 8, s), "clear length");
     if (rc || !n) return rc;
-    const uint64_t blocks = (n + TOPNZ_CHUNK - 1) / TOPNZ_CHUNK;
-    if (blocks > 0x7fffffffULL) return nttm_fail(NTTM_EINVAL, "top limb: %llu limbs", (unsigned long long)n);
-    hipLaunchKernelGGL(k_top_nonzero, dim3((unsigned)blocks), dim3(256), 0, s, x, n, (unsigned long long *)out_dev);
+    const uint64_t chunks = (n + TOPNZ_CHUNK - 1) / TOPNZ_CHUNK;
+    const unsigned blocks = (unsigned)(chunks < TOPNZ_GRID ? chunks : TOPNZ_GRID);
+    hipLaunchKernelGGL(k_top_nonzero, dim3(blocks), dim3(256), 0, s, x, n, (unsigned long long *)out_dev);
     return nttm_hip_check(hipGetLastError(), "k_top_nonzero launch");
 }
 /* the normalized length of x[0 .. n) into *len_dev (device pointers), for tests */

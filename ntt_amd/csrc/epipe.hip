@@ -33,6 +33,8 @@ int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, 
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
 int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
 int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s);
+int nttm_mul_launch(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, hipStream_t s,
+                    const uint32_t **flag);
 size_t nttm_wrap_len(size_t need);
 uint64_t nttm_crt_blocks(uint64_t nc, int per = 8);
 uint64_t nttm_crt_scratch_words(uint64_t nblocks);
@@ -237,14 +239,12 @@ struct DevOps {
         memcpy(v, hword, bytes);
         return 0;
     }
-    /* drop leading zero limbs: the top limb first, a device scan when it is zero */
+    /* drop leading zero limbs: the device scan from the top (a few limbs' work when there are few) and one read */
     int norm(DNum &a)
     {
         if (!a.n) { a = DNum(); return 0; }
         uint64_t v;
-        RCK(read_word(a.ptr() + a.n - 1, 8, &v));
-        if (v) return 0;
-        RCK(nttm_run_top_nonzero(a.ptr(), a.n - 1, dword, s));
+        RCK(nttm_run_top_nonzero(a.ptr(), a.n, dword, s));
         RCK(read_word(dword, 8, &v));
         a.n = v;
         if (!a.n) a = DNum();
@@ -261,15 +261,19 @@ struct DevOps {
         return v;
     }
     int low(const DNum &a, size_t k, DNum &r) { r = a; r.n = std::min(k, a.n); return norm(r); }
+    /* the product and its normalized length (normalized operands: at most one zero limb on top) behind one
+     * synchronization */
     int mul(const DNum &a, const DNum &b, DNum &c)
     {
         if (!a.n || !b.n) { c = DNum(); return 0; }
         DNum t;
         RCK(alloc(a.n + b.n, t));
-        RCK(nttm_mul(t.wptr(), a.ptr(), a.n, b.ptr(), b.n, s));
+        const uint32_t *flag = nullptr;
+        RCK(nttm_mul_launch(t.wptr(), a.ptr(), a.n, b.ptr(), b.n, s, &flag));
         uint64_t v;
         RCK(read_word(t.ptr() + t.n - 1, 8, &v));
-        if (!v) t.n--;                                             /* normalized operands: at most one zero limb on top */
+        if (flag && (*flag & 1)) return nttm_fail(NTTM_ECHECK, "e pipeline: crt: non-zero carry beyond the top limb");
+        if (!v) t.n--;
         c = t;
         return 0;
     }
@@ -291,14 +295,18 @@ struct DevOps {
         RCK(alloc(nout, t));
         RCK(nttm_run_addsub(t.wptr(), nout, a ? a->ptr() : nullptr, a ? a->n : 0, sa, apow, b ? b->ptr() : nullptr,
                             b ? b->n : 0, sb, bpow, sub ? 1 : 0, scratch, s));
+        /* the normalized length and (a subtraction) the final carry out, ctl[1], behind one synchronization */
+        RCK(nttm_run_top_nonzero(t.ptr(), nout, dword, s));
+        RCK(nttm_hip_check(hipMemcpyAsync(hword, dword, 8, hipMemcpyDeviceToHost, s), "read back"));
+        if (sub) RCK(nttm_hip_check(hipMemcpyAsync(hword + 1, scratch, 8, hipMemcpyDeviceToHost, s), "read back"));
+        RCK(nttm_hip_check(hipStreamSynchronize(s), "read back"));
         if (sub) {
-            uint64_t v;
-            RCK(read_word(scratch, 8, &v));                        /* ctl[1]: the final carry out */
-            const bool g = (v >> 32) != 0;
+            const bool g = (hword[1] >> 32) != 0;
             if (ge) *ge = g;
             if (!g) { r = DNum(); return 0; }
         }
-        RCK(norm(t));
+        t.n = hword[0];
+        if (!t.n) t = DNum();
         r = t;
         return 0;
     }
