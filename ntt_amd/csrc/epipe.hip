@@ -8,13 +8,15 @@
  *      into one leaf per thread (P <- P f + 1, Q <- Q f for f = k + 1, base-10^9 digits in registers); nttm_bs_tree
  *      combines the leaves into P = sum_{k=1..N} N!/k!, Q = N!;
  *   2. e - 2 = P/Q - 1 + tail, so the fraction is A / Q with A = P - Q < Q;
- *   3. x ~ B^(n+m) / Q by Newton's iteration with doubling precision (ebig.h), n = the limbs of Q;
- *   4. Y = floor(A x / B^n), corrected by the remainder A B^m - Y Q to floor(A B^m / Q).
+ *   3. x ~ B^(n+m+1) / Q by Newton's iteration with doubling precision (ebig.h), n = the limbs of Q, one guard limb;
+ *   4. Y = floor(A x / B^(n+1)), exact unless limb n of A x lies within 3 of a limb boundary (then corrected by the
+ *      remainder A B^m - Y Q) = floor(A B^m / Q).
  * The series tail after N terms is below 2/(N+1)! < B^-(m+2), so Y is e's fraction to m limbs unless the m+1-th and
  * m+2-th limbs of the exact fraction are all B - 1 (which the caller's check windows would catch).
- * DevOps is ebig.h's backend on the device: products by nttm_mul, additions and subtractions by k_addsub with the
- * CRT's block carry maps (crt.hip), normalized lengths read back as one word; only the few-limb base of the Newton
- * iteration and the final fraction cross to the host. */
+ * DevOps is ebig.h's backend on the device: products by the library's product (wrapped products by its cyclic
+ * convolution), additions and subtractions by k_addsub with the CRT's block carry maps (crt.hip), each result's
+ * normalized length (and carry) read back behind one synchronization; only the few-limb base of the Newton iteration
+ * and the final fraction cross to the host. */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -91,6 +93,8 @@ __global__ __launch_bounds__(256) void k_e_seed(uint64_t nterms, uint32_t G, uin
  * for w <= 256; it splits into three base-10^18 digits (div_b), and limb k = d0[k] + d1[k-1] + d2[k-2] + carry (one
  * lane, sequentially; the carry is at most 2). */
 #define BS_MAXW 256
+#define E_SCHOOL_W 128            /* the widest schoolbook input of the e pipeline's leaves: the 247 -> 495-limb level runs
+                                   * as 2^9-point transforms (k_fused2<8, 9, 3, true>), 5 ms faster at 10^9 digits */
 __device__ __forceinline__ void split3(unsigned __int128 v, uint64_t *d0, uint64_t *d1, uint64_t *d2)
 {
     const uint64_t hi = (uint64_t)(v >> 64), lo = (uint64_t)v;
@@ -394,51 +398,86 @@ static int seed_shape(size_t N, uint32_t *G, uint32_t *wl)
     return nttm_fail(NTTM_EBOUND, "e pipeline: no leaf shape for %zu terms", N);
 }
 
-/* P, Q of the N-term series, in device memory */
+/* upper bounds of the leaves' normalized lengths from the series itself, so that the tree is planned on the host while
+ * the device still forms the leaves: leaf i covers the terms [k0, k1) = [i s, min((i + 1) s, N)), Q_i = k1! / k0!, and
+ * P_i / Q_i = sum_(j < k1) 1 / ((k0 + 1) ... (j + 1)) < 1 / k0 (< 2 for k0 = 0).  With L = log10 Q_i from the
+ * difference of FP64 lgamma values (a few ulp each: below 10^-6 absolute up to 10^9 terms, under 10^-7 limbs),
+ * |Q_i| <= floor(L / 18 + 10^-5) + 1 and |P_i| <= floor((L - log10 k0) / 18 + 10^-5) + 1 (L + log10 2 for k0 = 0):
+ * at most one limb above the true length, clamped to the slot width w (every slot holds zeros above its number) */
+static void leaf_bounds(uint64_t N, uint64_t span, uint64_t nodes, uint32_t w, std::vector<uint32_t> &hl)
+{
+    hl.assign(2 * nodes, 0);
+    const double ln10 = log(10.0);
+    double lg0 = 0.0;                                              /* lgamma(k0 + 1), k0 = 0 */
+    for (uint64_t i = 0; i < nodes; i++) {
+        const uint64_t k0 = i * span, k1 = std::min<uint64_t>(k0 + span, N);
+        const double lg1 = lgamma((double)k1 + 1.0);
+        const double L = (lg1 - lg0) / ln10, Lp = k0 ? L - log10((double)k0) : L + log10(2.0);
+        const uint64_t qn = (uint64_t)floor(L / 18.0 + 1e-5) + 1;
+        const uint64_t pn = Lp > 0 ? (uint64_t)floor(Lp / 18.0 + 1e-5) + 1 : 1;
+        hl[2 * i] = (uint32_t)std::min<uint64_t>(pn, w);
+        hl[2 * i + 1] = (uint32_t)std::min<uint64_t>(qn, w);
+        lg0 = lg1;
+    }
+}
+
+/* P, Q of the N-term series, in device memory.  The leaves (k_e_seed and the schoolbook levels) are launched without
+ * a synchronization, the tree planned from the leaves' length bounds meanwhile and queued behind them; leaf and tree
+ * times come from events around the leaves. */
 static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
 {
     hipStream_t s = o.s;
     uint32_t G, wl;
     RCK(seed_shape(N, &G, &wl));
     const uint64_t nleaf = (N + G - 1) / G;
-    DNum pq;
-    uint32_t *lens = nullptr;
+    const char *sw = getenv("NTTM_E_SCHOOL_W");                  /* the widest schoolbook input (for comparisons) */
+    const uint32_t school_w = sw && atoi(sw) > 0 && atoi(sw) <= BS_MAXW ? (uint32_t)atoi(sw) : E_SCHOOL_W;
+    const char *lc = getenv("NTTM_E_LEAF_CHECK");                /* 1: check the length bounds against the leaves */
+    const bool leaf_check = lc && lc[0] == '1';
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; k++) RCK(nttm_hip_check(hipEventCreate(&ev[k]), "event"));
+    struct EvFree { hipEvent_t *e; ~EvFree() { for (int k = 0; k < 2; k++) if (e[k]) (void)hipEventDestroy(e[k]); } } evf{ev};
     auto t0 = eclk::now();
+    DNum pq, lw;
     RCK(o.alloc(nleaf * 2 * wl, pq));
-    if (hipMalloc(&lens, nleaf * 2 * 4) != hipSuccess) {
-        (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "e pipeline: %llu leaves", (unsigned long long)nleaf);
-    }
+    RCK(o.alloc(nleaf + 1, lw));                                   /* the seed's lengths, 2 nleaf words */
+    uint32_t *lens = (uint32_t *)lw.wptr();
+    RCK(nttm_hip_check(hipEventRecord(ev[0], s), "event"));
     hipLaunchKernelGGL(k_e_seed, dim3((unsigned)((nleaf + 255) / 256)), dim3(256), 0, s, (uint64_t)N, G, nleaf, wl, pq.wptr(), lens);
-    int rc = nttm_hip_check(hipGetLastError(), "k_e_seed launch");
-    /* the lowest tree levels by schoolbook products while the nodes fit (30 -> 61 -> 123 -> 247 -> 495 limbs) */
-    uint64_t nodes = nleaf;
+    RCK(nttm_hip_check(hipGetLastError(), "k_e_seed launch"));
+    /* the lowest tree levels by schoolbook products up to inputs of school_w limbs (30 -> 61 -> 123 -> 247 limbs) */
+    uint64_t nodes = nleaf, span = G;
     uint32_t w = wl;
-    while (!rc && nodes > 1 && w <= BS_MAXW) {
+    while (nodes > 1 && w <= school_w) {
         const uint64_t nout = (nodes + 1) / 2;
         const uint32_t wo = 2 * w + 1;
-        DNum next;
-        uint32_t *lens2 = nullptr;
-        if ((rc = o.alloc(nout * 2 * wo, next))) break;
-        if (hipMalloc(&lens2, nout * 2 * 4) != hipSuccess) { (void)hipGetLastError(); rc = nttm_fail(NTTM_ENOMEM, "e pipeline: lengths"); break; }
+        DNum next, lw2;
+        RCK(o.alloc(nout * 2 * wo, next));
+        RCK(o.alloc(nout + 1, lw2));
+        uint32_t *lens2 = (uint32_t *)lw2.wptr();
         if (w <= 64)
             hipLaunchKernelGGL(k_bs_school<64>, dim3((unsigned)nout), dim3(64), 0, s, pq.ptr(), w, lens, nodes, next.wptr(), wo, lens2);
         else
             hipLaunchKernelGGL(k_bs_school<BS_MAXW>, dim3((unsigned)nout), dim3(64), 0, s, pq.ptr(), w, lens, nodes, next.wptr(), wo, lens2);
-        rc = nttm_hip_check(hipGetLastError(), "k_bs_school launch");
-        if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "k_bs_school");
-        (void)hipFree(lens);
+        RCK(nttm_hip_check(hipGetLastError(), "k_bs_school launch"));
         lens = lens2;
         pq = next;
         nodes = nout;
         w = wo;
+        span *= 2;
     }
-    std::vector<uint32_t> hl(2 * nodes);
-    if (!rc) rc = nttm_hip_check(hipMemcpyAsync(hl.data(), lens, nodes * 2 * 4, hipMemcpyDeviceToHost, s), "copy leaf lengths");
-    if (!rc) rc = nttm_hip_check(hipStreamSynchronize(s), "e leaves");
-    (void)hipFree(lens);
-    if (rc) return rc;
-    g_e_ms[0] = ms_since(t0);
-    t0 = eclk::now();
+    RCK(nttm_hip_check(hipEventRecord(ev[1], s), "event"));
+    std::vector<uint32_t> hl;
+    leaf_bounds(N, span, nodes, w, hl);
+    if (leaf_check) {
+        std::vector<uint32_t> tl(2 * nodes);
+        RCK(nttm_hip_check(hipMemcpyAsync(tl.data(), lens, nodes * 2 * 4, hipMemcpyDeviceToHost, s), "copy leaf lengths"));
+        RCK(nttm_hip_check(hipStreamSynchronize(s), "e leaves"));
+        for (uint64_t i = 0; i < 2 * nodes; i++)
+            if (hl[i] < tl[i] || hl[i] > tl[i] + 1)
+                return nttm_fail(NTTM_ECHECK, "e pipeline: leaf %llu: length bound %u against %u limbs",
+                                 (unsigned long long)(i / 2), hl[i], tl[i]);
+    }
     size_t cap = 2;
     std::vector<nttm_bs_node> leaves(nodes);
     for (uint64_t i = 0; i < nodes; i++) {
@@ -446,16 +485,19 @@ static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
         leaves[i].q = pq.ptr() + 2 * (uint64_t)w * i + w; leaves[i].qn = hl[2 * i + 1];
         cap += hl[2 * i + 1];                                      /* |Q| <= sum of the leaves' |Q|; P < 2 Q */
     }
-    const uint64_t nleaves = nodes;
     /* P's buffer also holds the root's unnormalized length (|P_0| <= |Q_0| + 1, one limb per level), so the tree
      * writes the root in place */
     const size_t pcap = cap + 72;
     RCK(o.alloc(pcap, P));
     RCK(o.alloc(cap, Q));
     size_t pn = 0, qn = 0;
-    RCK(nttm_bs_tree(leaves.data(), nleaves, P.wptr(), pcap, &pn, Q.wptr(), cap, &qn, s));
+    RCK(nttm_bs_tree(leaves.data(), nodes, P.wptr(), pcap, &pn, Q.wptr(), cap, &qn, s));   /* synchronizes */
     P.n = pn; Q.n = qn;                                            /* normalized by the tree */
-    g_e_ms[1] = ms_since(t0);
+    float leaf_ms = 0;
+    RCK(nttm_hip_check(hipEventElapsedTime(&leaf_ms, ev[0], ev[1]), "event"));
+    const double total = ms_since(t0);
+    g_e_ms[0] = leaf_ms;
+    g_e_ms[1] = total - leaf_ms;
     return 0;
 }
 

@@ -1142,15 +1142,18 @@ int nttm_run_fused2(const PassArgs &a, const PassArgs &bq, const PassGeom &g, hi
     const unsigned nthr = (unsigned)(T / ppt < 64 ? 64 : T / ppt);
     if (nthr > 512) return nttm_fail(NTTM_EINVAL, "internal: %u threads per tile", nthr);
     /* the fixed shapes: 4096-point tiles of 2^12-point blocks, or of 4 x 2^10 and 2 x 2^11 (the last pass of a
-     * transform of more than one pass, or a whole transform of 2^10 to 2^12 points, which reads the limbs) */
-    const bool fixed_shape = g_fixed_shapes && g.m + g.logh == FUSED_LOG && g.m >= 10;
-    const bool fixed = fixed_shape && !alimb && !blimb;
+     * transform of more than one pass, or a whole transform of 2^10 to 2^12 points, which reads the limbs); whole
+     * transforms of 2^8 and 2^9 points from the limbs in tiles of 16 and 8 blocks (8 points per thread) */
+    const bool fixed_shape = g_fixed_shapes && g.m + g.logh == FUSED_LOG && (g.m >= 10 || (g.m >= 8 && ppt == 8));
+    const bool fixed = fixed_shape && !alimb && !blimb && g.m >= 10;
     const dim3 grid((unsigned)tiles), blk(nthr);
-    if (fixed_shape && !fixed) {                                /* a single-pass transform: both operands from limbs */
+    if (fixed_shape && !fixed && (alimb || blimb)) {            /* a single-pass transform: both operands from limbs */
         if (ppt == 8) {
             if (g.m == 12) hipLaunchKernelGGL((k_fused2<8, 12, 0, true>), grid, blk, 0, s, A, Bq);
             else if (g.m == 11) hipLaunchKernelGGL((k_fused2<8, 11, 1, true>), grid, blk, 0, s, A, Bq);
-            else hipLaunchKernelGGL((k_fused2<8, 10, 2, true>), grid, blk, 0, s, A, Bq);
+            else if (g.m == 10) hipLaunchKernelGGL((k_fused2<8, 10, 2, true>), grid, blk, 0, s, A, Bq);
+            else if (g.m == 9) hipLaunchKernelGGL((k_fused2<8, 9, 3, true>), grid, blk, 0, s, A, Bq);
+            else hipLaunchKernelGGL((k_fused2<8, 8, 4, true>), grid, blk, 0, s, A, Bq);
         } else {
             if (g.m == 12) hipLaunchKernelGGL((k_fused2<PASS_PTS, 12, 0, true>), grid, blk, 0, s, A, Bq);
             else if (g.m == 11) hipLaunchKernelGGL((k_fused2<PASS_PTS, 11, 1, true>), grid, blk, 0, s, A, Bq);

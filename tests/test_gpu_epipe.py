@@ -111,6 +111,24 @@ def test_e_fraction_wrapped_newton(nttm):
     assert d[49:99].decode() == WIN["50"] and d[999999:1000049].decode() == WIN["1000000"]
 
 
+@pytest.mark.parametrize("m", [1, 7, 300, 55600, 5555560])
+def test_e_leaf_length_bounds(nttm, m):
+    """the tree is planned from upper bounds of the leaves' lengths computed from the series (lgamma), not read back:
+    NTTM_E_LEAF_CHECK=1 reads the schoolbook levels' true lengths and fails unless every bound lies within one limb
+    above them; the digits stay exact"""
+    say("e leaf bounds", m)
+    os.environ["NTTM_E_LEAF_CHECK"] = "1"
+    try:
+        got = nttm.e_fraction(m)
+    finally:
+        os.environ.pop("NTTM_E_LEAF_CHECK", None)
+    if m <= 300:
+        assert sum(int(v) * B ** i for i, v in enumerate(got)) == e_fraction_python(m)
+    else:
+        d = nttm.limbs_to_ascii(got)
+        assert d[49:99].decode() == WIN["50"] and d[999999:1000049].decode() == WIN["1000000"]
+
+
 def test_e_million_digits_windows(nttm):
     m = 55600                                                    # 1,000,800 digits
     say("e 1e6 digits")
