@@ -500,9 +500,9 @@ def trace_summary(db):
 
 
 def replay_summary(db, replays=3):
-    """the graph replays of a kernel trace -- the last burst of library kernels (k_*, no gap above 200 us) made of at
-    least `replays` identical runs of one schedule -- summarized per run: kernel time per family, and the span from the
-    first start to the last end (the difference is the idle time between launches)"""
+    """the graph replays of a kernel trace -- the last `replays` runs of the period whose repetitions make the longest
+    tail of the last burst of library kernels (k_*, no gap above 200 us) -- summarized per run: kernel time per family,
+    and the span from the first start to the last end (the difference is the idle time between launches)"""
     import sqlite3
     con = sqlite3.connect(db)
     rows = con.execute("select name, start, end from kernels order by start").fetchall()
@@ -517,12 +517,16 @@ def replay_summary(db, replays=3):
     bursts.append(cur)
     runs = None
     for bt in reversed(bursts):
-        names = [r[0] for r in bt]
-        for P in range(1, len(bt) // replays + 1):       # the shortest period that tiles the whole burst
-            if len(bt) % P == 0 and all(names[i] == names[i % P] for i in range(len(bt))):
-                runs = [bt[len(bt) - (replays - i) * P:len(bt) - (replays - i - 1) * P] for i in range(replays)]
-                break
-        if runs:
+        names, n = [r[0] for r in bt], len(bt)
+        best, best_len = 0, 0
+        for P in range(1, n // replays + 1):            # the period with the longest periodic tail (the shortest on a tie)
+            L = P
+            while L < n and names[n - 1 - L] == names[n - 1 - L + P]:
+                L += 1
+            if L >= replays * P and L > best_len:
+                best, best_len = P, L
+        if best:
+            runs = [bt[n - (replays - i) * best:n - (replays - i - 1) * best] for i in range(replays)]
             break
     if not runs:
         print("no burst of %d repeated runs found" % replays)
