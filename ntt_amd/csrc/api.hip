@@ -105,6 +105,8 @@ struct Ctx {
     char *tdesc; size_t tdesc_bytes;         /* the planned tree's descriptors, every group of every level */
     char *tpin; size_t tpin_bytes;           /* their pinned host image, and the groups' flags */
     hipStream_t tcs;                         /* the tree's descriptor copies (ahead of the launches on the caller's stream) */
+    void *dfree[8]; int ndfree;              /* buffers the planned tree outgrew, freed after its final synchronization */
+    void *hfree[4]; int nhfree;
     hipEvent_t tev[8];                       /* the last copies on tcs, in turn */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     char *stage; size_t stage_bytes;         /* device copies of host-resident operands and results */
@@ -255,12 +257,37 @@ static int build_tables(void)
     return 0;
 }
 
-static int ws_need(size_t bytes)
+/* an outgrown buffer: freed at once after a device synchronization, or (defer: work queued before it may still read
+ * it, the caller synchronizes later) listed for free_deferred, so that growing it does not wait for that work */
+static int drop_dev(void *p, bool defer)
+{
+    if (!p) return 0;
+    if (defer && g.ndfree < 8) { g.dfree[g.ndfree++] = p; return 0; }
+    HIPCK(hipDeviceSynchronize());
+    HIPCK(hipFree(p));
+    return 0;
+}
+static int drop_host(void *p, bool defer)
+{
+    if (!p) return 0;
+    if (defer && g.nhfree < 4) { g.hfree[g.nhfree++] = p; return 0; }
+    HIPCK(hipDeviceSynchronize());
+    HIPCK(hipHostFree(p));
+    return 0;
+}
+/* after the synchronization that ends every use of the listed buffers */
+static void free_deferred(void)
+{
+    for (int i = 0; i < g.ndfree; i++) (void)hipFree(g.dfree[i]);
+    for (int i = 0; i < g.nhfree; i++) (void)hipHostFree(g.hfree[i]);
+    g.ndfree = g.nhfree = 0;
+}
+static int ws_need(size_t bytes, bool defer = false)
 {
     if (g.cfg.workspace_bytes && bytes > g.cfg.workspace_bytes)
         return nttm_fail(NTTM_ENOMEM, "workspace of %zu bytes exceeds the configured cap %zu", bytes, g.cfg.workspace_bytes);
     if (bytes <= g.ws_bytes) return 0;
-    if (g.ws) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.ws)); g.ws = nullptr; g.ws_bytes = 0; }
+    if (g.ws) { RCK(drop_dev(g.ws, defer)); g.ws = nullptr; g.ws_bytes = 0; }
     if (hipMalloc(&g.ws, bytes) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "hipMalloc of %zu bytes failed", bytes); }
     g.ws_bytes = bytes;
     return 0;
@@ -411,6 +438,7 @@ extern "C" void nttm_shutdown(void)
     for (int k = 0; k < 2; k++) if (g.bev[k]) (void)hipEventDestroy(g.bev[k]);
     for (int k = 0; k < 8; k++) if (g.tev[k]) (void)hipEventDestroy(g.tev[k]);
     if (g.tcs) (void)hipStreamDestroy(g.tcs);
+    free_deferred();
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) if (g.pev[i]) (void)hipEventDestroy(g.pev[i]);
     memset(&g, 0, sizeof g);
@@ -1554,10 +1582,10 @@ extern "C" int nttm_mul_each_async(nttm_prod *P, size_t N, void *stream)
 }
 
 /* ---- the binary-splitting product tree ----------------------------------------------------------------------------------- */
-static int bsp_need(int k, size_t limbs)
+static int bsp_need(int k, size_t limbs, bool defer = false)
 {
     if (limbs <= g.bsp_limbs[k]) return 0;
-    if (g.bsp[k]) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.bsp[k])); g.bsp[k] = nullptr; g.bsp_limbs[k] = 0; }
+    if (g.bsp[k]) { RCK(drop_dev(g.bsp[k], defer)); g.bsp[k] = nullptr; g.bsp_limbs[k] = 0; }
     if (hipMalloc(&g.bsp[k], limbs * 8) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "hipMalloc of %zu limbs failed", limbs); }
     g.bsp_limbs[k] = limbs;
     return 0;
@@ -1572,18 +1600,18 @@ static int bsp_need(int k, size_t limbs)
  * Returns 1, having launched nothing, when a pair's |P_R| exceeds |P_L| + |Q_R| on the planned lengths or the residue
  * post-condition is configured (the level-by-level path handles both). */
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
-static int tdesc_need(size_t bytes)
+static int tdesc_need(size_t bytes, bool defer = false)
 {
     if (bytes <= g.tdesc_bytes) return 0;
-    if (g.tdesc) { HIPCK(hipDeviceSynchronize()); HIPCK(hipFree(g.tdesc)); g.tdesc = nullptr; g.tdesc_bytes = 0; }
+    if (g.tdesc) { RCK(drop_dev(g.tdesc, defer)); g.tdesc = nullptr; g.tdesc_bytes = 0; }
     if (hipMalloc(&g.tdesc, bytes) != hipSuccess) { (void)hipGetLastError(); return nttm_fail(NTTM_ENOMEM, "tree descriptors: %zu bytes", bytes); }
     g.tdesc_bytes = bytes;
     return 0;
 }
-static int tpin_need(size_t bytes)
+static int tpin_need(size_t bytes, bool defer = false)
 {
     if (bytes <= g.tpin_bytes) return 0;
-    if (g.tpin) { HIPCK(hipDeviceSynchronize()); HIPCK(hipHostFree(g.tpin)); g.tpin = nullptr; g.tpin_bytes = 0; }
+    if (g.tpin) { RCK(drop_host(g.tpin, defer)); g.tpin = nullptr; g.tpin_bytes = 0; }
     HIPCK(hipHostMalloc((void **)&g.tpin, bytes, hipHostMallocDefault));
     g.tpin_bytes = bytes;
     return 0;
@@ -1699,10 +1727,13 @@ static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, si
                 }
         }
     }
-    RCK(bsp_need(0, pool_max[0])); RCK(bsp_need(1, pool_max[1]));
-    RCK(ws_need(planes_max ? planes_max : 256));
-    RCK(tdesc_need(desc_total + 512));
-    RCK(tpin_need(img_total + 8 * ngroups + 2048));
+    /* grown without waiting for the work queued before the tree (the caller's leaves): the outgrown buffers are freed
+     * after the tree's final synchronization (a failed call's, at the next call's start) */
+    if (g.ndfree || g.nhfree) { HIPCK(hipDeviceSynchronize()); free_deferred(); }
+    RCK(bsp_need(0, pool_max[0], true)); RCK(bsp_need(1, pool_max[1], true));
+    RCK(ws_need(planes_max ? planes_max : 256, true));
+    RCK(tdesc_need(desc_total + 512, true));
+    RCK(tpin_need(img_total + 8 * ngroups + 2048, true));
     if (!g.tcs) {
         HIPCK(hipStreamCreateWithFlags(&g.tcs, hipStreamNonBlocking));
         for (int j = 0; j < 8; j++) HIPCK(hipEventCreateWithFlags(&g.tev[j], hipEventDisableTiming));
@@ -1786,6 +1817,7 @@ static int bs_tree_planned(const nttm_bs_node *leaves, size_t n, uint64_t *p, si
     HIPCK(hipMemcpyAsync(hlen, dlen, 16, hipMemcpyDeviceToHost, s));
     for (size_t j = 0; j < flag_at.size(); j++) HIPCK(hipMemcpyAsync(hf + 4 * j, flag_at[j], 4, hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
+    free_deferred();
     for (size_t j = 0; j < flag_at.size(); j++)
         if (((const uint32_t *)hf)[j] & 1) return nttm_fail(NTTM_ECHECK, "crt: non-zero carry beyond a top limb (wrong residues)");
     const size_t rp = hlen[0], rq = hlen[1];
