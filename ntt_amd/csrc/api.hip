@@ -1282,6 +1282,18 @@ static int stage_download(const StageSet &st, hipStream_t s)
  * 64 MiB halves: the DMA of chunk k overlaps the host copy of chunk k - 1, which is split over up to 8 threads (the
  * destination's pages are first touched there, so their faults are taken in parallel).  Synchronizes. */
 #define NTTM_DL_HALF ((size_t)64 << 20)
+/* the pinned halves of nttm_copy_to_host, allocated ahead of the copy (pinning 128 MiB takes milliseconds; a caller can
+ * run this on another host thread while the device works) */
+int nttm_copy_to_host_prepare(void)
+{
+    if (g.dl) return 0;
+    char *dl = nullptr;
+    HIPCK(hipHostMalloc((void **)&dl, 2 * NTTM_DL_HALF, hipHostMallocDefault));
+    memset(dl, 0, 2 * NTTM_DL_HALF);                             /* its pages touched here, not during the copy */
+    for (int k = 0; k < 2; k++) HIPCK(hipEventCreateWithFlags(&g.dlev[k], hipEventDisableTiming));
+    g.dl = dl;
+    return 0;
+}
 int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s)
 {
     if (!bytes) return nttm_hip_check(hipStreamSynchronize(s), "copy to host");
@@ -1290,10 +1302,7 @@ int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s)
         HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
         return nttm_hip_check(hipStreamSynchronize(s), "copy to host");
     }
-    if (!g.dl) {
-        HIPCK(hipHostMalloc((void **)&g.dl, 2 * NTTM_DL_HALF, hipHostMallocDefault));
-        for (int k = 0; k < 2; k++) HIPCK(hipEventCreateWithFlags(&g.dlev[k], hipEventDisableTiming));
-    }
+    RCK(nttm_copy_to_host_prepare());
     const size_t H = NTTM_DL_HALF, nch = (bytes + H - 1) / H;
     const unsigned hw = std::thread::hardware_concurrency(), T = hw >= 8 ? 8 : hw ? hw : 1;
     auto host_copy = [&](size_t c) {

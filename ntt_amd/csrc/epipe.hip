@@ -22,6 +22,7 @@
 #include <string.h>
 #include <chrono>
 #include <memory>
+#include <thread>
 #include <vector>
 #include "internal.h"
 #include "ebig.h"
@@ -34,6 +35,7 @@ int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, 
                     uint64_t nb, uint64_t sb, int64_t bpow, int sub, uint32_t *scratch, hipStream_t s);
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
 int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
+int nttm_copy_to_host_prepare(void);
 int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s);
 int nttm_mul_launch(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, hipStream_t s,
                     const uint32_t **flag);
@@ -510,6 +512,14 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     size_t N = 2;
     while (lgamma((double)N + 1.0) < need) N += N / 2 + 1;
     { size_t lo = N / 2, hi = N; while (lo + 1 < hi) { size_t md = (lo + hi) / 2; if (lgamma((double)md + 1.0) < need) lo = md; else hi = md; } N = hi; }
+    /* the download's pinned buffers (a fraction above 64 MiB goes through them), allocated and first touched on
+     * another host thread while the device forms the series (inside the copy, their first use cost about 7 ms at
+     * 10^9 digits) */
+    struct Prep {
+        std::thread t;
+        explicit Prep(bool on) { if (on) t = std::thread([]() { (void)nttm_copy_to_host_prepare(); }); }
+        ~Prep() { if (t.joinable()) t.join(); }
+    } prep(m * 8 > ((size_t)64 << 20));
     DevOps o;
     /* the arena: the series' leaves (about 2 m limbs per schoolbook level) and root, the Newton levels' numbers
      * (about 9 m limbs over all levels) and the quotient's (about 5 m), with room to spare (14 GB at 10^9 digits);
@@ -520,6 +530,22 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     if (nw && nw[0] == '1') o.wrap_h = SIZE_MAX;
     DNum P, Q, A, x, Y;
     RCK(e_series(o, N, P, Q));
+    /* the first touch of the caller's output pages (zeros) on 4 host threads while the device runs the reciprocal:
+     * inside the copy, their page faults cost about 4 ms at 10^9 digits (started after the tree, whose planning
+     * runs on this thread while the leaves are formed) */
+    struct Touch {
+        std::thread t[4];
+        Touch(bool on, char *p, size_t bytes)
+        {
+            const size_t part = (bytes / 4 + 4095) & ~(size_t)4095;
+            for (int i = 0; on && i < 4; i++) {
+                const size_t off = i * part, len = off < bytes ? std::min(part, bytes - off) : 0;
+                if (len) t[i] = std::thread([=]() { memset(p + off, 0, len); });
+            }
+        }
+        void join() { for (std::thread &x : t) if (x.joinable()) x.join(); }
+        ~Touch() { join(); }
+    } touch(m * 8 > ((size_t)64 << 20), (char *)out, m * 8);
     bool ge;
     RCK(o.addsub(A, P, 0, Q, 0, true, &ge));                      /* e - 2 ~ A / Q, 0 <= A < Q */
     if (!ge || !A.n) return nttm_fail(NTTM_ECHECK, "e pipeline: P <= Q");
@@ -533,6 +559,8 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     rc = eb_quotient(o, A, Q, m, x, Y, fc && fc[0] == '1');       /* floor(A B^m / Q) */
     if (rc) return rc > 0 ? rc : nttm_fail(NTTM_ECHECK, "e pipeline: quotient correction did not converge");
     if (Y.n > m) return nttm_fail(NTTM_ECHECK, "e pipeline: fraction of %zu limbs above %zu", Y.n, m);
+    if (prep.t.joinable()) prep.t.join();
+    touch.join();
     if (Y.n) RCK(nttm_copy_to_host(out, Y.ptr(), Y.n * 8, o.s));
     if (Y.n < m) memset(out + Y.n, 0, (m - Y.n) * 8);
     g_e_ms[3] = ms_since(t0);
