@@ -763,9 +763,13 @@ struct ResChunkH { const uint64_t *x; uint64_t n; };          /* must match digi
  * cyclic convolution c_k = sum_(i + j = k mod L) a_i b_j, i.e. a b mod (x^L - 1) at x = B before its carries wrap (the
  * caller folds v modulo B^L - 1).  The transforms are DFTs of length L, so the pointwise product gives the cyclic
  * convolution directly; each c_k has at most min(na, nb) terms, the bound of an ordinary product.  na, nb <= L, device
- * pointers; synchronous.  The Newton reciprocal's wrapped product (GMP's mpn_mulmod_bnm1 plays this part there). */
-int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s)
+ * pointers.  flag null: synchronous; else the launches only, the CRT's flag word copied to the pinned word *flag (bit
+ * 0 set: a carry beyond the top limb), read by the caller after it synchronizes the stream.  The Newton reciprocal's
+ * wrapped product (GMP's mpn_mulmod_bnm1 plays this part there). */
+int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s,
+                    const uint32_t **flag)
 {
+    if (flag) *flag = nullptr;
     if (!g.init) return nttm_fail(NTTM_ESTATE, "not initialised");
     int logn = -1, r3 = 0;
     if (L >= 16 && !(L & (L - 1))) logn = __builtin_ctzll(L);
@@ -776,7 +780,7 @@ int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b
     if (na > L || nb > L) return nttm_fail(NTTM_EINVAL, "cyclic product: operands of %zu and %zu limbs above %zu", na, nb, L);
     if (!v || (na && !a) || (nb && !b)) return nttm_fail(NTTM_EINVAL, "cyclic product: null pointer");
     const size_t nc = L + 3;
-    if (!na || !nb) { HIPCK(hipMemsetAsync(v, 0, nc * 8, s)); HIPCK(hipStreamSynchronize(s)); return 0; }
+    if (!na || !nb) { HIPCK(hipMemsetAsync(v, 0, nc * 8, s)); return flag ? 0 : nttm_hip_check(hipStreamSynchronize(s), "cyclic product"); }
     int np; RCK(choose_np(na < nb ? na : nb, &np));
     const size_t planes_bytes = (size_t)(np + 1) * L * 8;
     const size_t off_desc = align_up(planes_bytes, 256), off_crt = off_desc + 256;
@@ -791,6 +795,7 @@ int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b
     uint32_t *crt_scr = (uint32_t *)(g.ws + off_crt);
     RCK(nttm_crt_launch_batch(g.ws + off_desc, 1, nttm_crt_blocks(nc), np, g.cc, crt_scr, s, 8.0 * np * L + 8.0 * nc));
     HIPCK(hipMemcpyAsync(hs + 96, crt_scr, 8, hipMemcpyDeviceToHost, s));
+    if (flag) { *flag = (const uint32_t *)(hs + 96) + 1; return 0; }
     HIPCK(hipStreamSynchronize(s));
     if (((const uint32_t *)(hs + 96))[1] & 1) return nttm_fail(NTTM_ECHECK, "cyclic product: crt: non-zero carry beyond the top limb");
     return 0;
@@ -802,7 +807,7 @@ extern "C" int nttm_debug_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, 
     if ((na && nttm_require_device(a, na * 8, "a")) || (nb && nttm_require_device(b, nb * 8, "b")) ||
         nttm_require_device(v, (L + 3) * 8, "v"))
         return nttm_fail(NTTM_EINVAL, "nttm_debug_mul_cyclic: device pointers only");
-    return nttm_mul_cyclic(v, a, na, b, nb, L, (hipStream_t)stream);
+    return nttm_mul_cyclic(v, a, na, b, nb, L, (hipStream_t)stream, nullptr);
 }
 /* the transform length (points) the product rule gives `need` coefficients: a supported length of the cyclic product */
 size_t nttm_wrap_len(size_t need)

@@ -36,7 +36,8 @@ int nttm_run_addsub(uint64_t *r, uint64_t nout, const uint64_t *a, uint64_t na, 
 int nttm_run_top_nonzero(const uint64_t *x, uint64_t n, uint64_t *out_dev, hipStream_t s);
 int nttm_copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
 int nttm_copy_to_host_prepare(void);
-int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s);
+int nttm_mul_cyclic(uint64_t *v, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, size_t L, hipStream_t s,
+                    const uint32_t **flag);
 int nttm_mul_launch(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, hipStream_t s,
                     const uint32_t **flag);
 size_t nttm_wrap_len(size_t need);
@@ -339,10 +340,11 @@ struct DevOps {
         if (!a.n || !b.n) { r = DNum(); return 0; }
         DNum v, lo, hi, t;
         RCK(alloc(L + 3, v));
-        RCK(nttm_mul_cyclic(v.wptr(), a.ptr(), a.n, b.ptr(), b.n, L, s));
-        RCK(norm(v));
-        RCK(low(v, L, lo));
-        hi = shr(v, L);
+        const uint32_t *flag = nullptr;
+        RCK(nttm_mul_cyclic(v.wptr(), a.ptr(), a.n, b.ptr(), b.n, L, s, &flag));
+        RCK(low(v, L, lo));                                       /* synchronizes: the CRT's flags are in */
+        if (flag && (*flag & 1)) return nttm_fail(NTTM_ECHECK, "e pipeline: cyclic product: carry beyond the top limb");
+        hi = shr(v, L);                                           /* 3 limbs, top ones possibly 0 */
         RCK(addsub(t, lo, 0, hi, 0, false, nullptr));             /* v mod B^L + v div B^L < B^L + B^3 */
         if (t.n > L) {                                            /* B^L = 1 (mod B^L - 1) */
             RCK(low(t, L, lo));
