@@ -43,21 +43,40 @@ __device__ __forceinline__ constexpr uint32_t brev_c(uint32_t j)
  * jj = 0 then have the twiddle w^0 = 1 */
 #define NTTM_TRIVIAL_W(rlow, jj) ((jj) == 0 && __builtin_constant_p(rlow) && (rlow) == 0)
 
-template <int LOGR>
-__device__ __forceinline__ void dif_group(double *v, int s_lo, uint32_t rlow, const double2 *__restrict__ tw, double p,
-                                          double pinv)
+/* a stage table in global memory, read through a buffer resource: the table's base in scalar registers, the lane's
+ * part of an index as the 32-bit vector offset and its compile-time part as the scalar offset, so a twiddle's address
+ * costs no vector instruction (a 64-bit pointer sum costs two, in kernels bound by vector issue) */
+struct GTab {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ explicit GTab(const double2 *p)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, 0x7fffffff, 0x00020000)) {}
+};
+/* a stage table in LDS (or any pointer): restrict, so that its loads move freely around the tile's LDS stores */
+typedef const double2 *__restrict__ LTab;
+/* entry lane + fixed of a stage table (lane: the part that varies across lanes; fixed: a compile-time part) */
+__device__ __forceinline__ double2 tw_at(LTab tw, uint32_t lane, uint32_t fixed) { return tw[lane + fixed]; }
+__device__ __forceinline__ double2 tw_at(const GTab &tw, uint32_t lane, uint32_t fixed)
+{
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i x = __builtin_amdgcn_raw_buffer_load_b128(tw.r, (int)(lane * 16u), (int)(fixed * 16u), 0);
+    double2 d;
+    __builtin_memcpy(&d, &x, 16);
+    return d;
+}
+
+template <int LOGR, typename TW>
+__device__ __forceinline__ void dif_group(double *v, int s_lo, uint32_t rlow, TW tw, double p, double pinv)
 {
     constexpr int R = 1 << LOGR;
 #pragma unroll
     for (int lev = 0; lev < LOGR; lev++) {
         const int k = LOGR - 1 - lev, h = 1 << k;
         const bool red = (lev & 1) || lev == LOGR - 1;
-        const uint32_t tbase = (1u << (s_lo + k)) + rlow;
 #pragma unroll
         for (int jj = 0; jj < h; jj++) {
             /* a trivial twiddle: the difference follows the sum path's bound and reduction rule */
             const bool one = NTTM_TRIVIAL_W(rlow, jj);
-            const double2 w = one ? make_double2(1.0, 0.0) : tw[tbase + ((uint32_t)jj << s_lo)];
+            const double2 w = one ? make_double2(1.0, 0.0) : tw_at(tw, rlow, (1u << (s_lo + k)) + ((uint32_t)jj << s_lo));
 #pragma unroll
             for (int j = jj; j < R; j += 2 * h) {
                 const double u = v[j], x = v[j + h];
@@ -69,22 +88,20 @@ __device__ __forceinline__ void dif_group(double *v, int s_lo, uint32_t rlow, co
     }
 }
 
-template <int LOGR>
-__device__ __forceinline__ void dit_group(double *v, int s_lo, uint32_t rlow, const double2 *__restrict__ tw, double p,
-                                          double pinv)
+template <int LOGR, typename TW>
+__device__ __forceinline__ void dit_group(double *v, int s_lo, uint32_t rlow, TW tw, double p, double pinv)
 {
     constexpr int R = 1 << LOGR;
 #pragma unroll
     for (int lev = 0; lev < LOGR; lev++) {
         const int k = lev, h = 1 << k;
         const bool red = lev == LOGR - 1;
-        const uint32_t tbase = (1u << (s_lo + k)) + rlow;
 #pragma unroll
         for (int jj = 0; jj < h; jj++) {
             /* a trivial twiddle at the first level only (inputs within p: outputs within 2p, and the later bounds
              * below hold with 0.25p to spare) */
             const bool one = lev == 0 && NTTM_TRIVIAL_W(rlow, jj);
-            const double2 w = one ? make_double2(1.0, 0.0) : tw[tbase + ((uint32_t)jj << s_lo)];
+            const double2 w = one ? make_double2(1.0, 0.0) : tw_at(tw, rlow, (1u << (s_lo + k)) + ((uint32_t)jj << s_lo));
 #pragma unroll
             for (int j = jj; j < R; j += 2 * h) {
                 const double u = v[j], x = one ? v[j + h] : mul_shoup(v[j + h], w.x, w.y, p);
@@ -131,9 +148,9 @@ __device__ __forceinline__ void for_slots(const PassPos &pp, int s_lo, F f)
 }
 
 /* the radix step of one round on all PTS slots: PTS / R groups of R points */
-template <int LOGR, bool INV, int PTS>
-__device__ __forceinline__ void round_compute(double *v, const PassPos &pp, int s_lo, const double2 *tw, double p,
-                                              double pinv)
+/* TW: GTab (a global stage table) or LTab (a table in LDS, named explicitly so that it stays restrict) */
+template <int LOGR, bool INV, int PTS, typename TW>
+__device__ __forceinline__ void round_compute(double *v, const PassPos &pp, int s_lo, TW tw, double p, double pinv)
 {
     constexpr int R = 1 << LOGR, G = PTS / R;
     const int L = pp.logw + s_lo;
@@ -141,8 +158,8 @@ __device__ __forceinline__ void round_compute(double *v, const PassPos &pp, int 
     for (int g = 0; g < G; g++) {
         const uint32_t tau = (uint32_t)(g * pp.nthr + pp.t);
         const uint32_t rlow = (tau & ((1u << L) - 1)) >> pp.logw;      /* row bits below s_lo */
-        if (INV) dit_group<LOGR>(v + g * R, s_lo, rlow, tw, p, pinv);
-        else dif_group<LOGR>(v + g * R, s_lo, rlow, tw, p, pinv);
+        if (INV) dit_group<LOGR, TW>(v + g * R, s_lo, rlow, tw, p, pinv);
+        else dif_group<LOGR, TW>(v + g * R, s_lo, rlow, tw, p, pinv);
     }
 }
 
@@ -333,6 +350,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     pp.nthr = T / PTS; pp.t = threadIdx.x;
     const bool active = pp.t < pp.nthr;
     const double p = a.pk.p, pinv = a.pk.pinv;
+    const GTab gtw(a.tw.stage);
     constexpr int MAXR = PTS >= 16 ? 4 : PTS == 8 ? 3 : PTS == 4 ? 2 : 1;
     const int lom = a.lo + gm, nr = round_count(gm, MAXR);
     /* the tile: column block cb (W columns below bit lo) of block group hb (H blocks of 2^lom points) */
@@ -399,7 +417,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                            uint32_t) { v[sl] = lds.at(e); });)
         }
-        if (active) ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
+        if (active) ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, gtw, p, pinv);)
         if (i + 1 < nr && active)
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                uint32_t) { lds.at(e) = v[sl]; });)
@@ -603,7 +621,7 @@ __global__ __launch_bounds__(STREAM_T) void k_pass_stream(PassArgs a, uint32_t n
                     }
                 }
             }
-            ROUND_SWITCH(logR, round_compute<LR, INV, PTS>(v, pp, s_lo, stw, p, pinv);)
+            ROUND_SWITCH(logR, round_compute<LR, INV, PTS, LTab>(v, pp, s_lo, stw, p, pinv);)
             if (i + 1 < NR)
                 ROUND_SWITCH(logR, for_slots<LR, PTS>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t, uint32_t) {
                     cur[e] = v[sl]; });)
@@ -668,7 +686,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t,
                                                                                 uint32_t, uint32_t) { v[sl] = xs.at(e); });)
         }
-        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, a.tw.stage, p, pinv);)
+        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, GTab(a.tw.stage), p, pinv);)
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                             uint32_t) { xs.at(e) = v[sl]; });)
     }
@@ -685,7 +703,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         __syncthreads();
         if (active) ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                             uint32_t) { v[sl] = xs.at(e); });)
-        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+        if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, GTab(a.twi.stage), p, pinv);)
         if (i + 1 < nr && active)
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                     uint32_t) { xs.at(e) = v[sl]; });)
@@ -720,10 +738,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 /* stw: the forward stage table's first 2^STW_LOG entries in LDS (nullptr: none); rounds whose stages all lie below
  * STW_LOG read their twiddles there */
 #define STW_LOG 8
-template <int PTS, int MAXR, bool LIMBS>
+/* LT: stw is the LDS copy of the stage table's low entries, for the rounds after the first */
+template <int PTS, int MAXR, bool LIMBS, bool LT>
 __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &pp, bool active, uint64_t gbase,
-                                             const SwzLds &xs, double *v, double p, double pinv,
-                                             const double2 *stw = nullptr)
+                                             const SwzLds &xs, double *v, double p, double pinv, const double2 *stw)
 {
     const bool limb = LIMBS && (q.limbs || q.ldesc);
     /* a shared B: the tile (inside one transform) is read from the transform the map names */
@@ -758,8 +776,11 @@ __device__ __forceinline__ void tile_forward(const PassArgs &q, const PassPos &p
                                                                            uint32_t) { v[sl] = xs.at(e); });)
         }
         /* with stw (the 4096-point tile in rounds of 4 bits), rounds 1 and 2 cover stages 7..0 */
-        const double2 *tw = stw && i > 0 ? stw : q.tw.stage;
-        if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, tw, p, pinv);)
+        if (LT && i > 0) {
+            if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS, LTab>(v, pp, s_lo, stw, p, pinv);)
+        } else {
+            if (active) ROUND_SWITCH(logR, round_compute<LR, false, PTS>(v, pp, s_lo, GTab(q.tw.stage), p, pinv);)
+        }
         if (i + 1 < nr && active)
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
                                                                uint32_t) { xs.at(e) = v[sl]; });)
@@ -797,9 +818,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     int s_lo, logR;
 
     const double2 *stw = LT ? (const double2 *)stw_lds : nullptr;
-    tile_forward<PTS, MAXR, LIMBS>(bq, pp, active, gbase, xs, bv, p, pinv, stw);
+    tile_forward<PTS, MAXR, LIMBS, LT>(bq, pp, active, gbase, xs, bv, p, pinv, stw);
     __syncthreads();                                             /* B's last LDS reads before A's first writes */
-    tile_forward<PTS, MAXR, LIMBS>(a, pp, active, gbase, xs, v, p, pinv, stw);
+    tile_forward<PTS, MAXR, LIMBS, LT>(a, pp, active, gbase, xs, v, p, pinv, stw);
     if (active) {
 #pragma unroll
         for (int sl = 0; sl < PTS; sl++) v[sl] = mul_barrett(v[sl], bv[sl], p, pinv);
@@ -815,7 +836,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (LT && i < 2) {                                       /* inverse rounds 0 and 1: stages 0..7 */
             if (active) ROUND_SWITCH(logR, round_compute_mirror<LR, PTS>(v, pp, s_lo, stw, p, pinv);)
         } else {
-            if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, a.twi.stage, p, pinv);)
+            if (active) ROUND_SWITCH(logR, round_compute<LR, true, PTS>(v, pp, s_lo, GTab(a.twi.stage), p, pinv);)
         }
         if (i + 1 < nr && active)
             ROUND_SWITCH(logR, for_slots<LR, PTS, true>(pp, s_lo, [&](int sl, uint32_t e, uint32_t, uint32_t,
