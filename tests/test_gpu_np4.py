@@ -1,5 +1,5 @@
-"""Products on four primes end to end (the path above the three-prime bound, E/crt.c:91-99; BASELINE's "4 if the CRT
-bound exceeds the prime product"): the library re-initialised with nprimes = 4 forms a 2^20 x 2^20 product, a fused
+"""Products on four primes end to end (the path above the three-prime bound, E/crt.c:91-99: four primes when the CRT
+bound exceeds the three primes' product): the library re-initialised with nprimes = 4 forms a 2^20 x 2^20 product, a fused
 + x, a mixed batch, a graph plan and a grid product, each exact against GMP or the oracle; the k_crt launches carry
 four planes (the live profile's byte count); and a product above the three-prime bound gives NTTM_EBOUND under a
 fixed three-prime set, before any launch."""
