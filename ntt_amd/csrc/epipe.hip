@@ -524,8 +524,9 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
     } prep(m * 8 > ((size_t)64 << 20));
     DevOps o;
     /* the arena: the series' leaves (about 2 m limbs per schoolbook level) and root, the Newton levels' numbers
-     * (about 9 m limbs over all levels) and the quotient's (about 5 m), with room to spare (14 GB at 10^9 digits);
-     * NTTM_E_ARENA=0 allocates per number */
+     * (their products, wrapped products of L + 3 limbs and differences: about 12 m limbs over all levels) and the
+     * quotient's (about 5 m), with room to spare (14 GB at 10^9 digits); past it, a number is allocated on its own;
+     * NTTM_E_ARENA=0 allocates every number on its own */
     const char *ea = getenv("NTTM_E_ARENA");
     RCK(o.init((hipStream_t)stream, ea && ea[0] == '0' ? 0 : 32 * (m + 1024)));
     const char *nw = getenv("NTTM_E_NO_WRAP");
