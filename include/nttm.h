@@ -21,8 +21,8 @@
  * return complete, E/rns_mul.c:792-1028.)  Growing the library's workspace synchronizes the device.  One host thread
  * per library instance; calls are not reentrant.
  *
- * Errors: every entry returns 0 on success or a negative NTTM_E* code, and nttm_last_error() describes the last
- * failure.  The reference instead ends the process through ec_fatal (E/fatal.h:19-23); its limits are kept as
+ * Errors: every entry returns 0 on success or a negative NTTM_E* code, and nttm_last_error() describes the calling
+ * thread's last failure.  The reference instead ends the process through ec_fatal (E/fatal.h:19-23); its limits are kept as
  * errors here: transform length above 2^32 (E/ntt.c:517-520), a product above the prime set's term bound
  * (E/crt.c:91-99), limbs >= 10^18 are the caller's contract (not checked on the device).
  *
@@ -150,7 +150,8 @@ typedef struct nttm_bs_node {
 int nttm_bs_tree(const nttm_bs_node *leaves, size_t n, uint64_t *p, size_t p_cap, size_t *pn, uint64_t *q, size_t q_cap,
                  size_t *qn, void *stream);
 /* e's fraction to m limbs: out[0 .. m) = floor((e - 2) 10^(18 m)), least significant limb first, HOST memory (the
- * limbs read from the top are e's decimal digits after "2.", 18 per limb).  The reference's pipeline on one GPU
+ * limbs read from the top are e's decimal digits after "2.", 18 per limb; NTTM_EINVAL for device memory, before any
+ * work).  The reference's pipeline on one GPU
  * (E/ALGORITHM.md:12-20,59-94): binary splitting of sum 1/k! (leaves of up to 64 terms folded on the device, then
  * nttm_bs_tree), a Newton reciprocal of Q with doubling precision and a corrected quotient, every product on the
  * device.  N terms with N! > 10^(18 (m + 2)) must stay below 10^9 - 2 (m up to about 4.7 10^8 limbs, 8.5 10^9 digits).  Synchronous. */
@@ -178,6 +179,9 @@ int nttm_e_write_packed(const char *path, const uint64_t *frac, size_t m, uint64
  * the products are grouped by transform length; with autotune != 0 each group's schedule is chosen by timing the
  * candidates on these operands -- per length class, 3 2^k points for the products that fit them or 2^(k+2); per group,
  * a strided-pass tile of 2^12 or 2^13 points and 8 or 16 points per thread -- else the library's defaults are taken.
+ * The choices are remembered until nttm_shutdown, per prime count, transform length and group size; autotune == 2
+ * takes the remembered choice of a group of the same size, else of the nearest size in the same power-of-two class,
+ * without timing it (autotune == 1 times every group and refreshes what is remembered).
  * Every descriptor is uploaded once and all launches are captured into one hipGraph.  P is HOST memory holding device
  * pointers; the operands and outputs it names must stay allocated until the plan is destroyed (their contents may
  * change between runs).  Creating an autotuned plan runs the products several times.  Destroy plans before

@@ -460,11 +460,14 @@ class BatchPlan:
     uploaded, launches captured as one hipGraph -- and run many times"""
 
     def __init__(self, prods, autotune=True, stream=None):
+        """autotune: False (the library's defaults), True (every candidate timed; the choices are remembered) or
+        "reuse" (a choice remembered for a group of the same length and size class is taken untimed)"""
         self.prods = prods                                       # the records (their graph has the addresses built in)
         self._keep = getattr(prods, "_keep", None)               # and the arrays they point to, kept alive with the plan
         self.n = len(prods)
         h = C.c_void_p()
-        _check(lib().nttm_batch_plan_create(prods, self.n, 1 if autotune else 0, C.byref(h), _stream(stream)))
+        mode = 2 if autotune == "reuse" else (1 if autotune else 0)
+        _check(lib().nttm_batch_plan_create(prods, self.n, mode, C.byref(h), _stream(stream)))
         self.h = h
 
     def run(self, stream=None):

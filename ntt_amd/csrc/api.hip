@@ -49,7 +49,8 @@ int nttm_run_r3(bool inv, double *x, uint64_t batch, int logk, const PrimeK &pk,
                 hipStream_t s);
 
 /* ---- errors ----------------------------------------------------------------------------------------------------- */
-static char g_err[512] = "";
+/* per host thread: the library's own helper threads (the e pipeline's download preparation) may fail too */
+static thread_local char g_err[512] = "";
 int nttm_fail(int code, const char *fmt, ...)
 {
     va_list ap; va_start(ap, fmt); vsnprintf(g_err, sizeof g_err, fmt, ap); va_end(ap);
@@ -354,6 +355,15 @@ int nttm_require_device(const void *p, size_t bytes, const char *what)
     return nttm_fail(NTTM_EINVAL, "%s %s", what, k == MEM_BAD ? mm.why : "is host memory (this entry point takes device memory)");
 }
 #define DEVCK(p, bytes, what) RCK(nttm_require_device((p), (bytes), (what)))
+/* NTTM_EINVAL unless [p, p + bytes) is host memory (pageable, pinned or managed): entry points whose output the host
+ * writes or first-touches */
+int nttm_require_host(const void *p, size_t bytes, const char *what)
+{
+    MemMap mm;
+    const int k = mm.kind(p, bytes);
+    if (k == MEM_HOST || k == MEM_PINNED) return 0;
+    return nttm_fail(NTTM_EINVAL, "%s %s", what, k == MEM_BAD ? mm.why : "is device memory (this entry point takes host memory)");
+}
 
 extern "C" int nttm_init(int device, const nttm_config *cfg)
 {
@@ -2188,6 +2198,7 @@ struct nttm_batch_plan {
     hipGraphExec_t exec = nullptr;
     double tune_ms = 0;
     int candidates = 0;
+    int reused = 0;                                              /* groups that took a remembered choice */
 };
 
 static void plan_free(nttm_batch_plan *pl)
@@ -2215,13 +2226,53 @@ struct TuneBuf {
         return 0;
     }
 };
-/* the fastest (tile, points per thread) for one group of products, timed on the stream; *ms its time */
+/* the autotuner's choices, kept across plans: per prime count, transform length (2^logn, or 3 2^logn with r3) and group
+ * size (its products), the (tile, points per thread) that won and its time per product.  Each entry belongs to the
+ * nttm_init it was timed under.  Plans created with autotune == 2 take the entry of the same size, else the nearest
+ * size of the same power-of-two class, instead of timing the candidates again */
+struct TuneMemo { uint64_t gen; int np, r3, logn; uint64_t cnt; int tile, pts; double ms_per; };
+#define TUNE_MEMO_MAX 4096
+static TuneMemo g_memo[TUNE_MEMO_MAX];
+static int g_nmemo = 0;
+static const TuneMemo *memo_find(int np, int r3, int logn, uint64_t cnt)
+{
+    const TuneMemo *best = nullptr;
+    uint64_t dbest = 0;
+    for (int i = 0; i < g_nmemo; i++) {
+        const TuneMemo &m = g_memo[i];
+        if (m.gen != g.gen || m.np != np || m.r3 != r3 || m.logn != logn || ceil_log2(m.cnt) != ceil_log2(cnt)) continue;
+        const uint64_t d = m.cnt > cnt ? m.cnt - cnt : cnt - m.cnt;
+        if (!best || d < dbest) { best = &m; dbest = d; }
+    }
+    return best;
+}
+static void memo_put(const TuneMemo &e)
+{
+    TuneMemo *slot = nullptr;
+    for (int i = 0; i < g_nmemo && !slot; i++) {                 /* the same key, or an entry of an earlier nttm_init */
+        TuneMemo &m = g_memo[i];
+        if (m.gen != g.gen || (m.np == e.np && m.r3 == e.r3 && m.logn == e.logn && m.cnt == e.cnt)) slot = &m;
+    }
+    if (!slot && g_nmemo < TUNE_MEMO_MAX) slot = &g_memo[g_nmemo++];
+    if (slot) *slot = e;                                         /* (a full table keeps its entries) */
+}
+/* the fastest (tile, points per thread) for one group of products, timed on the stream; *ms its time.  reuse: a
+ * remembered choice for the same class is taken untimed (*ms scaled from its time per product; *nreused counts it) */
 static int tune_group(const nttm_prod *P, const std::vector<size_t> &G, int np, int r3, int logn, bool tune, TuneBuf &tb,
-                      std::vector<char> &img, hipStream_t s, GroupBuild &best, double *ms, int *ntried)
+                      std::vector<char> &img, hipStream_t s, GroupBuild &best, double *ms, int *ntried, bool reuse = false,
+                      int *nreused = nullptr)
 {
     static const int cand[4][2] = {{12, 16}, {13, 16}, {12, 8}, {13, 8}};
     *ms = 0;
     if (!tune) { group_build(best, P, G, np, r3, logn, g.cfg.tile_log, batch_pts(logn)); return 0; }
+    if (reuse) {
+        if (const TuneMemo *m = memo_find(np, r3, logn, G.size())) {
+            group_build(best, P, G, np, r3, logn, m->tile, m->pts);
+            *ms = m->ms_per * (double)G.size();
+            if (nreused) (*nreused)++;
+            return 0;
+        }
+    }
     double best_ms = 1e30;
     hipEvent_t e0 = g.ev[0], e1 = g.ev[1];
     for (int c = 0; c < 4; c++) {
@@ -2246,6 +2297,7 @@ static int tune_group(const nttm_prod *P, const std::vector<size_t> &G, int np, 
         if (t < best_ms) { best_ms = t; best = std::move(gb); }
     }
     *ms = best_ms;
+    memo_put(TuneMemo{g.gen, np, r3, logn, (uint64_t)G.size(), best.tile, best.pts, best_ms / (double)G.size()});
     return 0;
 }
 
@@ -2290,7 +2342,7 @@ extern "C" int nttm_batch_plan_create(const nttm_prod *P, size_t N, int autotune
      * points or stays at 2^k; per group the tile and points per thread */
     TuneBuf tb;
     std::vector<char> img;
-    const bool tune = autotune != 0;
+    const bool tune = autotune != 0, reuse = autotune == 2;
     auto t_start = std::chrono::steady_clock::now();
     for (int k = 0; k <= NTTM_LOGN_MAX; k++) {
         const std::vector<size_t> &C = cls[k];
@@ -2304,12 +2356,14 @@ extern "C" int nttm_batch_plan_create(const nttm_prod *P, size_t N, int autotune
         GroupBuild a, b3, b2;
         double ta = 0, t3 = 0, t2 = 0;
         if (tune && !S3.empty()) {                               /* both variants timed */
-            if ((rc = tune_group(P, C, np, 0, k, true, tb, img, s, a, &ta, &pl->candidates))) return fail(rc);
-            if ((rc = tune_group(P, S3, np, 1, k - 2, true, tb, img, s, b3, &t3, &pl->candidates))) return fail(rc);
-            if (!S2.empty() && (rc = tune_group(P, S2, np, 0, k, true, tb, img, s, b2, &t2, &pl->candidates))) return fail(rc);
+            if ((rc = tune_group(P, C, np, 0, k, true, tb, img, s, a, &ta, &pl->candidates, reuse, &pl->reused))) return fail(rc);
+            if ((rc = tune_group(P, S3, np, 1, k - 2, true, tb, img, s, b3, &t3, &pl->candidates, reuse, &pl->reused)))
+                return fail(rc);
+            if (!S2.empty() && (rc = tune_group(P, S2, np, 0, k, true, tb, img, s, b2, &t2, &pl->candidates, reuse, &pl->reused)))
+                return fail(rc);
             if (ta <= t3 + t2) { pl->gb.push_back(std::move(a)); pl->gidx.push_back(C); pl->gms.push_back(ta); continue; }
         } else if (S3.empty()) {
-            if ((rc = tune_group(P, C, np, 0, k, tune, tb, img, s, a, &ta, &pl->candidates))) return fail(rc);
+            if ((rc = tune_group(P, C, np, 0, k, tune, tb, img, s, a, &ta, &pl->candidates, reuse, &pl->reused))) return fail(rc);
             pl->gb.push_back(std::move(a)); pl->gidx.push_back(C); pl->gms.push_back(ta);
             continue;
         } else {                                                 /* untuned: the library's rule */
@@ -2419,8 +2473,9 @@ extern "C" int nttm_batch_plan_info(const nttm_batch_plan *pl, char *buf, size_t
                       (unsigned long long)x.n, x.r3, x.logn, (unsigned long long)x.cnt,
                       (unsigned long long)(x.share ? x.cnt - x.cnt_b : 0), x.tile, x.pts, pl->gms[i]);
     }
-    if (k < cap) k += snprintf(buf + k, cap - k, "plan products=%zu empty=%zu primes=%d candidates=%d tune_ms=%.1f blob_bytes=%zu\n",
-                               pl->N, pl->zidx.size(), pl->np, pl->candidates, pl->tune_ms, pl->blob_bytes);
+    if (k < cap) k += snprintf(buf + k, cap - k, "plan products=%zu empty=%zu primes=%d candidates=%d reused=%d tune_ms=%.1f "
+                               "blob_bytes=%zu\n", pl->N, pl->zidx.size(), pl->np, pl->candidates, pl->reused, pl->tune_ms,
+                               pl->blob_bytes);
     return k < cap ? 0 : nttm_fail(NTTM_EINVAL, "nttm_batch_plan_info: buffer too small");
 }
 

@@ -508,6 +508,7 @@ static int e_series(DevOps &o, size_t N, DNum &P, DNum &Q)
 extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
 {
     if (!out || !m) return nttm_fail(NTTM_EINVAL, "nttm_e_fraction arguments");
+    RCK(nttm_require_host(out, m * 8, "out"));                    /* host threads write its pages */
     for (double &v : g_e_ms) v = 0;
     /* N: log(N!) > 18 (m + 2) log 10 */
     const double need = 18.0 * (double)(m + 2) * log(10.0);
@@ -519,7 +520,12 @@ extern "C" int nttm_e_fraction(uint64_t *out, size_t m, void *stream)
      * 10^9 digits) */
     struct Prep {
         std::thread t;
-        explicit Prep(bool on) { if (on) t = std::thread([]() { (void)nttm_copy_to_host_prepare(); }); }
+        explicit Prep(bool on)
+        {
+            int dev = 0;
+            if (on && hipGetDevice(&dev) == hipSuccess)         /* a new thread starts on device 0: the caller's device */
+                t = std::thread([dev]() { if (hipSetDevice(dev) == hipSuccess) (void)nttm_copy_to_host_prepare(); });
+        }
         ~Prep() { if (t.joinable()) t.join(); }
     } prep(m * 8 > ((size_t)64 << 20));
     DevOps o;

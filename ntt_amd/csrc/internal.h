@@ -68,6 +68,7 @@ int nttm_fail(int code, const char *fmt, ...);
 int nttm_hip_check(hipError_t e, const char *what);
 /* NTTM_EINVAL unless [p, p + bytes) is device memory of the library's device (null or empty ranges pass) */
 int nttm_require_device(const void *p, size_t bytes, const char *what);
+int nttm_require_host(const void *p, size_t bytes, const char *what);
 #define HIPCK(call) do { int rc_ = nttm_hip_check((call), #call); if (rc_) return rc_; } while (0)
 #define RCK(call) do { int rc_ = (call); if (rc_) return rc_; } while (0)
 

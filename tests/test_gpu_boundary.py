@@ -210,6 +210,17 @@ def test_device_only_entries_refuse_host_memory(nttm):
     torch.cuda.synchronize()
 
 
+def test_e_fraction_refuses_device_output(nttm):
+    """nttm_e_fraction writes its output from the host (host threads first-touch the pages, the fraction arrives by
+    DMA): a device array is refused with NTTM_EINVAL before any work"""
+    import torch
+    L = nttm.lib()
+    d = torch.zeros(64, dtype=torch.int64, device="cuda")
+    assert L.nttm_e_fraction(C.cast(d.data_ptr(), C.POINTER(C.c_uint64)), 64, None) == -1
+    assert b"device memory" in L.nttm_last_error()
+    assert int(nttm.e_fraction(64)[63]) == 718281828459045235   # host memory: e = 2.718281828459045235...
+
+
 def test_stale_plan_refused_after_reinit(nttm):
     """a plan made before nttm_shutdown + nttm_init returns NTTM_ESTATE from run and ncn (its captured tables are
     gone); destroying it is still safe"""

@@ -2,6 +2,7 @@
 the eager batch (nttm_mul_batch, itself checked against the oracle and GMP in test_gpu_mul.py) on the same operands,
 with eager work and synchronizations between the replays; operands rewritten between runs are read anew; products
 with an empty operand, a fused + x and a shared B operand are included; the schedule is reported per group."""
+import os
 import re
 
 import numpy as np
@@ -102,6 +103,47 @@ def test_plan_replays_equal_eager(nttm, autotune):
             assert int(re.search(r"candidates=(\d+)", info).group(1)) >= 4 * len(groups)
     finally:
         plan.close()
+
+
+def test_plan_reuses_remembered_choices(nttm):
+    """autotune="reuse": a plan over products of the same classes as an autotuned one times nothing and takes the same
+    schedule, and its replays equal the eager batch; after nttm_shutdown + nttm_init nothing is remembered"""
+    import torch
+    ops = mixed_ops(35)
+    items = device_items(ops)
+    sched = lambda info: sorted(re.sub(r" ms=\S+", "", l) for l in info.splitlines() if l.startswith("group"))
+    count = lambda info, key: int(re.search(key + r"=(\d+)", info).group(1))
+    say("plan tuned")
+    p1 = nttm.BatchPlan(nttm.make_prods(items), autotune=True)
+    items2 = device_items(ops)                                 # other buffers, the same shapes
+    say("plan reuse")
+    p2 = nttm.BatchPlan(nttm.make_prods(items2), autotune="reuse")
+    try:
+        i1, i2 = p1.info(), p2.info()
+        assert count(i1, "candidates") > 0 and count(i1, "reused") == 0
+        assert count(i2, "candidates") == 0 and count(i2, "reused") >= len(sched(i2)), i2
+        assert sched(i2) == sched(i1)
+        want, want_ncn = eager(nttm, ops, items2)
+        for rep in range(2):
+            for it in items2:
+                it[3].fill_(-1)
+            torch.cuda.synchronize()
+            say("reuse plan replay", rep)
+            p2.run()
+            assert p2.ncn() == want_ncn
+            assert all(np.array_equal(g, w) for g, w in zip(outputs(ops, items2), want)), rep
+    finally:
+        p1.close()
+        p2.close()
+    nttm.shutdown()
+    nttm.init(0, tile_log=int(os.environ.get("NTTM_TEST_TILE_LOG", "0")), pts=int(os.environ.get("NTTM_TEST_PTS", "0")))
+    say("plan reuse after re-init")
+    p3 = nttm.BatchPlan(nttm.make_prods(items2), autotune="reuse")
+    try:
+        i3 = p3.info()
+        assert count(i3, "reused") == 0 and count(i3, "candidates") > 0
+    finally:
+        p3.close()
 
 
 def test_plan_leaf_workload_equals_eager(nttm):
