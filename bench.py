@@ -730,6 +730,10 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(name), "kernel": name,
             "launches": launches, "avg_launch_ms": round(kms / launches, 4),
             "alg_bytes_per_launch": kbytes / launches}
+    if name == "k_fused2":
+        # what bounds it instead of HBM (rocprofv3 counters of the same product)
+        roof["limit"] = ("vector issue: 178 VALU instructions per point (166.5 FP64) at 79% of the issue rate "
+                         "(profiles/product_2e27_valu_issue.md)")
 
     cfg1 = None
     if not args.no_config1:
