@@ -176,6 +176,26 @@ def bench_config1(nttm, reps=12):
            "median_ms": round(1e3 * med, 4), "min_ms": round(1e3 * min(ts), 4), "max_ms": round(1e3 * max(ts), 4),
            "b_alg_bytes": b_alg, "hbm_GBs": round(b_alg / med / 1e9, 1),
            "hbm_frac": round(b_alg / med / 1e9 / HBM_PEAK_GBS, 4)}
+    # the same product planned once as a one-product batch plan: each call is one graph launch and the read of its
+    # length (it returns complete), so the host's per-kernel launch costs are paid at planning time
+    dp = torch.empty(2 * n, dtype=torch.int64, device="cuda")
+    plan = nttm.BatchPlan(nttm.make_prods([(da, db, None, dp)]), autotune=True)
+    try:
+        for _ in range(3):
+            plan.run()
+            plan.ncn()
+        tp = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            plan.run()
+            plan.ncn()
+            tp.append(time.perf_counter() - t0)
+        pmed = float(np.median(tp))
+        out.update({"plan_median_ms": round(1e3 * pmed, 4), "plan_min_ms": round(1e3 * min(tp), 4),
+                    "plan_hbm_frac": round(b_alg / pmed / 1e9 / HBM_PEAK_GBS, 4),
+                    "plan_equals_call": bool(torch.equal(dp, dc))})
+    finally:
+        plan.close()
     if oracle.have_gmp():
         ch = dc.cpu().numpy().view(np.uint64)
         out["exact_vs_gmp"] = bool(oracle.gmp_check(a, b, ch))

@@ -109,6 +109,8 @@ struct Ctx {
     void *dfree[8]; int ndfree;              /* buffers the planned tree outgrew, freed after its final synchronization */
     void *hfree[4]; int nhfree;
     hipEvent_t tev[8];                       /* the last copies on tcs, in turn */
+    hipStream_t ps[NTTM_MAX_PRIMES - 1];     /* primes 1.. of a small product, beside the caller's stream (prime 0) */
+    hipEvent_t pfev, pjev[NTTM_MAX_PRIMES - 1];   /* their fork from and join to the caller's stream */
     uint64_t *hpin; size_t hpin_bytes;       /* pinned host staging (descriptors, flags) */
     char *stage; size_t stage_bytes;         /* device copies of host-resident operands and results */
     size_t stage_chunk;                      /* staged bytes per sub-batch of a batch */
@@ -448,6 +450,11 @@ extern "C" void nttm_shutdown(void)
     for (int k = 0; k < 2; k++) if (g.bev[k]) (void)hipEventDestroy(g.bev[k]);
     for (int k = 0; k < 8; k++) if (g.tev[k]) (void)hipEventDestroy(g.tev[k]);
     if (g.tcs) (void)hipStreamDestroy(g.tcs);
+    for (int k = 0; k < NTTM_MAX_PRIMES - 1; k++) {
+        if (g.ps[k]) (void)hipStreamDestroy(g.ps[k]);
+        if (g.pjev[k]) (void)hipEventDestroy(g.pjev[k]);
+    }
+    if (g.pfev) (void)hipEventDestroy(g.pfev);
     free_deferred();
     for (int i = 0; i < 4; i++) (void)hipEventDestroy(g.ev[i]);
     for (int i = 0; i < 2 * NTTM_MAX_PRIMES; i++) if (g.pev[i]) (void)hipEventDestroy(g.pev[i]);
@@ -575,11 +582,14 @@ static void defer_set(PassArgs &q, const Plan &pl, int j, bool on)
     if (on && j == 0) { q.dfr_do_scale = q.do_in_scale; q.dfr_scale = q.in_scale; }
     if (j > 1) q.dfr_do_scale = 0;
 }
+/* ps: prime i's work on stream ps[i] with its own B plane at Bp + i bstride (the caller forks and joins them); null:
+ * every prime on s, one after another, through the one B plane at Bp */
 static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A, double *Bp, const uint64_t *a,
-                              uint64_t na, const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s,
+                              uint64_t na, const uint64_t *b, uint64_t nb, const LoadDesc *ldesc, hipStream_t s0,
                               hipEvent_t *pev, uint64_t cnt_b = 0, const LoadDesc *ldesc_b = nullptr,
-                              const uint32_t *bmap = nullptr)
+                              const uint32_t *bmap = nullptr, const hipStream_t *ps = nullptr, uint64_t bstride = 0)
 {
+    double *const Bp0 = Bp;
     if (!bmap) { cnt_b = cnt; ldesc_b = ldesc; }
     const uint64_t tb = r3 ? 3 * cnt : cnt, tbb = r3 ? 3 * cnt_b : cnt_b;   /* engine transforms of 2^logn points */
     Plan pl; product_plan(&pl, logn, tb, product_tile_log(logn));
@@ -591,6 +601,8 @@ static int product_transforms(int np, int logn, int r3, uint64_t cnt, double *A,
     for (int i = 0; i < np; i++) {
         double *Ai = A + (size_t)i * plane;
         const uint64_t p = NTTM_PRIMES[i];
+        const hipStream_t s = ps ? ps[i] : s0;
+        Bp = ps ? Bp0 + (size_t)i * bstride : Bp0;
         /* B, prime i, all passes but the last */
         if (pev) HIPCK(hipEventRecord(pev[2 * i], s));
         const double2 ninv = h_shoup(h_inv(ntot % p, p), p);     /* the inverse's 1/length, folded into B */
@@ -686,8 +698,22 @@ static bool fits_one(size_t na, size_t nb, size_t nc)
 }
 
 /* the workspace of a one-transform product: the planes, its CRT descriptor and the CRT's scratch (bytes from ws) */
-struct CoreLayout { int np, logn, r3; uint64_t n, ncoef; size_t nc, nc_full, off_desc, off_crt, bytes; };
-static int core_layout(size_t na, size_t nb, bool has_x, size_t wcut, CoreLayout *L)
+struct CoreLayout { int np, logn, r3, par; uint64_t n, ncoef; size_t nc, nc_full, off_desc, off_crt, bytes; };
+/* transforms of at most this many points run their primes concurrently (a launch of one prime's pass has too few tiles
+ * to fill the device); NTTM_PAR_POINTS overrides it, 0 turns it off */
+static uint64_t par_points(void)
+{
+    static uint64_t v = 0;
+    static bool set = false;
+    if (!set) {
+        const char *e = getenv("NTTM_PAR_POINTS");
+        v = e ? strtoull(e, nullptr, 0) : ((uint64_t)1 << 22);
+        set = true;
+    }
+    return v;
+}
+/* allow_par: the primes may run concurrently, each with its own B plane (the library's workspace, not captured) */
+static int core_layout(size_t na, size_t nb, bool has_x, size_t wcut, CoreLayout *L, bool allow_par = false)
 {
     L->nc_full = na + nb + (has_x ? 1 : 0); L->nc = L->nc_full < wcut ? L->nc_full : wcut;
     RCK(choose_np(na < nb ? na : nb, &L->np));
@@ -696,11 +722,16 @@ static int core_layout(size_t na, size_t nb, bool has_x, size_t wcut, CoreLayout
     choose_len(ncoef_full, &L->logn, &L->r3);
     if (L->logn > NTTM_LOGN_MAX) return nttm_fail(NTTM_EBOUND, "transform length 2^%d above 2^%d", L->logn, NTTM_LOGN_MAX);
     L->n = (uint64_t)(L->r3 ? 3 : 1) << L->logn;                  /* points per plane */
-    /* np planes of A and one of B: B's transform of each prime is made just before A's of that prime */
-    const size_t planes_bytes = (size_t)(L->np + 1) * L->n * 8;
+    /* np planes of A and one of B: B's transform of each prime is made just before A's of that prime; concurrent
+     * primes take a B plane each */
     const size_t crt_words = nttm_crt_scratch_words(nttm_crt_blocks(L->nc));
-    L->off_desc = align_up(planes_bytes, 256); L->off_crt = L->off_desc + 256;
-    L->bytes = L->off_crt + crt_words * 4 + 256;
+    for (int par = allow_par && L->np > 1 && L->n <= par_points(); par >= 0; par--) {
+        const size_t planes_bytes = (size_t)(par ? 2 * L->np : L->np + 1) * L->n * 8;
+        L->par = par;
+        L->off_desc = align_up(planes_bytes, 256); L->off_crt = L->off_desc + 256;
+        L->bytes = L->off_crt + crt_words * 4 + 256;
+        if (!g.cfg.workspace_bytes || L->bytes <= g.cfg.workspace_bytes) break;   /* concurrent only within the cap */
+    }
     return 0;
 }
 
@@ -725,7 +756,9 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
         return 0;
     }
     if (!nc) return 0;
-    CoreLayout L; RCK(core_layout(na, nb, x != nullptr, wcut, &L));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (!ws && hipStreamIsCapturing(s, &cap) != hipSuccess) { (void)hipGetLastError(); cap = hipStreamCaptureStatusActive; }
+    CoreLayout L; RCK(core_layout(na, nb, x != nullptr, wcut, &L, !ws && cap == hipStreamCaptureStatusNone));
     if (ws) {
         if (L.bytes > ws_cap) return nttm_fail(NTTM_EINVAL, "the product needs %zu bytes of workspace, above the %zu given", L.bytes, ws_cap);
     } else {
@@ -738,7 +771,23 @@ static int mul_core(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b
     char *hs = (char *)g.hpin + 128 * slot;                       /* pinned: the descriptor, then (at 96) the flags */
 
     if (sync) HIPCK(hipEventRecord(g.ev[0], s));
-    RCK(product_transforms(L.np, L.logn, L.r3, 1, A, Bp, a, na, b, nb, nullptr, s, sync ? g.pev : nullptr));
+    hipStream_t ps[NTTM_MAX_PRIMES];
+    if (L.par) {                                                 /* prime 0 on s, the others forked from it */
+        if (!g.pfev) {
+            HIPCK(hipEventCreateWithFlags(&g.pfev, hipEventDisableTiming));
+            for (int k = 0; k < NTTM_MAX_PRIMES - 1; k++) {
+                HIPCK(hipStreamCreateWithFlags(&g.ps[k], hipStreamNonBlocking));
+                HIPCK(hipEventCreateWithFlags(&g.pjev[k], hipEventDisableTiming));
+            }
+        }
+        HIPCK(hipEventRecord(g.pfev, s));
+        ps[0] = s;
+        for (int i = 1; i < L.np; i++) { ps[i] = g.ps[i - 1]; HIPCK(hipStreamWaitEvent(ps[i], g.pfev, 0)); }
+    }
+    RCK(product_transforms(L.np, L.logn, L.r3, 1, A, Bp, a, na, b, nb, nullptr, s, sync ? g.pev : nullptr, 0, nullptr,
+                           nullptr, L.par ? ps : nullptr, L.n));
+    if (L.par)
+        for (int i = 1; i < L.np; i++) { HIPCK(hipEventRecord(g.pjev[i - 1], ps[i])); HIPCK(hipStreamWaitEvent(s, g.pjev[i - 1], 0)); }
     if (sync) HIPCK(hipEventRecord(g.ev[2], s));
     CrtDesc hd; memset(&hd, 0, sizeof hd);
     hd.pl = A; hd.plane = L.n; hd.ncoef = L.ncoef; hd.x = x; hd.nx = x ? nx : 0; hd.c = c; hd.nc = nc; hd.first_block = 0;
@@ -1577,7 +1626,10 @@ extern "C" int nttm_mul_each_async(nttm_prod *P, size_t N, void *stream)
         int ki[4];
         RCK(prod_kinds(mm, q.a, q.na, q.b, q.nb, q.x, q.nx, q.c, nc, ki, i));
         if (any_host(ki)) return nttm_fail(NTTM_EINVAL, "nttm_mul_each_async: product %zu: host memory (device memory only)", i);
-        if (q.na && q.nb && fits_one(q.na, q.nb, nc)) { CoreLayout L; RCK(core_layout(q.na, q.nb, q.x != nullptr, SIZE_MAX, &L)); ws_max = std::max(ws_max, L.bytes); }
+        if (q.na && q.nb && fits_one(q.na, q.nb, nc)) {                /* (with the concurrent primes' B planes) */
+            CoreLayout L; RCK(core_layout(q.na, q.nb, q.x != nullptr, SIZE_MAX, &L, true));
+            ws_max = std::max(ws_max, L.bytes);
+        }
     }
     const size_t off_ncn = align_up(N * sizeof(CrtDesc), 256);
     RCK(ws_need(ws_max));                                        /* sized once: no reallocation between launches */
