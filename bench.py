@@ -4,8 +4,8 @@
 Default workload: one 2^27 x 2^27-limb product (base-10^18 uint64 limbs drawn by splitmix64 from seeds 0x5EED0003 /
 0x5EED0004, top limb nonzero), transform length 2^28, three primes, operands resident in HBM.  A step is one full
 product c = a b through the C ABI (load, 9 transforms with the fused pointwise product, CRT with the device carry scan).
-Also measured in the same run: the plain 2^20 x 2^20 product (median of 12 calls, exact against GMP, GMP timed on the
-same operands); the batched binary-splitting leaves (4096 products, log-uniform lengths in [64, 65536] limbs) as one
+Also measured in the same run: the plain 2^20 x 2^20 product (median of 12 calls and of 12 replays of it as a
+one-product plan, exact against GMP, GMP timed on the same operands); the batched binary-splitting leaves (4096 products, log-uniform lengths in [64, 65536] limbs) as one
 graph replay, one batched call, one call per product (synchronizing, and launched back to back), and one batched call
 on host-resident operands (PCIe included); one binary-splitting level (256 nodes, each P = P1 Q2 + P2 and
 Q = Q1 Q2 sharing Q2) batched with Q2 shared and unshared; e to 10^9 digits through the whole pipeline (checked
@@ -155,7 +155,8 @@ def cpu_full(nttm, a, b):
 def bench_config1(nttm, reps=12):
     """the plain product: two 2^20-limb operands (seeds 0x5EED0001 / 0x5EED0002), transform 2^21, three primes; the
     median of `reps` timed calls (each returns complete), its fraction of the HBM roofline on the algorithmic bytes
-    352 2^21, the product checked exactly against GMP, and GMP's mpz_mul on the same operands (1 thread)"""
+    352 2^21, the median of `reps` replays of the same product as a one-product batch plan, the product checked exactly
+    against GMP, and GMP's mpz_mul on the same operands (1 thread)"""
     import oracle
     import torch
     n = 1 << 20
@@ -657,6 +658,7 @@ def main():
     ap.add_argument("--cpu-full", action="store_true",
                     help="one-off: GMP on the full operands (conversions, one timed mpz_mul, exact compare); minutes")
     ap.add_argument("--no-config1", action="store_true", help="skip the 2^20 x 2^20 plain product")
+    ap.add_argument("--config1-only", action="store_true", help="only the 2^20 x 2^20 plain product, one JSON object")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only the size sweep: one product per transform length 2^24 .. 2^32 points")
     ap.add_argument("--sweep-max", type=int, default=32, help="largest transform of the sweep, log2 points")
@@ -685,6 +687,10 @@ def main():
     if args.e_only:                                              # the e pipeline alone (for profiling)
         nttm.init(0, tile_log=args.tile_log, pts=args.pts)
         print(json.dumps({"e_pipeline": bench_e(nttm, args.e_digits)}))
+        return
+    if args.config1_only:                                        # the plain product alone (for profiling)
+        nttm.init(0, tile_log=args.tile_log, pts=args.pts)
+        print(json.dumps({"plain_product_2e20": bench_config1(nttm)}))
         return
     if args.sweep_only:
         nttm.init(0, tile_log=args.tile_log, pts=args.pts)
